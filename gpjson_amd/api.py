@@ -1,0 +1,451 @@
+"""Host-side mirror of GPJSON's polyglot surface over the C ABI.
+
+Reference interface (paths relative to gpjson/src/main/java/com/koenv/gpjson/):
+
+* ``Polyglot.eval('gpjson', 'jsonpath')`` returns the library object
+  (GPJSONLanguage.java:27-34, GPJSONLibrary.java:48-54) -> ``GPJSON()``;
+* its member ``query(file, path | [paths], {disable_fallback})``
+  (functions/QueryFunction.java:48-265) -> ``GPJSON.query``;
+* the returned ``Result[query][line][k]`` -> ``str`` (UTF-8 of the span) or
+  ``None`` for the interop null (result/Result.java, ResultArray.java,
+  ResultItem.java:32-52, NullValue.java).
+
+Errors map to the reference's exception classes: GPJSONException (device /
+runtime failures and bad arguments), JSONPathException (parse errors) and
+UnsupportedJSONPathException (paths outside the subset; the reference would
+fall back to jayway json-path on the CPU, which this library does not carry).
+
+Lower-level handles (Context, Buffer, Index) expose the index stages for the
+parity tests the way the reference's KernelTest exposes its stage classes
+(gpjson/src/test/java/com/koenv/gpjson/KernelTest.java:17-45).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _native as N
+
+
+class GPJSONException(RuntimeError):
+    """Runtime failure (J/GPJSONException.java)."""
+
+    def __init__(self, msg: str, code: int = N.GPJSON_ERR_HIP):
+        super().__init__(msg)
+        self.code = code
+
+
+class JSONPathException(GPJSONException):
+    """Malformed JSONPath (J/jsonpath/JSONPathException.java)."""
+
+
+class UnsupportedJSONPathException(JSONPathException):
+    """JSONPath outside the supported subset (J/jsonpath/UnsupportedJSONPathException.java)."""
+
+
+def _check(rc: int) -> None:
+    if rc == N.GPJSON_OK:
+        return
+    msg = N.last_error()
+    if rc == N.GPJSON_ERR_UNSUPPORTED:
+        raise UnsupportedJSONPathException(msg, rc)
+    if rc == N.GPJSON_ERR_PARSE:
+        raise JSONPathException(msg, rc)
+    if rc == N.GPJSON_ERR_INVALID_ARG:
+        raise ValueError(msg)
+    raise GPJSONException(msg, rc)
+
+
+class CompiledPath:
+    __slots__ = ("path", "ir", "max_depth", "num_results")
+
+    def __init__(self, path: str, ir: bytes, max_depth: int, num_results: int):
+        self.path = path
+        self.ir = ir
+        self.max_depth = max_depth
+        self.num_results = num_results
+
+    def __repr__(self) -> str:
+        return f"CompiledPath({self.path!r}, ir={self.ir.hex(' ')}, max_depth={self.max_depth})"
+
+
+def compile_path(path: str) -> CompiledPath:
+    """JSONPathParser.compile (J/jsonpath/JSONPathParser.java:20-30)."""
+    lib = N.load()
+    raw = path.encode("utf-8")
+    n = ctypes.c_size_t(0)
+    d = ctypes.c_int(0)
+    r = ctypes.c_int(0)
+    _check(lib.gpjson_compile(raw, None, 0, ctypes.byref(n), ctypes.byref(d), ctypes.byref(r)))
+    buf = (ctypes.c_uint8 * max(n.value, 1))()
+    _check(lib.gpjson_compile(raw, buf, n.value, ctypes.byref(n), ctypes.byref(d), ctypes.byref(r)))
+    return CompiledPath(path, bytes(buf[: n.value]), d.value, r.value)
+
+
+class Context:
+    """One device + one stream (replaces CUDARuntime, J/gpu/CUDARuntime.java:76-94)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.load()
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_cus(self) -> int:
+        v = ctypes.c_int(0)
+        _check(self._lib.gpjson_ctx_num_cus(self._h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self) -> None:
+        _check(self._lib.gpjson_ctx_synchronize(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.gpjson_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def stage(self, data: Union[bytes, bytearray, memoryview, np.ndarray]) -> "Buffer":
+        """Pinned async H2D copy of host bytes."""
+        arr = np.frombuffer(memoryview(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else \
+            np.ascontiguousarray(data, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        ptr = arr.ctypes.data_as(ctypes.c_void_p) if arr.size else None
+        _check(self._lib.gpjson_buf_from_host(self._h, ptr, arr.size, ctypes.byref(h)))
+        return Buffer(self, h)
+
+    def stage_file(self, path: str) -> "Buffer":
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_buf_from_file(self._h, path.encode(), ctypes.byref(h)))
+        return Buffer(self, h)
+
+    def wrap_device(self, ptr: int, n: int) -> "Buffer":
+        """Wraps n bytes already in this device's memory (e.g. a torch tensor's data_ptr())."""
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_buf_wrap_device(self._h, ctypes.c_void_p(ptr), n, ctypes.byref(h)))
+        return Buffer(self, h)
+
+    def index(self, buf: "Buffer", levels: int, verify: bool = False) -> "Index":
+        h = ctypes.c_void_p()
+        flags = N.GPJSON_BUILD_VERIFY if verify else 0
+        _check(self._lib.gpjson_index_build(self._h, buf.handle, levels, flags, ctypes.byref(h)))
+        return Index(self, h, buf)
+
+    def run_queries(self, idx: "Index", irs: Sequence[bytes], results_per_line: int) -> np.ndarray:
+        """int64[nq][lines][R][2] spans for compiled IRs over one index."""
+        nq = len(irs)
+        bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
+        ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
+        lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
+        lines = idx.lines
+        out = np.empty((nq, lines, results_per_line, 2), dtype=np.int64)
+        _check(self._lib.gpjson_query_run(self._h, idx.handle, ptrs, lens, nq, results_per_line,
+                                          out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def plan(self, buf: "Buffer", irs: Sequence[bytes]) -> "Plan":
+        """Index + queries prepared once over a resident buffer (runs allocate nothing)."""
+        nq = len(irs)
+        bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
+        ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
+        lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_plan_create(self._h, buf.handle, ptrs, lens, nq, ctypes.byref(h)))
+        return Plan(self, h, buf, nq)
+
+    def time_plain(self, buf: "Buffer", irs: Sequence[bytes], iterations: int):
+        """(ms per index+query pass, index ms, query ms) measured with HIP events on the library stream."""
+        nq = len(irs)
+        bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
+        ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
+        lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.gpjson_time_plain(self._h, buf.handle, ptrs, lens, nq, iterations, ctypes.byref(a),
+                                           ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+
+class Plan:
+    """Prepared plain path: one run = index kernel + one interpreter launch, enqueued without waiting."""
+
+    def __init__(self, ctx: Context, h, buf: "Buffer", nq: int):
+        self.ctx, self._h, self.buf, self.nq = ctx, h, buf, nq
+
+    @property
+    def lines(self) -> int:
+        return int(N.load().gpjson_plan_lines(self._h))
+
+    @property
+    def results_per_line(self) -> int:
+        return int(N.load().gpjson_plan_results_per_line(self._h))
+
+    def run(self, timed: bool = False) -> None:
+        _check(N.load().gpjson_plan_run(self._h, 1 if timed else 0))
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(N.load().gpjson_plan_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def spans(self) -> np.ndarray:
+        lib = N.load()
+        out = np.empty((self.nq, self.lines, self.results_per_line, 2), dtype=np.int64)
+        _check(lib.gpjson_plan_spans(self._h, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def free(self) -> None:
+        if self._h:
+            N.load().gpjson_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Buffer:
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def size(self) -> int:
+        return int(N.load().gpjson_buf_size(self._h))
+
+    def free(self) -> None:
+        if self._h:
+            N.load().gpjson_buf_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Index:
+    """Newline / string / leveled (+ escape / quote in verify mode) indexes on the device."""
+
+    def __init__(self, ctx: Context, h, buf: Buffer):
+        self.ctx = ctx
+        self._h = h
+        self.buf = buf  # keeps the input alive
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def lines(self) -> int:
+        return int(N.load().gpjson_index_lines(self._h))
+
+    @property
+    def levels(self) -> int:
+        return int(N.load().gpjson_index_levels(self._h))
+
+    def rebuild(self) -> None:
+        _check(N.load().gpjson_index_rebuild(self._h))
+
+    def export(self, which: int) -> np.ndarray:
+        lib = N.load()
+        size = ctypes.c_int64(0)
+        _check(lib.gpjson_index_export(self._h, which, None, 0, ctypes.byref(size)))
+        dtype = np.int64 if which == N.GPJSON_INDEX_NEWLINE else np.uint64
+        out = np.empty(size.value // 8, dtype=dtype)
+        if size.value:
+            _check(lib.gpjson_index_export(self._h, which, out.ctypes.data_as(ctypes.c_void_p), size.value,
+                                           ctypes.byref(size)))
+        return out
+
+    def newline_index(self) -> np.ndarray:
+        return self.export(N.GPJSON_INDEX_NEWLINE)
+
+    def string_index(self) -> np.ndarray:
+        return self.export(N.GPJSON_INDEX_STRING)
+
+    def leveled_bitmaps(self) -> np.ndarray:
+        return self.export(N.GPJSON_INDEX_LEVELED)
+
+    def escape_index(self) -> np.ndarray:
+        return self.export(N.GPJSON_INDEX_ESCAPE)
+
+    def quote_index(self) -> np.ndarray:
+        return self.export(N.GPJSON_INDEX_QUOTE)
+
+    def free(self) -> None:
+        if self._h:
+            N.load().gpjson_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ResultItem(Sequence):
+    """Results of one query on one line: R entries, each str or None."""
+
+    def __init__(self, res: "Result", q: int, line: int):
+        self._res, self._q, self._line = res, q, line
+
+    def __len__(self) -> int:
+        return self._res.results_per_line
+
+    def __getitem__(self, k):
+        if isinstance(k, slice):
+            return [self[i] for i in range(*k.indices(len(self)))]
+        if k < 0:
+            k += len(self)
+        if not 0 <= k < len(self):
+            raise IndexError(k)
+        return self._res._value(self._q, self._line, k)
+
+
+class ResultArray(Sequence):
+    """One query's results: one ResultItem per line."""
+
+    def __init__(self, res: "Result", q: int):
+        self._res, self._q = res, q
+
+    def __len__(self) -> int:
+        return self._res.number_of_lines
+
+    def __getitem__(self, line):
+        if isinstance(line, slice):
+            return [self[i] for i in range(*line.indices(len(self)))]
+        if line < 0:
+            line += len(self)
+        if not 0 <= line < len(self):
+            raise IndexError(line)
+        return ResultItem(self._res, self._q, line)
+
+    @property
+    def spans(self) -> np.ndarray:
+        return self._res.spans(self._q)
+
+
+class Result(Sequence):
+    """Result[query][line][k] -> str | None (J/result/Result.java:23-72)."""
+
+    def __init__(self, h):
+        self._lib = N.load()
+        self._h = h
+        self.number_of_queries = int(self._lib.gpjson_result_queries(h))
+        self.number_of_lines = int(self._lib.gpjson_result_lines(h))
+        self.results_per_line = int(self._lib.gpjson_result_per_line(h))
+
+    def __len__(self) -> int:
+        return self.number_of_queries
+
+    def __getitem__(self, q):
+        if isinstance(q, slice):
+            return [self[i] for i in range(*q.indices(len(self)))]
+        if q < 0:
+            q += len(self)
+        if not 0 <= q < len(self):
+            raise IndexError(q)
+        return ResultArray(self, q)
+
+    def spans(self, q: int) -> np.ndarray:
+        p = self._lib.gpjson_result_spans(self._h, q)
+        n = self.number_of_lines * self.results_per_line * 2
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.number_of_lines, self.results_per_line, 2).copy()
+
+    def _value(self, q: int, line: int, k: int) -> Optional[str]:
+        ptr = ctypes.c_void_p()
+        n = ctypes.c_int64(0)
+        rc = self._lib.gpjson_result_value(self._h, q, line, k, ctypes.byref(ptr), ctypes.byref(n))
+        if rc < 0:
+            raise IndexError(N.last_error())
+        if rc == 0:
+            return None
+        return ctypes.string_at(ptr.value, n.value).decode("utf-8", "replace") if n.value else ""
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.gpjson_result_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GPJSON:
+    """The library object of ``Polyglot.eval('gpjson', 'jsonpath')`` (J/GPJSONLibrary.java:48-98)."""
+
+    MEMBERS = ("query",)
+
+    def __init__(self, device: int = 0):
+        self._ctx = Context(device)
+
+    @property
+    def context(self) -> Context:
+        return self._ctx
+
+    def query(self, file: str, queries: Union[str, Sequence[str]], options: Optional[dict] = None) -> Result:
+        """query(file, path | [paths], {disable_fallback}) (J/functions/QueryFunction.java:48-265)."""
+        if isinstance(queries, str):
+            paths: List[str] = [queries]
+        elif isinstance(queries, (list, tuple)):
+            if len(queries) > N.GPJSON_MAX_QUERIES:
+                raise GPJSONException("Maximum size of queries is 255", N.GPJSON_ERR_INVALID_ARG)
+            paths = []
+            for q in queries:
+                if not isinstance(q, str):
+                    raise TypeError("expected queries to be a string or string array")
+                paths.append(q)
+        else:
+            raise TypeError("expected queries to be a string or string array")
+        if len(paths) < 1:
+            raise TypeError("expected queries to contain at least one query")
+        flags = 0
+        if options is not None:
+            if not isinstance(options, dict):
+                raise GPJSONException("Invalid options", N.GPJSON_ERR_INVALID_ARG)
+            if options.get("disable_fallback"):
+                flags |= N.GPJSON_QUERY_DISABLE_FALLBACK
+        arr = (ctypes.c_char_p * len(paths))(*[p.encode("utf-8") for p in paths])
+        h = ctypes.c_void_p()
+        _check(self._ctx._lib.gpjson_query_file(self._ctx.handle, str(file).encode(), arr, len(paths), flags,
+                                                ctypes.byref(h)))
+        return Result(h)
+
+    def close(self) -> None:
+        self._ctx.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def eval(language: str = "gpjson", source: str = "jsonpath") -> GPJSON:  # noqa: A001 - mirrors Polyglot.eval
+    """Polyglot.eval('gpjson', 'jsonpath'); only that source text is accepted (J/GPJSONLanguage.java:29-33)."""
+    if language != "gpjson" or source.strip() != "jsonpath":
+        raise GPJSONException("Only 'jsonpath' is supported as source", N.GPJSON_ERR_INVALID_ARG)
+    return GPJSON()

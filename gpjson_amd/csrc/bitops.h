@@ -1,0 +1,205 @@
+// bitops.h -- word-level (64 input bytes per 64-bit word) bit arithmetic of the
+// structural index.  Header-only, compiled both by hipcc for gfx950 and by g++
+// for the CPU harness that checks these functions exhaustively.
+//
+// Bit i%64 of word i/64 stands for input byte i (LSB first), as in the
+// reference's bitmaps (K/create_combined_escape_newline_index.cu:34-36,
+// K/create_string_index.cu:16-30; K/ = gpjson/src/main/resources/com/koenv/
+// gpjson/kernels/).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GJ_HD __host__ __device__ __forceinline__
+#else
+#define GJ_HD static inline
+#endif
+
+namespace gj {
+
+constexpr uint64_t kEven = 0x5555555555555555ull;  // bits at even positions
+constexpr uint64_t kOdd = 0xAAAAAAAAAAAAAAAAull;
+
+GJ_HD int popc64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __popcll(x);
+#else
+  return __builtin_popcountll(x);
+#endif
+}
+
+GJ_HD int ctz64(uint64_t x) {  // x != 0
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ffsll((unsigned long long)x) - 1;
+#else
+  return __builtin_ctzll(x);
+#endif
+}
+
+GJ_HD int clz64(uint64_t x) {  // x != 0
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __clzll(x);
+#else
+  return __builtin_clzll(x);
+#endif
+}
+
+// Escape bitmap of one word for escape carry-in p (reference definition
+// T/stages/StringIndexTest.java:109-127: E[i] = c_i, c_{i+1} = F[i]=='\\' ?
+// 1-c_i : 0).  Backslash runs are resolved with one add per run parity class:
+// a run starting at bit s (not continued from the previous word) marks the
+// positions s+1, s+3, ... up to and including the byte after the run; a run
+// continued from the previous word with carry 1 marks s, s+2, ... instead.
+GJ_HD uint64_t escape_word(uint64_t B, uint32_t p) {
+  uint64_t starts = B & ~((B << 1) | (uint64_t)p);
+  uint64_t es = starts & kEven, os = starts & kOdd;
+  uint64_t E = (((B + es) ^ B) & kOdd) | (((B + os) ^ B) & kEven);
+  if (p) E |= ((B + 1) ^ B) & kEven;
+  return E;
+}
+
+// Escape carry into the next word: an all-backslash word passes the carry
+// through (64 is even); otherwise it is the parity of the trailing run.
+GJ_HD uint32_t escape_carry_out(uint64_t B, uint32_t p) {
+  if (B == ~0ull) return p;
+  uint64_t nb = ~B;
+  int run = clz64(nb);  // leading (high) ones of B
+  return (uint32_t)(run & 1);
+}
+
+// Inclusive prefix XOR (bit i = XOR of bits 0..i), done on 32-bit halves.
+GJ_HD uint64_t prefix_xor(uint64_t q) {
+  uint32_t lo = (uint32_t)q, hi = (uint32_t)(q >> 32);
+  lo ^= lo << 1; lo ^= lo << 2; lo ^= lo << 4; lo ^= lo << 8; lo ^= lo << 16;
+  hi ^= hi << 1; hi ^= hi << 2; hi ^= hi << 4; hi ^= hi << 8; hi ^= hi << 16;
+  hi ^= (uint32_t)((int32_t)lo >> 31);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bytes of one word classified into 64-bit masks.
+struct WordClass {
+  uint64_t B;   // '\\'
+  uint64_t Q;   // '"' (escaped or not)
+  uint64_t NL;  // '\n'
+  uint64_t O;   // '{' '['
+  uint64_t C;   // '}' ']'
+  uint64_t P;   // ':' ','
+};
+
+// Per byte of t: bit 7 set iff the byte's low seven bits equal k (k < 0x80).
+// lo7 = t & 0x7f7f7f7f.  Byte-wise 0x80 - (x ^ k) with no cross-byte borrow.
+GJ_HD uint32_t eq7(uint32_t lo7, uint32_t k4) { return (lo7 ^ ~k4) - 0x7F7F7F7Fu; }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
+  return __builtin_amdgcn_perm(s0, s1, sel);
+}
+#else
+// v_perm_b32: result byte i = byte sel_i of the 8-byte value {s0:s1} (0..3 from s1).
+GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
+  uint64_t v = ((uint64_t)s0 << 32) | s1;
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) {
+    uint32_t s = (sel >> (8 * i)) & 0xFF;
+    r |= (uint32_t)((v >> (8 * s)) & 0xFF) << (8 * i);
+  }
+  return r;
+}
+#endif
+
+// Classifies 32 bytes (d[0..7], byte b of d[i] = position 4i+b) into 32-bit
+// masks.  The bytes are first transposed so that dword j holds positions j,
+// j+8, j+16, j+24; a class's per-byte marker (bit 7) then lands on its natural
+// bit position with one right shift by 7-j.
+GJ_HD void classify32(const uint32_t d[8], uint32_t& B, uint32_t& Q, uint32_t& NL, uint32_t& O,
+                      uint32_t& C, uint32_t& P) {
+  uint32_t t[8];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    uint32_t a0 = d[h], a1 = d[h + 2], a2 = d[h + 4], a3 = d[h + 6];
+    uint32_t u0 = perm_b32(a1, a0, 0x05010400u);
+    uint32_t u1 = perm_b32(a1, a0, 0x07030602u);
+    uint32_t u2 = perm_b32(a3, a2, 0x05010400u);
+    uint32_t u3 = perm_b32(a3, a2, 0x07030602u);
+    t[4 * h + 0] = perm_b32(u2, u0, 0x05040100u);
+    t[4 * h + 1] = perm_b32(u2, u0, 0x07060302u);
+    t[4 * h + 2] = perm_b32(u3, u1, 0x05040100u);
+    t[4 * h + 3] = perm_b32(u3, u1, 0x07060302u);
+  }
+  uint32_t b = 0, q = 0, nl = 0, o = 0, c = 0, p = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t x = t[j];
+    uint32_t lo7 = x & 0x7F7F7F7Fu;
+    uint32_t lo7c = lo7 | 0x20202020u;
+    uint32_t nv = (~x & 0x80808080u) >> (7 - j);
+    b |= (eq7(lo7, 0x5C5C5C5Cu) >> (7 - j)) & nv;
+    q |= (eq7(lo7, 0x22222222u) >> (7 - j)) & nv;
+    nl |= (eq7(lo7, 0x0A0A0A0Au) >> (7 - j)) & nv;
+    o |= (eq7(lo7c, 0x7B7B7B7Bu) >> (7 - j)) & nv;
+    c |= (eq7(lo7c, 0x7D7D7D7Du) >> (7 - j)) & nv;
+    p |= ((eq7(lo7, 0x3A3A3A3Au) | eq7(lo7, 0x2C2C2C2Cu)) >> (7 - j)) & nv;
+  }
+  B = b; Q = q; NL = nl; O = o; C = c; P = p;
+}
+
+// Classifies one 64-byte word given as 16 little-endian dwords.
+GJ_HD WordClass classify64(const uint32_t d[16]) {
+  uint32_t b0, q0, n0, o0, c0, p0, b1, q1, n1, o1, c1, p1;
+  classify32(d, b0, q0, n0, o0, c0, p0);
+  classify32(d + 8, b1, q1, n1, o1, c1, p1);
+  WordClass w;
+  w.B = ((uint64_t)b1 << 32) | b0;
+  w.Q = ((uint64_t)q1 << 32) | q0;
+  w.NL = ((uint64_t)n1 << 32) | n0;
+  w.O = ((uint64_t)o1 << 32) | o0;
+  w.C = ((uint64_t)c1 << 32) | c0;
+  w.P = ((uint64_t)p1 << 32) | p0;
+  return w;
+}
+
+// Leveled-bitmap bits of one word for levels [lo, lo+NLV): depth d (the depth
+// before the word's first byte, -1 at file start) follows the reference rule
+// (J/sequential/Sequential.java:84-100): '{' '[' increment then mark, '}' ']'
+// mark then decrement, ':' ',' mark at the current depth.  O, C, P are already
+// restricted to bytes outside strings.  Returns the depth after the word.
+template <int NLV>
+GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L,
+                           uint64_t out[NLV]) {
+#pragma unroll
+  for (int k = 0; k < NLV; k++) out[k] = 0;
+  uint64_t T = O | C;
+  uint64_t rem = ~0ull;  // positions not yet assigned
+  while (T) {
+    int t = ctz64(T);
+    uint64_t bit = 1ull << t;
+    uint64_t before = rem & (bit - 1);  // separators strictly before t
+    uint64_t seps = P & before;
+    int is_open = (O & bit) != 0;
+    int64_t lvl_open = d + 1;
+    // separators before t and a closing bracket at t sit at depth d; an
+    // opening bracket at t sits at depth d + 1.
+    uint64_t at_d = seps | (is_open ? 0 : bit);
+    uint64_t at_d1 = is_open ? bit : 0;
+#pragma unroll
+    for (int k = 0; k < NLV; k++) {
+      int64_t lv = lo + k;
+      if (lv < L) {
+        out[k] |= (lv == d) ? at_d : 0;
+        out[k] |= (lv == lvl_open) ? at_d1 : 0;
+      }
+    }
+    d += is_open ? 1 : -1;
+    rem &= ~((bit - 1) | bit);
+    T &= T - 1;
+  }
+  uint64_t seps = P & rem;
+#pragma unroll
+  for (int k = 0; k < NLV; k++) {
+    int64_t lv = lo + k;
+    if (lv < L) out[k] |= (lv == d) ? seps : 0;
+  }
+  return d;
+}
+
+}  // namespace gj

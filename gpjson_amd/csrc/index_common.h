@@ -1,0 +1,49 @@
+// index_common.h -- host/device shared definitions of the structural index.
+#pragma once
+#include <stdint.h>
+
+namespace gj {
+
+// Levels the interpreter supports (reference: 16, J/stages/LeveledBitmapsIndex.java:16).
+constexpr int kMaxLevels = 64;
+
+// Per-tile lookback state: self-validating 64-bit words (bit 63 = written), so a
+// reader never needs a fence between a flag and its payload.  One 64-byte line
+// per tile.
+struct alignas(64) TileState {
+  unsigned long long agg0;  // valid | allB(62) | carry(61) | q0(60) | q1(59) | nl u32 (0..31)
+  unsigned long long agg1;  // valid | d0 for carry-in 0 (31 bits) | total open-close (31 bits)
+  unsigned long long agg2;  // valid | d0 for carry-in 1 (31 bits)
+  unsigned long long inc0;  // valid | p(62) | s(61) | depth (signed 48 bits)
+  unsigned long long inc1;  // valid | lines (63 bits)
+  unsigned long long pad[3];
+};
+
+// One file of an index launch.  The plain path launches with one file.
+struct FileJob {
+  const uint8_t* F;     // input bytes
+  int64_t N;            // bytes
+  int64_t W;            // words = ceil(N/64)
+  int64_t Wp;           // level stride (W rounded up to 8)
+  uint64_t* S;          // string index, W words
+  uint64_t* LB;         // leveled bitmaps, L * Wp words
+  int64_t* NL;          // newline index, capacity nl_cap
+  int64_t nl_cap;
+  uint64_t* E;          // escape index (verify mode, may be null)
+  uint64_t* Q;          // quote index (verify mode, may be null)
+  int64_t tile0;        // first global tile id of this file
+  int64_t ntiles;
+  int L;                // levels to write
+  int pad_;
+};
+
+struct IndexLaunch {
+  const FileJob* jobs;     // device array, or null: the single job `one`
+  FileJob one;
+  int64_t njobs;
+  int64_t total_tiles;
+  TileState* states;       // total_tiles entries, zeroed before launch
+  unsigned int* ticket;    // zeroed before launch
+};
+
+}  // namespace gj

@@ -1,0 +1,499 @@
+// index_kernel.hip -- fused structural index for gfx950.
+//
+// One pass over the input builds what the reference builds with six kernels
+// and three host round trips (J/stages/CombinedIndex.java:28-151,
+// J/stages/LeveledBitmapsIndex.java:40-103; kernels K/create_combined_*.cu,
+// K/create_quote_index.cu, K/create_string_index.cu, K/create_leveled_*.cu):
+//   newline index   int64[lines]   ([0] = 0, then every '\n')
+//   string index    uint64[W]      (in-string bits)
+//   leveled bitmaps uint64[L][Wp]  (structural chars per depth level)
+//   escape / quote  uint64[W]      (only in verify mode)
+//
+// Work decomposition.  A tile is BLOCK*WPT 64-byte words.  Tiles are claimed in
+// order through an atomic ticket, so every tile's predecessors are already
+// running.  The block stages its tile into LDS with coalesced 16-byte loads,
+// then each thread classifies its WPT consecutive words with byte-parallel
+// (SWAR) compares into 64-bit masks -- one lane owns one word, so all further
+// bit arithmetic is word-parallel across the wavefront.
+//
+// Cross-word state is a transfer function, not a value: a word's escape carry
+// out is fixed unless the word is all backslashes, and given the escape carry,
+// its effect on (in-string, depth) is (q, d0, tot): string parity flips by q,
+// depth moves by d0 if the word starts outside a string and by tot-d0 if it
+// starts inside one.  These compose associatively, so the tile computes its
+// aggregate with a block scan, publishes it, and resolves its carry-in with a
+// decoupled lookback over earlier tiles' published aggregates / inclusive
+// prefixes.  The only dependence of a tile on the escape carry coming from the
+// previous tile is through its first word that is not all backslashes (a quote
+// right after a leading backslash run flips), so a tile publishes its
+// aggregate for both carry values.
+#include <hip/hip_runtime.h>
+#include "bitops.h"
+#include "index_common.h"
+
+namespace gj {
+
+namespace {
+
+constexpr unsigned long long kValid = 1ull << 63;
+
+struct M4 {
+  int q;    // string-parity flip (0/1)
+  int d0;   // depth change when entering outside a string
+  int tot;  // opens minus closes, strings ignored
+  int nl;   // newlines
+};
+
+__device__ __forceinline__ M4 m_id() { return M4{0, 0, 0, 0}; }
+
+__device__ __forceinline__ M4 m_cat(const M4& x, const M4& y) {
+  return M4{x.q ^ y.q, x.d0 + (x.q ? (y.tot - y.d0) : y.d0), x.tot + y.tot, x.nl + y.nl};
+}
+
+// y such that m_cat(x, y) == z
+__device__ __forceinline__ M4 m_left_div(const M4& x, const M4& z) {
+  M4 y;
+  y.q = x.q ^ z.q;
+  y.tot = z.tot - x.tot;
+  y.nl = z.nl - x.nl;
+  int dd = z.d0 - x.d0;
+  y.d0 = x.q ? (y.tot - dd) : dd;
+  return y;
+}
+
+// The same word evaluated with the opposite escape carry at its leading
+// backslash run, when the byte after that run is a quote.
+__device__ __forceinline__ M4 m_flip(const M4& x) { return M4{x.q ^ 1, x.tot - x.d0, x.tot, x.nl}; }
+
+__device__ __forceinline__ M4 shfl_up_m(const M4& v, int off) {
+  return M4{__shfl_up(v.q, off), __shfl_up(v.d0, off), __shfl_up(v.tot, off), __shfl_up(v.nl, off)};
+}
+
+// Carry state at a position.
+struct Carry {
+  int p;          // escape carry
+  int s;          // inside a string
+  int64_t d;      // depth
+  int64_t lines;  // newline-index slot of the next '\n'
+};
+
+// Tile aggregate as a function of the incoming escape carry.
+struct TileAgg {
+  int esc;  // 0: passes the carry through, 2/3: sets it to 0/1
+  M4 m[2];
+};
+
+__device__ __forceinline__ Carry apply_agg(const TileAgg& a, const Carry& c) {
+  const M4& m = a.m[c.p & 1];
+  Carry r;
+  r.p = a.esc ? (a.esc - 2) : c.p;
+  r.s = c.s ^ m.q;
+  r.d = c.d + (c.s ? (int64_t)(m.tot - m.d0) : (int64_t)m.d0);
+  r.lines = c.lines + m.nl;
+  return r;
+}
+
+__device__ __forceinline__ TileAgg agg_cat(const TileAgg& x, const TileAgg& y) {
+  TileAgg z;
+  z.esc = y.esc ? y.esc : x.esc;
+#pragma unroll
+  for (int P = 0; P < 2; P++) {
+    int px = x.esc ? (x.esc - 2) : P;
+    z.m[P] = m_cat(x.m[P], y.m[px]);
+  }
+  return z;
+}
+
+__device__ __forceinline__ int64_t sext(unsigned long long v, int bits) {
+  return ((int64_t)(v << (64 - bits))) >> (64 - bits);
+}
+
+__device__ __forceinline__ unsigned long long ld_state(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_state(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void publish_agg(TileState* st, const TileAgg& a, int nl) {
+  unsigned long long w0 = kValid | ((unsigned long long)(a.esc == 0) << 62) |
+                          ((unsigned long long)(a.esc == 3) << 61) |
+                          ((unsigned long long)(a.m[0].q & 1) << 60) |
+                          ((unsigned long long)(a.m[1].q & 1) << 59) | (unsigned)nl;
+  unsigned long long w1 = kValid | ((unsigned long long)(a.m[0].d0 & 0x7FFFFFFF)) |
+                          ((unsigned long long)(a.m[0].tot & 0x7FFFFFFF) << 31);
+  unsigned long long w2 = kValid | ((unsigned long long)(a.m[1].d0 & 0x7FFFFFFF));
+  st_state(&st->agg1, w1);
+  st_state(&st->agg2, w2);
+  st_state(&st->agg0, w0);
+}
+
+__device__ void publish_inc(TileState* st, const Carry& c) {
+  unsigned long long w0 = kValid | ((unsigned long long)(c.p & 1) << 62) |
+                          ((unsigned long long)(c.s & 1) << 61) |
+                          ((unsigned long long)c.d & 0xFFFFFFFFFFFFull);
+  unsigned long long w1 = kValid | ((unsigned long long)c.lines & ~kValid);
+  st_state(&st->inc0, w0);
+  st_state(&st->inc1, w1);
+}
+
+// Exclusive carry of local tile lt of a file whose tiles start at st0.
+__device__ Carry lookback(TileState* st0, int64_t lt) {
+  TileAgg acc;
+  acc.esc = 0;
+  acc.m[0] = m_id();
+  acc.m[1] = m_id();
+  int64_t k = lt - 1;
+  for (;;) {
+    TileState* s = st0 + k;
+    unsigned long long i0 = ld_state(&s->inc0), i1 = ld_state(&s->inc1);
+    if ((i0 & kValid) && (i1 & kValid)) {
+      Carry c;
+      c.p = (int)((i0 >> 62) & 1);
+      c.s = (int)((i0 >> 61) & 1);
+      c.d = sext(i0 & 0xFFFFFFFFFFFFull, 48);
+      c.lines = (int64_t)(i1 & ~kValid);
+      return apply_agg(acc, c);
+    }
+    unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
+    if ((a0 & kValid) && (a1 & kValid) && (a2 & kValid)) {
+      TileAgg a;
+      int allb = (int)((a0 >> 62) & 1);
+      a.esc = allb ? 0 : (2 + (int)((a0 >> 61) & 1));
+      int nl = (int)(a0 & 0xFFFFFFFFu);
+      int tot = (int)sext((a1 >> 31) & 0x7FFFFFFF, 31);
+      a.m[0] = M4{(int)((a0 >> 60) & 1), (int)sext(a1 & 0x7FFFFFFF, 31), tot, nl};
+      a.m[1] = M4{(int)((a0 >> 59) & 1), (int)sext(a2 & 0x7FFFFFFF, 31), tot, nl};
+      acc = agg_cat(a, acc);
+      k--;
+      continue;  // k >= 0: tile 0 of a file always publishes its inclusive prefix
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+}  // namespace
+
+template <int BLOCK, int WPT, int NLV, bool VERIFY>
+__global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
+  constexpr int CH = WPT * 4;  // 16-byte chunks per thread
+  constexpr int TILE_CHUNKS = BLOCK * CH;
+  constexpr int64_t TILE_BYTES = (int64_t)TILE_CHUNKS * 16;
+  constexpr int NW = BLOCK / 64;
+
+  __shared__ uint4 tile[TILE_CHUNKS];
+  __shared__ unsigned s_ticket;
+  __shared__ int s_job;
+  __shared__ int s_wave_esc[NW];
+  __shared__ M4 s_wave_m[NW];
+  __shared__ int s_w0;        // tile word index of the first word that is not all '\\'
+  __shared__ int s_w0_flip;
+  __shared__ M4 s_w0_m;
+  __shared__ Carry s_carry;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+
+  for (;;) {
+    if (tid == 0) {
+      s_ticket = atomicAdd(launch.ticket, 1u);
+      s_w0 = -1;
+      s_w0_flip = 0;
+      unsigned t = s_ticket;
+      int lo = 0, hi = (int)launch.njobs - 1;
+      if (t < launch.total_tiles && launch.jobs) {
+        while (lo < hi) {  // last job with tile0 <= t
+          int mid = (lo + hi + 1) >> 1;
+          if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
+        }
+      }
+      s_job = lo;
+    }
+    __syncthreads();
+    const unsigned ticket = s_ticket;
+    if (ticket >= launch.total_tiles) break;
+    const FileJob J = launch.jobs ? launch.jobs[s_job] : launch.one;
+    const int64_t lt = (int64_t)ticket - J.tile0;
+    const int64_t byte0 = lt * TILE_BYTES;
+    const int64_t word0 = lt * (int64_t)(BLOCK * WPT);
+
+    // ---- stage the tile into LDS (coalesced 16-byte loads, swizzled slots) ----
+    for (int c = tid; c < TILE_CHUNKS; c += BLOCK) {
+      int64_t off = byte0 + 16 * (int64_t)c;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (off + 16 <= J.N) {
+        v = *reinterpret_cast<const uint4*>(J.F + off);
+      } else if (off < J.N) {
+        uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+          uint32_t x = (off + b < J.N) ? ((uint32_t)J.F[off + b] << (8 * (b & 3))) : 0u;
+          if (b < 4) w0 |= x; else if (b < 8) w1 |= x; else if (b < 12) w2 |= x; else w3 |= x;
+        }
+        v = make_uint4(w0, w1, w2, w3);
+      }
+      int th = c / CH, j = c % CH;
+      tile[th * CH + (j ^ (th & (CH - 1)))] = v;
+    }
+    __syncthreads();
+
+    // ---- classify this thread's words ----
+    WordClass wc[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+      uint32_t d[16];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        uint4 v = tile[tid * CH + ((4 * k + q) ^ (tid & (CH - 1)))];
+        d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
+      }
+      wc[k] = classify64(d);
+    }
+
+    // ---- escape carry: "last word that is not all backslashes" scan ----
+    int code[WPT];  // 0: all backslashes, else 2 + carry out
+    int tex = 0;
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+      code[k] = (wc[k].B == ~0ull) ? 0 : (2 + (int)escape_carry_out(wc[k].B, 0));
+      if (code[k]) tex = code[k];
+    }
+    const unsigned long long nzm = __ballot(tex != 0);
+    const unsigned long long below = nzm & ((1ull << lane) - 1ull);
+    int ent = __shfl(tex, below ? (63 - __clzll(below)) : 0);
+    if (!below) ent = 0;
+    const int wlast = __shfl(tex, nzm ? (63 - __clzll(nzm)) : 0);
+    if (lane == 0) s_wave_esc[wave] = nzm ? wlast : 0;
+    __syncthreads();
+    int tile_esc = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      int e = s_wave_esc[w];
+      if (w < wave && e && !below) ent = e;
+      if (e) tile_esc = e;
+    }
+
+    // ---- per-word transfer functions (escape carry assumed 0 where it comes from P) ----
+    int pcode[WPT];
+    M4 wm[WPT];
+    {
+      int run = ent;
+#pragma unroll
+      for (int k = 0; k < WPT; k++) {
+        pcode[k] = run;
+        int p = run ? (run - 2) : 0;
+        const WordClass& w = wc[k];
+        uint64_t E = escape_word(w.B, (uint32_t)p);
+        uint64_t Qe = w.Q & ~E;
+        uint64_t S0 = prefix_xor(Qe);
+        int opens = popc64(w.O), closes = popc64(w.C);
+        M4 m;
+        m.q = popc64(Qe) & 1;
+        m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
+        m.tot = opens - closes;
+        m.nl = popc64(w.NL);
+        wm[k] = m;
+        if (run == 0 && code[k] != 0) {
+          // first word of the tile that is not all backslashes
+          int r = ctz64(~w.B);
+          s_w0 = tid * WPT + k;
+          s_w0_flip = (int)((w.Q >> r) & 1);
+          s_w0_m = m;
+        }
+        if (code[k]) run = code[k];
+      }
+    }
+
+    // ---- block scan of the word transfer functions ----
+    M4 tin = m_id();
+#pragma unroll
+    for (int k = 0; k < WPT; k++) tin = m_cat(tin, wm[k]);
+    M4 inc = tin;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      M4 y = shfl_up_m(inc, off);
+      if (lane >= off) inc = m_cat(y, inc);
+    }
+    M4 lex = shfl_up_m(inc, 1);
+    if (lane == 0) lex = m_id();
+    if (lane == 63) s_wave_m[wave] = inc;
+    __syncthreads();
+    M4 wpre = m_id(), mtile = m_id();
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      if (w < wave) wpre = m_cat(wpre, s_wave_m[w]);
+      mtile = m_cat(mtile, s_wave_m[w]);
+    }
+    M4 texcl = m_cat(wpre, lex);
+
+    // ---- publish the tile aggregate and resolve the carry-in ----
+    if (tid == 0) {
+      TileState* st0 = launch.states + J.tile0;
+      Carry cin;
+      if (lt == 0) {
+        cin = Carry{0, 0, -1, 1};
+        if (J.nl_cap > 0) J.NL[0] = 0;
+      } else {
+        TileAgg a;
+        a.esc = tile_esc;
+        a.m[0] = mtile;
+        a.m[1] = mtile;
+        if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
+        publish_agg(st0 + lt, a, mtile.nl);
+        cin = lookback(st0, lt);
+      }
+      TileAgg a;
+      a.esc = tile_esc;
+      a.m[0] = mtile;
+      a.m[1] = mtile;
+      if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
+      publish_inc(st0 + lt, apply_agg(a, cin));
+      s_carry = cin;
+    }
+    __syncthreads();
+    const Carry X = s_carry;
+    const int w0 = s_w0;
+    const bool fix = (X.p == 1) && w0 >= 0 && s_w0_flip;
+    const M4 w0m = s_w0_m;
+
+    // ---- emit the indexes ----
+    M4 run = texcl;
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+      const int tw = tid * WPT + k;
+      const int64_t w = word0 + tw;
+      M4 mx = run;
+      if (fix && tw > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
+      run = m_cat(run, wm[k]);
+      if (w >= J.W) continue;
+      const int p = pcode[k] ? (pcode[k] - 2) : X.p;
+      const int s = X.s ^ mx.q;
+      const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
+      const int64_t line = X.lines + mx.nl;
+      const WordClass& wd = wc[k];
+      uint64_t valid = ~0ull;
+      const int64_t rem = J.N - w * 64;
+      if (rem < 64) valid = (1ull << rem) - 1ull;
+      uint64_t E = escape_word(wd.B, (uint32_t)p);
+      uint64_t Qe = wd.Q & ~E;
+      uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
+      J.S[w] = S;
+      if (VERIFY) {
+        if (J.E) J.E[w] = E & valid;
+        if (J.Q) J.Q[w] = Qe & valid;
+      }
+      const uint64_t O = wd.O & ~S, C = wd.C & ~S, P = wd.P & ~S;
+      const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
+      for (int lo = 0; lo < J.L; lo += NLV) {
+        uint64_t out[NLV];
+        if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
+#pragma unroll
+          for (int i = 0; i < NLV; i++) out[i] = 0;
+        } else {
+          leveled_word<NLV>(O, C, P, d, lo, J.L, out);
+        }
+#pragma unroll
+        for (int i = 0; i < NLV; i++)
+          if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+      }
+      uint64_t nlm = wd.NL;
+      int64_t idx = line;
+      while (nlm) {
+        int b = ctz64(nlm);
+        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
+        idx++;
+        nlm &= nlm - 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Zeroes the lookback states and the ticket (a kernel, not hipMemsetAsync, so
+// the index pipeline can be captured into a graph).
+__global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(states);
+  for (int64_t k = i; k < n * 8; k += stride) p[k] = 0;
+  if (i == 0) *ticket = 0;
+}
+
+// Reads the line count of each file from its last tile's inclusive prefix.
+__global__ void index_lines_kernel(const FileJob* jobs, FileJob one, int64_t njobs, const TileState* states,
+                                   int64_t* lines_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= njobs) return;
+  const FileJob& J = jobs ? jobs[i] : one;
+  if (J.ntiles == 0) { lines_out[i] = 1; return; }
+  unsigned long long v = states[J.tile0 + J.ntiles - 1].inc1;
+  lines_out[i] = (int64_t)(v & ~kValid);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers
+// ---------------------------------------------------------------------------
+
+struct IndexConfig {
+  int block;
+  int wpt;
+  int nlv;
+};
+
+template <int BLOCK, int WPT, int NLV>
+static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
+  if (verify)
+    hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
+  else
+    hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, false>), dim3(grid), dim3(BLOCK), 0, stream, L);
+  return hipGetLastError();
+}
+
+template <int BLOCK, int WPT>
+static hipError_t launch_nlv(const IndexLaunch& L, int nlv, bool verify, int grid, hipStream_t s) {
+  switch (nlv) {
+    case 2: return launch_cfg<BLOCK, WPT, 2>(L, verify, grid, s);
+    case 4: return launch_cfg<BLOCK, WPT, 4>(L, verify, grid, s);
+    default: return launch_cfg<BLOCK, WPT, 8>(L, verify, grid, s);
+  }
+}
+
+int64_t index_tile_bytes(int block, int wpt) { return (int64_t)block * wpt * 64; }
+
+bool index_config_supported(int block, int wpt) {
+  return (block == 256 && (wpt == 1 || wpt == 2 || wpt == 4)) || (block == 512 && (wpt == 1 || wpt == 2)) ||
+         (block == 128 && (wpt == 2 || wpt == 4));
+}
+
+hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
+                        hipStream_t stream) {
+  if (block == 256 && wpt == 1) return launch_nlv<256, 1>(L, nlv, verify, grid, stream);
+  if (block == 256 && wpt == 2) return launch_nlv<256, 2>(L, nlv, verify, grid, stream);
+  if (block == 256 && wpt == 4) return launch_nlv<256, 4>(L, nlv, verify, grid, stream);
+  if (block == 512 && wpt == 1) return launch_nlv<512, 1>(L, nlv, verify, grid, stream);
+  if (block == 512 && wpt == 2) return launch_nlv<512, 2>(L, nlv, verify, grid, stream);
+  if (block == 128 && wpt == 2) return launch_nlv<128, 2>(L, nlv, verify, grid, stream);
+  if (block == 128 && wpt == 4) return launch_nlv<128, 4>(L, nlv, verify, grid, stream);
+  return hipErrorInvalidValue;
+}
+
+hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream) {
+  int64_t n8 = ntiles * 8;
+  int grid = (int)((n8 + 255) / 256);
+  if (grid < 1) grid = 1;
+  if (grid > 4096) grid = 4096;
+  hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket);
+  return hipGetLastError();
+}
+
+hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream) {
+  int grid = (int)((L.njobs + 255) / 256);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(index_lines_kernel, dim3(grid), dim3(256), 0, stream, L.jobs, L.one, L.njobs, L.states,
+                     lines_out);
+  return hipGetLastError();
+}
+
+}  // namespace gj

@@ -1,0 +1,49 @@
+// query_common.h -- host/device shared definitions of the per-line interpreter.
+#pragma once
+#include <stdint.h>
+
+namespace gj {
+
+// IR opcodes: ordinal of the reference's Opcode enum
+// (J/jsonpath/Opcode.java:3-11, K/find_value.cu:4-10).
+enum Opcode : uint8_t {
+  kEnd = 0,
+  kStoreResult = 1,
+  kMoveUp = 2,
+  kMoveDown = 3,
+  kMoveToKey = 4,
+  kMoveToIndex = 5,
+  kExpressionStringEquals = 6,
+};
+
+// One (file, query) pair of an interpreter launch.
+struct QueryJob {
+  const uint8_t* F;
+  int64_t N;
+  const int64_t* NL;  // newline index
+  int64_t lines;
+  const uint64_t* S;  // string index
+  const uint64_t* LB; // leveled bitmaps, level stride Wp
+  int64_t Wp;
+  int L;              // levels present in LB
+  int R;              // result slots per line
+  const uint8_t* ir;  // device copy of the IR bytes
+  int64_t ir_len;
+  int64_t* out;       // int64[lines][R][2]
+  int64_t item0;      // first global work item (line) of this job
+};
+
+struct QueryLaunch {
+  const QueryJob* jobs;           // device array
+  int64_t njobs;
+  int64_t total_items;            // sum of lines over jobs (when lines_dev is null)
+  // When set, every job runs over the same index whose line count is read from
+  // device memory (no host round trip between index and query): item g is
+  // line g % lines of job g / lines, and job q writes its spans at
+  // jobs[q].out + q * lines * R * 2.
+  const int64_t* lines_dev;
+  unsigned long long* counter;    // zeroed before launch
+  int lines_per_wave;             // lines a wavefront claims per atomic
+};
+
+}  // namespace gj

@@ -1,0 +1,361 @@
+// query_kernel.hip -- per-line JSONPath IR interpreter for gfx950.
+//
+// Semantics are those of the reference's find_value kernel
+// (K/find_value.cu:21-263, == J/sequential/Sequential.java:146-337): a line is
+// evaluated with a byte cursor j that only moves forward while the interpreter
+// waits for a structural event at the current level (a ':' whose key matches,
+// or the n-th ','), and IR ops run between events.
+//
+// Mapping: one wavefront evaluates one line.  The IR, the cursor and the
+// per-level state are wave-uniform.  The waits are the expensive part, and
+// they are done wave-parallel: each lane takes one 64-bit word of the current
+// level's bitmap (64 words = 4 KiB of text per step), checks its candidate
+// positions, and the earliest event wins by ballot -- so the first match in
+// byte order is the one taken, exactly as in the sequential loop.  Level ends
+// (MOVE_DOWN) are found the same way.  Wavefronts claim lines_per_wave lines
+// at a time from an atomic counter.
+#include <hip/hip_runtime.h>
+#include "query_common.h"
+
+namespace gj {
+
+namespace {
+
+constexpr int kLv = 65;  // per-level state slots (levels 0..64)
+
+__device__ __forceinline__ bool is_blank(uint8_t c) { return c == '\n' || c == '\r' || c == '\t' || c == ' '; }
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int ctz64(unsigned long long x) { return __builtin_ctzll(x); }
+
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int src) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl(lo, src);
+  hi = __shfl(hi, src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
+  return (int64_t)shfl64((unsigned long long)v, src);
+}
+
+// Bits [lo_bit, 63] of a word.
+__device__ __forceinline__ uint64_t mask_from(int lo_bit) { return ~0ull << lo_bit; }
+// Bits [0, hi_bit] of a word.
+__device__ __forceinline__ uint64_t mask_upto(int hi_bit) { return hi_bit >= 63 ? ~0ull : ((2ull << hi_bit) - 1ull); }
+
+// Word wi of a level row restricted to positions [from, to).
+__device__ __forceinline__ uint64_t window_word(const uint64_t* row, int64_t wi, int64_t wbeg, int64_t wend,
+                                                int64_t from, int64_t to) {
+  if (wi > wend) return 0;
+  uint64_t v = row[wi];
+  if (wi == wbeg) v &= mask_from((int)(from & 63));
+  if (wi == wend) v &= mask_upto((int)((to - 1) & 63));
+  return v;
+}
+
+// First set bit of row in [from, to), or -1.  All lanes call with the same arguments.
+__device__ int64_t wave_first_bit(const uint64_t* row, int64_t from, int64_t to) {
+  if (from >= to) return -1;
+  const int lane = lane_id();
+  const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
+  for (int64_t base = wbeg; base <= wend; base += 64) {
+    int64_t wi = base + lane;
+    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    unsigned long long m = __ballot(v != 0);
+    if (m) {
+      int l = ctz64(m);
+      uint64_t vv = shfl64(v, l);
+      return ((base + l) << 6) + (ctz64(vv));
+    }
+  }
+  return -1;
+}
+
+// Key test at a ':' at position p (K/find_value.cu:227-250): the last in-string
+// byte before p must end a string of exactly klen bytes that starts with '"'
+// after the line start, and those bytes must equal the key.
+__device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, int64_t p, const uint8_t* key,
+                            int klen) {
+  int64_t k = p - 1;
+  if (k <= start) return false;
+  int64_t wi = k >> 6;
+  const int64_t wstart = start >> 6;
+  uint64_t sw = S[wi] & mask_upto((int)(k & 63));
+  int64_t se = -1;
+  for (;;) {
+    if (wi == wstart) sw &= ~mask_upto((int)(start & 63));
+    if (sw) { se = (wi << 6) + 63 - __clzll(sw); break; }
+    if (wi <= wstart) return false;
+    wi--;
+    sw = S[wi];
+  }
+  int64_t ss = se - klen;
+  if (ss < start || F[ss] != '"') return false;
+  for (int i = 0; i < klen; i++)
+    if (key[i] != F[ss + 1 + i]) return false;
+  return true;
+}
+
+// First structural ':' in [from, to) at this level whose key matches, or -1.
+__device__ int64_t wave_find_key(const uint64_t* row, const uint8_t* F, const uint64_t* S, int64_t start,
+                                 int64_t from, int64_t to, const uint8_t* key, int klen) {
+  if (from >= to) return -1;
+  const int lane = lane_id();
+  const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
+  for (int64_t base = wbeg; base <= wend; base += 64) {
+    int64_t wi = base + lane;
+    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    int64_t found = -1;
+    while (v) {
+      int b = ctz64(v);
+      int64_t p = (wi << 6) + b;
+      if (F[p] == ':' && key_matches(F, S, start, p, key, klen)) { found = p; break; }
+      v &= v - 1;
+    }
+    unsigned long long m = __ballot(found >= 0);
+    if (m) return shfl_i64(found, ctz64(m));
+  }
+  return -1;
+}
+
+// Walks structural ',' in [from, to), incrementing *count per comma; returns the
+// position of the comma at which *count becomes want, or -1 (count advanced by
+// all commas in range).
+__device__ int64_t wave_find_comma(const uint64_t* row, const uint8_t* F, int64_t from, int64_t to, int* count,
+                                   int want) {
+  if (from >= to) return -1;
+  const int lane = lane_id();
+  const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
+  int cnt0 = *count;
+  for (int64_t base = wbeg; base <= wend; base += 64) {
+    int64_t wi = base + lane;
+    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    uint64_t commas = 0;
+    uint64_t t = v;
+    while (t) {
+      int b = ctz64(t);
+      if (F[(wi << 6) + b] == ',') commas |= 1ull << b;
+      t &= t - 1;
+    }
+    int c = __popcll(commas);
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    int total = __shfl(incl, 63);
+    // the event is the first comma where cnt0 + (commas so far) == want
+    int need = want - cnt0;  // 1-based rank of the event comma in this window
+    bool hit = need >= 1 && need <= total && incl >= need && (incl - c) < need;
+    unsigned long long m = __ballot(hit);
+    if (m) {
+      int l = ctz64(m);
+      int64_t pos = -1;
+      if (lane == l) {
+        int rank = need - (incl - c);
+        uint64_t cm = commas;
+        for (int r = 1; r < rank; r++) cm &= cm - 1;
+        pos = (wi << 6) + (ctz64(cm));
+      }
+      *count = want;
+      return shfl_i64(pos, l);
+    }
+    cnt0 += total;
+  }
+  *count = cnt0;
+  return -1;
+}
+
+__device__ __forceinline__ int read_varint(const uint8_t* ir, int64_t ir_len, int64_t* pc) {
+  int v = 0, shift = 0;
+  while (*pc < ir_len) {
+    uint8_t b = ir[(*pc)++];
+    v |= (int)(b & 0x7F) << shift;
+    if (!(b & 0x80)) break;
+    shift += 7;
+    if (shift > 28) break;
+  }
+  return v;
+}
+
+// Evaluates one line of one job.  le/ci are this wavefront's LDS slots.
+__device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* out, int64_t* le, int* ci) {
+  const int lane = lane_id();
+  const uint8_t* F = J.F;
+  const int64_t n = J.N;
+  const int64_t start = J.NL[i];
+  const int64_t end = (i + 1 < lines) ? J.NL[i + 1] : n;
+  if (lane == 0)
+    for (int k = 0; k < J.R * 2; k++) out[k] = -1;
+  // Per-level state lives in this wavefront's LDS slots.  They are written by
+  // lane 0 (lane-parallel for the reset) and read by every lane; LDS operations
+  // of one wavefront complete in program order, so a later read sees the write.
+  for (int k = lane; k < kLv; k += 64) { le[k] = -1; ci[k] = -1; }
+  if (lane == 0) le[0] = end;
+
+  int lvl = 0;
+  int64_t j = start;
+  int64_t pc = 0;
+  int r = 0;
+  int op = kEnd;
+  int want_len = 0, want_idx = 0;
+  const uint8_t* want = nullptr;
+  bool exec = true;
+
+  for (;;) {
+    if (lvl < 0) return;
+    int64_t lim = le[lvl];
+    if (!(j < lim && j < n)) return;
+    while (exec) {
+      if (pc >= J.ir_len) return;
+      op = J.ir[pc++];
+      switch (op) {
+        case kEnd:
+          return;
+        case kStoreResult: {
+          while (j < n && is_blank(F[j])) j++;
+          if (r < J.R && lane == 0) {
+            out[r * 2] = j;
+            out[r * 2 + 1] = le[lvl];
+          }
+          r++;
+          break;
+        }
+        case kMoveUp:
+          j = le[lvl];
+          if (lane == 0) le[lvl] = -1;
+          lvl--;
+          if (lvl < 0) return;
+          break;
+        case kMoveDown: {
+          lvl++;
+          if (lvl >= kLv) return;
+          if (le[lvl] == -1) {
+            int64_t e = -1;
+            if (lvl - 1 < J.L) e = wave_first_bit(J.LB + (int64_t)(lvl - 1) * J.Wp, j + 1, le[lvl - 1]);
+            if (lane == 0) le[lvl] = e;
+          }
+          break;
+        }
+        case kMoveToKey:
+          want_len = read_varint(J.ir, J.ir_len, &pc);
+          want = J.ir + pc;
+          pc += want_len;
+          exec = false;
+          break;
+        case kMoveToIndex: {
+          want_idx = read_varint(J.ir, J.ir_len, &pc);
+          if (ci[lvl] == -1) {
+            while (j < n && F[j] != '[') {
+              if (!is_blank(F[j])) return;  // not an array: the line has no further results
+              j++;
+            }
+            if (j >= n) return;
+            if (lane == 0) ci[lvl] = 0;
+          }
+          if (want_idx == 0) { j++; exec = true; } else { exec = false; }
+          break;
+        }
+        case kExpressionStringEquals: {
+          want_len = read_varint(J.ir, J.ir_len, &pc);
+          want = J.ir + pc;
+          pc += want_len;
+          while (j < n && is_blank(F[j])) j++;
+          int64_t actual = le[lvl] - j;
+          if (actual != want_len) return;
+          for (int k = 0; k < want_len; k++)
+            if (want[k] != F[j + k]) return;
+          break;
+        }
+        default:
+          return;
+      }
+    }
+    if (!(j < n)) return;
+    // wait for the next event at this level
+    lim = le[lvl];
+    int64_t to = lim < n ? lim : n;
+    int64_t p = -1;
+    if (lvl < J.L) {
+      const uint64_t* row = J.LB + (int64_t)lvl * J.Wp;
+      if (op == kMoveToKey) {
+        p = wave_find_key(row, F, J.S, start, j, to, want, want_len);
+      } else if (op == kMoveToIndex) {
+        int cnt = ci[lvl];
+        p = wave_find_comma(row, F, j, to, &cnt, want_idx);
+        if (lane == 0) ci[lvl] = cnt;
+      }
+    }
+    if (p < 0) return;
+    exec = true;
+    j = p + 1;
+  }
+}
+
+}  // namespace
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch) {
+  constexpr int NW = BLOCK / 64;
+  __shared__ int64_t s_le[NW][kLv];
+  __shared__ int s_ci[NW][kLv];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
+  const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
+  const int64_t total = launch.lines_dev ? common * launch.njobs : launch.total_items;
+  for (;;) {
+    unsigned long long g0 = 0;
+    if (lane == 0) g0 = atomicAdd(launch.counter, (unsigned long long)lpw);
+    g0 = shfl64(g0, 0);
+    if ((int64_t)g0 >= total) break;
+    if (launch.lines_dev) {
+      for (int64_t g = (int64_t)g0; g < (int64_t)g0 + lpw && g < total; g++) {
+        const int64_t q = g / common, i = g - q * common;
+        const QueryJob& J = launch.jobs[q];
+        int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
+        eval_line(J, common, i, out, s_le[wave], s_ci[wave]);
+      }
+      continue;
+    }
+    // job of the first item
+    int lo = 0, hi = (int)launch.njobs - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
+    }
+    for (int64_t g = (int64_t)g0; g < (int64_t)g0 + lpw && g < total; g++) {
+      while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= g) lo++;
+      const QueryJob& J = launch.jobs[lo];
+      const int64_t i = g - J.item0;
+      eval_line(J, J.lines, i, J.out + i * (int64_t)J.R * 2, s_le[wave], s_ci[wave]);
+    }
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+hipError_t query_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream) {
+  switch (block) {
+    case 64: hipLaunchKernelGGL(query_kernel<64>, dim3(grid), dim3(64), 0, stream, L); break;
+    case 128: hipLaunchKernelGGL(query_kernel<128>, dim3(grid), dim3(128), 0, stream, L); break;
+    case 256: hipLaunchKernelGGL(query_kernel<256>, dim3(grid), dim3(256), 0, stream, L); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  int grid = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace gj

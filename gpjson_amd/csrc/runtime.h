@@ -1,0 +1,116 @@
+// runtime.h -- internal host-side structures of libgpjson_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gpjson.h"
+#include "index_common.h"
+#include "query_common.h"
+
+namespace gj {
+
+// Kernel entry points (index_kernel.hip, query_kernel.hip).
+hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
+                        hipStream_t stream);
+hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream);
+hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
+bool index_config_supported(int block, int wpt);
+int64_t index_tile_bytes(int block, int wpt);
+hipError_t query_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream);
+hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
+
+// Error reporting: every C entry point funnels failures through these.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define GJ_HIP(expr)                                  \
+  do {                                                \
+    hipError_t gj_e_ = (expr);                        \
+    if (gj_e_ != hipSuccess) return hip_fail(gj_e_, #expr); \
+  } while (0)
+
+struct IndexTune {
+  int block = 256;
+  int wpt = 2;
+};
+
+struct QueryTune {
+  int block = 256;
+  int waves_per_cu = 16;  // persistent grid size factor
+  int lines_per_wave = 1;
+};
+
+int nlv_for_levels(int L);
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+}  // namespace gj
+
+struct gpjson_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  gj::DevBuf states;      // lookback tile states
+  gj::DevBuf ticket;      // index ticket + query counter + line counts
+  gj::DevBuf jobs;        // device job arrays
+  gj::DevBuf irbuf;       // IR bytes of a query launch
+  void* pinned[2] = {nullptr, nullptr};
+  size_t pinned_size = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  void* host_scratch = nullptr;  // pinned scratch for small uploads
+  size_t host_scratch_size = 0;
+  gj::IndexTune itune;
+  gj::QueryTune qtune;
+};
+
+struct gpjson_buf {
+  gpjson_ctx* ctx = nullptr;
+  uint8_t* d = nullptr;
+  int64_t n = 0;
+  bool owned = false;
+};
+
+struct gpjson_index {
+  gpjson_ctx* ctx = nullptr;
+  const gpjson_buf* buf = nullptr;
+  int64_t n = 0, W = 0, Wp = 0;
+  int L = 0;
+  int flags = 0;
+  uint64_t* S = nullptr;
+  uint64_t* LB = nullptr;
+  int64_t* NL = nullptr;
+  int64_t nl_cap = 0;
+  uint64_t* E = nullptr;
+  uint64_t* Q = nullptr;
+  int64_t lines = 0;
+  int64_t* lines_dev = nullptr;  // device copy of the line count
+};
+
+namespace gj {
+// Device job table + launch descriptor for nq queries over one index.
+struct PreparedQueries {
+  QueryLaunch launch{};
+  int grid = 0;
+};
+int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs, const int64_t* ir_lens,
+                    int nq, int R, int64_t* spans_dev, PreparedQueries* out);
+int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel = nullptr,
+                    hipEvent_t after_kernel = nullptr);
+// Enqueues reset + index + line count for one index (no host sync).
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel = nullptr, hipEvent_t after_kernel = nullptr);
+// Enqueues one interpreter launch of nq queries over idx with the line count
+// read on the device; spans_dev receives int64[nq][lines][R][2].
+int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
+                        const int64_t* ir_lens, int nq, int R, int64_t* spans_dev);
+int upload_irs(gpjson_ctx* ctx, const uint8_t* const* irs, const int64_t* ir_lens, int nq,
+               std::vector<int64_t>* offsets, const uint8_t** dev_base);
+}  // namespace gj

@@ -1,0 +1,192 @@
+/*
+ * gpjson.h -- C ABI of the MI355X (gfx950) GPJSON index + query library
+ * (libgpjson_hip.so).
+ *
+ * The reference reaches the GPU through Truffle NFI bindings of libcudart /
+ * libcuda / libnvrtc made by CUDARuntime
+ * (gpjson/src/main/java/com/koenv/gpjson/gpu/CUDARuntime.java:76-94, symbol
+ * table CUDARuntimeFunction.java:39-194, CUDADriverFunction.java:34-46,
+ * NVRuntimeCompiler.java:297-307) and drives seven kernels from Java
+ * (functions/QueryFunction.java:48-265).  This library replaces that whole
+ * layer: a HIPRuntime on the Java side binds the entry points below instead
+ * of cudaMalloc / cuLaunchKernel / nvrtc*.  Paths cited below are relative to
+ * gpjson/src/main/java/com/koenv/gpjson/ in the reference.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: GPJSON_OK (0) or a GPJSON_ERR_*
+ *    code, mirroring CUDARuntime.checkCUDAReturnCode (gpu/CUDARuntime.java:506-516);
+ *    gpjson_last_error() gives the message of the calling thread's last failure.
+ *  - The library owns all device memory behind opaque handles; the caller owns
+ *    host buffers and passes their capacity.
+ *  - A context owns one HIP stream; calls on one context are serialized by the
+ *    caller (the reference is single-threaded per context).
+ *  - Offsets are 64-bit (the reference is limited to files < 2 GiB).
+ */
+#ifndef GPJSON_H
+#define GPJSON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GPJSON_OK = 0,
+  GPJSON_ERR_HIP = 1,          /* HIP runtime error (CUDA error in the reference) */
+  GPJSON_ERR_INVALID_ARG = 2,  /* bad argument (interop UnsupportedTypeException / ArityException) */
+  GPJSON_ERR_PARSE = 3,        /* JSONPathException: "Failed to parse JSON path" */
+  GPJSON_ERR_UNSUPPORTED = 4,  /* UnsupportedJSONPathException (reference falls back to jayway) */
+  GPJSON_ERR_IO = 5,           /* file missing / changed while reading */
+  GPJSON_ERR_CAPACITY = 6,     /* caller buffer too small or a stated limit exceeded */
+  GPJSON_ERR_NO_MEMORY = 7     /* device or pinned allocation failed / HBM budget exceeded */
+};
+
+/* Index export selectors for gpjson_index_export. */
+enum {
+  GPJSON_INDEX_NEWLINE = 0, /* int64[lines] */
+  GPJSON_INDEX_STRING = 1,  /* uint64[ceil(n/64)] */
+  GPJSON_INDEX_LEVELED = 2, /* uint64[levels][ceil(n/64)], level-major */
+  GPJSON_INDEX_ESCAPE = 3,  /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
+  GPJSON_INDEX_QUOTE = 4    /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
+};
+
+/* Flags of gpjson_index_build. */
+enum { GPJSON_BUILD_VERIFY = 1 /* also materialize escape and quote bitmaps */ };
+
+/* Flags of gpjson_query_file (QueryOptions, functions/QueryOptions.java:8-14). */
+enum { GPJSON_QUERY_DISABLE_FALLBACK = 1 };
+
+#define GPJSON_MAX_LEVELS 64   /* reference: 16, stages/LeveledBitmapsIndex.java:16 */
+#define GPJSON_MAX_QUERIES 255 /* functions/QueryFunction.java:74-77 */
+
+typedef struct gpjson_ctx gpjson_ctx;
+typedef struct gpjson_buf gpjson_buf;
+typedef struct gpjson_index gpjson_index;
+typedef struct gpjson_result gpjson_result;
+typedef struct gpjson_batch gpjson_batch;
+typedef struct gpjson_server gpjson_server;
+
+/* ---- errors -------------------------------------------------------------- */
+
+/* Message of the calling thread's last failed call (replaces GPJSONException
+ * text, GPJSONException.java:33-39).  Never NULL. */
+const char *gpjson_last_error(void);
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Opens device `device` with one stream (replaces `new CUDARuntime`,
+ * gpu/CUDARuntime.java:76-94, and the lazy context of :480-488). */
+int gpjson_ctx_create(int device, gpjson_ctx **out);
+/* Frees every device allocation of the context (CUDARuntime has no close). */
+void gpjson_ctx_destroy(gpjson_ctx *ctx);
+/* Waits for the context's stream (cudaDeviceSynchronize, gpu/CUDARuntime.java:388). */
+int gpjson_ctx_synchronize(gpjson_ctx *ctx);
+/* Compute units of the device (cudaDeviceGetAttribute, CUDARuntimeFunction.java:39-194). */
+int gpjson_ctx_num_cus(gpjson_ctx *ctx, int *out);
+
+/* ---- JSONPath compiler ---------------------------------------------------- */
+
+/* Compiles a JSONPath of the supported subset to IR bytes
+ * (jsonpath/JSONPathParser.java:20-261 + IRBuilder.java:19-81).  Writes at
+ * most cap bytes; *ir_len gets the full size.  GPJSON_ERR_PARSE or
+ * GPJSON_ERR_UNSUPPORTED on failure. */
+int gpjson_compile(const char *path, uint8_t *ir, size_t cap, size_t *ir_len, int *max_depth,
+                   int *num_results);
+
+/* ---- staged input -------------------------------------------------------- */
+
+/* Copies n host bytes to device memory through pinned staging buffers with
+ * hipMemcpyAsync (replaces cudaMalloc + synchronous pageable cudaMemcpy of
+ * QueryFunction.readFile, functions/QueryFunction.java:149,316-335). */
+int gpjson_buf_from_host(gpjson_ctx *ctx, const void *data, int64_t n, gpjson_buf **out);
+/* Reads a file straight into pinned staging and on to the device; fails with
+ * GPJSON_ERR_IO if its size changes while reading (QueryFunction.java:320-322). */
+int gpjson_buf_from_file(gpjson_ctx *ctx, const char *path, gpjson_buf **out);
+/* Wraps n bytes already resident on the context's device (no copy; the caller
+ * keeps ownership and must keep them alive while the handle is used). */
+int gpjson_buf_wrap_device(gpjson_ctx *ctx, const void *device_ptr, int64_t n, gpjson_buf **out);
+int64_t gpjson_buf_size(const gpjson_buf *buf);
+void gpjson_buf_free(gpjson_buf *buf);
+
+/* ---- structural index ---------------------------------------------------- */
+
+/* Builds newline, string and leveled indexes for `levels` levels in one fused
+ * launch (replaces CombinedIndex.create + LeveledBitmapsIndex.create,
+ * stages/CombinedIndex.java:28-56, stages/LeveledBitmapsIndex.java:40-54, and
+ * their kernels).  The index keeps a reference to buf. */
+int gpjson_index_build(gpjson_ctx *ctx, gpjson_buf *buf, int levels, int flags, gpjson_index **out);
+/* Rebuilds an index in place over the same buffers (no allocation). */
+int gpjson_index_rebuild(gpjson_index *idx);
+int64_t gpjson_index_lines(const gpjson_index *idx);
+int gpjson_index_levels(const gpjson_index *idx);
+/* Copies one index array to host memory (GPUUtils.readLongs,
+ * debug/GPUUtils.java:51-72); *size_bytes gets the full size. */
+int gpjson_index_export(const gpjson_index *idx, int which, void *dst, int64_t cap_bytes, int64_t *size_bytes);
+void gpjson_index_free(gpjson_index *idx);
+
+/* ---- per-line query ------------------------------------------------------ */
+
+/* Runs nq compiled queries over an index in one interpreter launch
+ * (replaces the per-query find_value loop, QueryFunction.java:201-241).
+ * spans receives int64[nq][lines][R][2] (start, end) with -1 for "no result";
+ * cap_elems is its capacity in int64 elements.  R = results per line. */
+int gpjson_query_run(gpjson_ctx *ctx, const gpjson_index *idx, const uint8_t *const *irs,
+                     const int64_t *ir_lens, int nq, int R, int64_t *spans, int64_t cap_elems);
+
+/* ---- the whole query() member -------------------------------------------- */
+
+/* query(file, paths[, options]) of the polyglot library object
+ * (GPJSONLibrary.java:89-98 -> QueryFunction.call, QueryFunction.java:48-265):
+ * compile, stage, index, query, copy spans back.  1 <= npaths <= 255. */
+int gpjson_query_file(gpjson_ctx *ctx, const char *file, const char *const *paths, int npaths, int flags,
+                      gpjson_result **out);
+/* Result shape [query][line][k] (result/Result.java:23-72, ResultArray.java:50-69). */
+int64_t gpjson_result_queries(const gpjson_result *res);
+int64_t gpjson_result_lines(const gpjson_result *res);
+int gpjson_result_per_line(const gpjson_result *res);
+/* int64[lines][R][2] spans of query q (the long[] of ResultArray); NULL if q is out of range. */
+const int64_t *gpjson_result_spans(const gpjson_result *res, int64_t q);
+/* UTF-8 bytes of element [q][line][k] (ResultItem.readArrayElement,
+ * result/ResultItem.java:32-52): returns 1 and sets ptr and len, or 0 for null. */
+int gpjson_result_value(const gpjson_result *res, int64_t q, int64_t line, int k, const char **ptr, int64_t *len);
+void gpjson_result_free(gpjson_result *res);
+
+/* ---- prepared plain path -------------------------------------------------- */
+
+typedef struct gpjson_plan gpjson_plan;
+
+/* Prepares the plain path "index then nq queries" over a resident buffer:
+ * allocates the index and the device span buffer once and builds the index
+ * once to size the newline array (QueryFunction.call without the file I/O,
+ * functions/QueryFunction.java:165-241).  Later runs allocate nothing. */
+int gpjson_plan_create(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens, int nq,
+                       gpjson_plan **out);
+/* Enqueues one pass (index kernel + one interpreter launch) on the context's
+ * stream without waiting; with timed != 0, HIP events bracket both launches. */
+int gpjson_plan_run(gpjson_plan *plan, int timed);
+/* Waits, then returns index / query time in ms of the last timed run. */
+int gpjson_plan_timing(gpjson_plan *plan, double *index_ms, double *query_ms);
+int64_t gpjson_plan_lines(const gpjson_plan *plan);
+int gpjson_plan_results_per_line(const gpjson_plan *plan);
+/* Waits, then copies int64[nq][lines][R][2] spans of the last run to host memory. */
+int gpjson_plan_spans(gpjson_plan *plan, int64_t *spans, int64_t cap_elems);
+/* The plan's index (valid until gpjson_plan_free). */
+const gpjson_index *gpjson_plan_index(const gpjson_plan *plan);
+void gpjson_plan_free(gpjson_plan *plan);
+
+/* ---- measurement --------------------------------------------------------- */
+
+/* Times `iterations` passes of index + nq queries on a resident buffer with
+ * HIP events on the context's stream (no allocation inside the timed region).
+ * ms_per_pass: mean wall time of one pass; index_ms / query_ms: kernel time
+ * split of the last pass. */
+int gpjson_time_plain(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens,
+                      int nq, int iterations, double *ms_per_pass, double *index_ms, double *query_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPJSON_H */
