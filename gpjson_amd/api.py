@@ -22,6 +22,7 @@ parity tests the way the reference's KernelTest exposes its stage classes
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import List, Optional, Sequence, Union
 
 import numpy as np
@@ -92,6 +93,12 @@ class Context:
         h = ctypes.c_void_p()
         _check(self._lib.gpjson_ctx_create(device, ctypes.byref(h)))
         self._h = h
+        # handles created on this context; freed before the context itself
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
+        return child
 
     @property
     def handle(self):
@@ -108,6 +115,10 @@ class Context:
 
     def close(self) -> None:
         if self._h:
+            # plans / batches / indexes first, buffers last
+            kids = sorted(list(self._children), key=lambda c: isinstance(c, Buffer))
+            for c in kids:
+                c.free()
             self._lib.gpjson_ctx_destroy(self._h)
             self._h = None
 
@@ -124,24 +135,24 @@ class Context:
         h = ctypes.c_void_p()
         ptr = arr.ctypes.data_as(ctypes.c_void_p) if arr.size else None
         _check(self._lib.gpjson_buf_from_host(self._h, ptr, arr.size, ctypes.byref(h)))
-        return Buffer(self, h)
+        return self._adopt(Buffer(self, h))
 
     def stage_file(self, path: str) -> "Buffer":
         h = ctypes.c_void_p()
         _check(self._lib.gpjson_buf_from_file(self._h, path.encode(), ctypes.byref(h)))
-        return Buffer(self, h)
+        return self._adopt(Buffer(self, h))
 
     def wrap_device(self, ptr: int, n: int) -> "Buffer":
         """Wraps n bytes already in this device's memory (e.g. a torch tensor's data_ptr())."""
         h = ctypes.c_void_p()
         _check(self._lib.gpjson_buf_wrap_device(self._h, ctypes.c_void_p(ptr), n, ctypes.byref(h)))
-        return Buffer(self, h)
+        return self._adopt(Buffer(self, h))
 
     def index(self, buf: "Buffer", levels: int, verify: bool = False) -> "Index":
         h = ctypes.c_void_p()
         flags = N.GPJSON_BUILD_VERIFY if verify else 0
         _check(self._lib.gpjson_index_build(self._h, buf.handle, levels, flags, ctypes.byref(h)))
-        return Index(self, h, buf)
+        return self._adopt(Index(self, h, buf))
 
     def run_queries(self, idx: "Index", irs: Sequence[bytes], results_per_line: int) -> np.ndarray:
         """int64[nq][lines][R][2] spans for compiled IRs over one index."""
@@ -163,7 +174,19 @@ class Context:
         lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
         h = ctypes.c_void_p()
         _check(self._lib.gpjson_plan_create(self._h, buf.handle, ptrs, lens, nq, ctypes.byref(h)))
-        return Plan(self, h, buf, nq)
+        return self._adopt(Plan(self, h, buf, nq))
+
+    def batch(self, bufs: Sequence["Buffer"], irs_per_file: Sequence[Sequence[bytes]]) -> "Batch":
+        """Many resident files: one persistent index launch + one interpreter launch per run."""
+        flat = [bytes(ir) for irs in irs_per_file for ir in irs]
+        cbufs = [ctypes.create_string_buffer(ir, max(len(ir), 1)) for ir in flat]
+        ptrs = (ctypes.c_void_p * len(flat))(*[ctypes.cast(b, ctypes.c_void_p) for b in cbufs])
+        lens = (ctypes.c_int64 * len(flat))(*[len(ir) for ir in flat])
+        nq = (ctypes.c_int * len(bufs))(*[len(irs) for irs in irs_per_file])
+        hb = (ctypes.c_void_p * len(bufs))(*[b.handle for b in bufs])
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_batch_create(self._h, hb, len(bufs), ptrs, lens, nq, ctypes.byref(h)))
+        return self._adopt(Batch(self, h, list(bufs), [len(irs) for irs in irs_per_file]))
 
     def time_plain(self, buf: "Buffer", irs: Sequence[bytes], iterations: int):
         """(ms per index+query pass, index ms, query ms) measured with HIP events on the library stream."""
@@ -206,9 +229,49 @@ class Plan:
         return out
 
     def free(self) -> None:
-        if self._h:
+        if self._h and self.ctx._h:
             N.load().gpjson_plan_free(self._h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched index + query over many resident files (see gpjson_batch_* in include/gpjson.h)."""
+
+    def __init__(self, ctx: Context, h, bufs, nq_per_file):
+        self.ctx, self._h, self.bufs, self.nq_per_file = ctx, h, bufs, nq_per_file
+
+    def run(self, graph: bool = True, timed: bool = False) -> None:
+        _check(N.load().gpjson_batch_run(self._h, 1 if graph else 0, 1 if timed else 0))
+
+    def timing(self):
+        a, t = ctypes.c_double(), ctypes.c_double()
+        _check(N.load().gpjson_batch_timing(self._h, ctypes.byref(a), ctypes.byref(t)))
+        return a.value, t.value
+
+    def lines(self, f: int) -> int:
+        v = ctypes.c_int64()
+        _check(N.load().gpjson_batch_lines(self._h, f, ctypes.byref(v)))
+        return v.value
+
+    def spans(self, f: int, q: int) -> np.ndarray:
+        lib = N.load()
+        R = lib.gpjson_batch_results_per_line(self._h, f, q)
+        if R < 1:
+            raise IndexError((f, q))
+        out = np.empty((self.lines(f), R, 2), dtype=np.int64)
+        _check(lib.gpjson_batch_spans(self._h, f, q, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def free(self) -> None:
+        if self._h and self.ctx._h:
+            N.load().gpjson_batch_free(self._h)
+        self._h = None
 
     def __del__(self):
         try:
@@ -231,9 +294,9 @@ class Buffer:
         return int(N.load().gpjson_buf_size(self._h))
 
     def free(self) -> None:
-        if self._h:
+        if self._h and self.ctx._h:
             N.load().gpjson_buf_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -292,9 +355,9 @@ class Index:
         return self.export(N.GPJSON_INDEX_QUOTE)
 
     def free(self) -> None:
-        if self._h:
+        if self._h and self.ctx._h:
             N.load().gpjson_index_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

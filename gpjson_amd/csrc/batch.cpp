@@ -1,0 +1,372 @@
+// batch.cpp -- batched execution: many files of uneven size indexed by one
+// persistent index launch and queried by one interpreter launch.
+//
+// The reference handles one file per query() call (J/functions/QueryFunction.java:
+// 48-265, J/ = gpjson/src/main/java/com/koenv/gpjson/); batching has no
+// counterpart there.  Every file of a batch gets its own index arrays (sized
+// for its own depth L, the largest of its queries) inside shared arenas.
+//
+// One run is:
+//   reset kernel      lookback states + tile ticket
+//   index kernel      persistent grid; tiles of all files are claimed from one
+//                     atomic ticket (the device work queue of tile ranges); a
+//                     tile finds its file by binary search over the files'
+//                     first tile ids, and the lookback stops at the file's first
+//                     tile, which starts from the initial carry
+//   batch plan kernel per-file line counts -> per-(file, query) work ranges
+//   interpreter       one launch over every (file, query, line) item
+// After the first (uncaptured) run has sized the arrays, the sequence is
+// captured once into a hipGraph and replayed; nothing is allocated per run.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "runtime.h"
+
+using namespace gj;
+
+struct gpjson_batch {
+  gpjson_ctx* ctx = nullptr;
+  int nfiles = 0;
+  int njobs = 0;
+  std::vector<gpjson_buf*> bufs;
+  std::vector<FileJob> files;       // host copy of the file table
+  std::vector<QueryJob> qjobs;      // host copy of the job table (lines/item0 filled on device)
+  std::vector<int> job_file, job_q, job_R;
+  std::vector<int64_t> job_span_off;  // element offset of each job's spans in the span arena
+  std::vector<int> file_first_job;
+  int64_t total_tiles = 0;
+  // device memory
+  DevBuf arena_s, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
+  int64_t* total_dev = nullptr;
+  int64_t* lines_dev = nullptr;
+  int* overflow_dev = nullptr;
+  int* job_file_dev = nullptr;
+  unsigned int* ticket = nullptr;
+  unsigned long long* counter = nullptr;
+  IndexTune itune;
+  QueryTune qtune;
+  int index_grid = 0;
+  int query_grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed_graph = false;
+  std::vector<int64_t> lines;  // per file, after a run
+};
+
+static int levels_of(const uint8_t* ir, int64_t len, int* R) {
+  int depth = 0, maxd = 0, stores = 0;
+  for (int64_t pc = 0; pc < len;) {
+    uint8_t op = ir[pc++];
+    if (op == kMoveDown) { depth++; maxd = std::max(maxd, depth); }
+    else if (op == kMoveUp) depth--;
+    else if (op == kStoreResult) stores++;
+    else if (op == kMoveToKey || op == kExpressionStringEquals || op == kMoveToIndex) {
+      uint32_t v = 0;
+      int sh = 0;
+      while (pc < len) {
+        uint8_t b = ir[pc++];
+        v |= (uint32_t)(b & 0x7F) << sh;
+        if (!(b & 0x80)) break;
+        sh += 7;
+      }
+      if (op != kMoveToIndex) pc += v;
+    }
+  }
+  *R = std::max(stores, 1);
+  return maxd;
+}
+
+// Lays out the arenas for the current capacities and uploads the tables.
+static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const uint8_t* const* irs,
+                  const int64_t* ir_lens) {
+  gpjson_ctx* ctx = b->ctx;
+  const int64_t tile_bytes = index_tile_bytes(b->itune.block, b->itune.wpt);
+  int64_t s_words = 0, lb_words = 0, nl_elems = 0, span_elems = 0, tiles = 0;
+  std::vector<int64_t> s_off(b->nfiles), lb_off(b->nfiles), nl_off(b->nfiles);
+  for (int f = 0; f < b->nfiles; f++) {
+    FileJob& J = b->files[f];
+    const int64_t n = b->bufs[f]->n;
+    J.N = n;
+    J.W = (n + 63) / 64;
+    J.Wp = (J.W + 7) / 8 * 8;
+    J.nl_cap = nl_caps[f];
+    J.ntiles = (n + tile_bytes - 1) / tile_bytes;
+    J.tile0 = tiles;
+    tiles += J.ntiles;
+    s_off[f] = s_words;
+    s_words += (J.W + 7) / 8 * 8;
+    lb_off[f] = lb_words;
+    lb_words += J.Wp * std::max(J.L, 1);
+    nl_off[f] = nl_elems;
+    nl_elems += (J.nl_cap + 7) / 8 * 8;
+  }
+  b->job_span_off.assign(b->njobs, 0);
+  for (int j = 0; j < b->njobs; j++) {
+    b->job_span_off[j] = span_elems;
+    span_elems += b->files[b->job_file[j]].nl_cap * b->job_R[j] * 2;
+  }
+  b->total_tiles = tiles;
+  int rc = b->arena_s.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
+  if (!rc) rc = b->arena_lb.ensure((size_t)std::max<int64_t>(lb_words, 1) * 8);
+  if (!rc) rc = b->arena_nl.ensure((size_t)std::max<int64_t>(nl_elems, 1) * 8);
+  if (!rc) rc = b->arena_spans.ensure((size_t)std::max<int64_t>(span_elems, 1) * 8);
+  if (!rc) rc = b->states.ensure((size_t)std::max<int64_t>(tiles, 1) * sizeof(TileState));
+  if (!rc) rc = b->dev_files.ensure(sizeof(FileJob) * (size_t)b->nfiles);
+  if (!rc) rc = b->dev_qjobs.ensure(sizeof(QueryJob) * (size_t)b->njobs);
+  if (!rc) rc = b->dev_misc.ensure(4096 + (size_t)b->nfiles * 8 + (size_t)b->njobs * 4);
+  if (rc) return rc;
+  for (int f = 0; f < b->nfiles; f++) {
+    FileJob& J = b->files[f];
+    J.F = b->bufs[f]->d;
+    J.S = (uint64_t*)b->arena_s.p + s_off[f];
+    J.LB = (uint64_t*)b->arena_lb.p + lb_off[f];
+    J.NL = (int64_t*)b->arena_nl.p + nl_off[f];
+    J.E = nullptr;
+    J.Q = nullptr;
+  }
+  // IR bytes
+  int64_t ir_total = 0;
+  for (int j = 0; j < b->njobs; j++) ir_total += ir_lens[j];
+  rc = b->dev_ir.ensure((size_t)ir_total + 16);
+  if (rc) return rc;
+  std::vector<uint8_t> flat((size_t)ir_total + 16);
+  int64_t off = 0;
+  for (int j = 0; j < b->njobs; j++) {
+    QueryJob& q = b->qjobs[j];
+    const FileJob& J = b->files[b->job_file[j]];
+    std::memcpy(flat.data() + off, irs[j], (size_t)ir_lens[j]);
+    q.F = J.F;
+    q.N = J.N;
+    q.NL = J.NL;
+    q.lines = 0;
+    q.S = J.S;
+    q.LB = J.LB;
+    q.Wp = J.Wp;
+    q.L = J.L;
+    q.R = b->job_R[j];
+    q.ir = (const uint8_t*)b->dev_ir.p + off;
+    q.ir_len = ir_lens[j];
+    q.out = (int64_t*)b->arena_spans.p + b->job_span_off[j];
+    q.item0 = 0;
+    off += ir_lens[j];
+  }
+  uint8_t* misc = (uint8_t*)b->dev_misc.p;
+  b->total_dev = (int64_t*)misc;
+  b->overflow_dev = (int*)(misc + 8);
+  b->ticket = (unsigned int*)(misc + 64);
+  b->counter = (unsigned long long*)(misc + 128);
+  b->lines_dev = (int64_t*)(misc + 4096);
+  b->job_file_dev = (int*)(misc + 4096 + (size_t)b->nfiles * 8);
+  GJ_HIP(hipMemcpy(b->dev_ir.p, flat.data(), flat.size(), hipMemcpyHostToDevice));
+  GJ_HIP(hipMemcpy(b->dev_files.p, b->files.data(), sizeof(FileJob) * b->nfiles, hipMemcpyHostToDevice));
+  GJ_HIP(hipMemcpy(b->dev_qjobs.p, b->qjobs.data(), sizeof(QueryJob) * b->njobs, hipMemcpyHostToDevice));
+  GJ_HIP(hipMemcpy(b->job_file_dev, b->job_file.data(), sizeof(int) * b->njobs, hipMemcpyHostToDevice));
+  GJ_HIP(hipMemset(b->dev_misc.p, 0, 4096));
+  const int bpc = 2048 / b->itune.block;  // resident blocks per CU the grid aims for
+  b->index_grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ctx->num_cus * bpc));
+  b->query_grid = std::max(1, ctx->num_cus * b->qtune.waves_per_cu / (b->qtune.block / 64));
+  return GPJSON_OK;
+}
+
+// Enqueues one batched run on the context stream (capturable: kernels only).
+static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2) {
+  gpjson_ctx* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  IndexLaunch L{};
+  L.jobs = (const FileJob*)b->dev_files.p;
+  L.njobs = b->nfiles;
+  L.total_tiles = b->total_tiles;
+  L.states = (TileState*)b->states.p;
+  L.ticket = b->ticket;
+  int maxL = 0;
+  for (const auto& f : b->files) maxL = std::max(maxL, f.L);
+  GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s));
+  GJ_HIP(zero_u64(b->counter, 1, s));
+  if (e0) GJ_HIP(hipEventRecord(e0, s));
+  if (b->total_tiles > 0) GJ_HIP(index_launch(L, b->itune.block, b->itune.wpt, nlv_for_levels(maxL), false,
+                                              b->index_grid, s));
+  if (e1) GJ_HIP(hipEventRecord(e1, s));
+  GJ_HIP(batch_plan((const FileJob*)b->dev_files.p, b->nfiles, L.states, (QueryJob*)b->dev_qjobs.p, b->job_file_dev,
+                    b->njobs, b->total_dev, b->lines_dev, b->overflow_dev, s));
+  QueryLaunch Q{};
+  Q.jobs = (const QueryJob*)b->dev_qjobs.p;
+  Q.njobs = b->njobs;
+  Q.total_items = 0;
+  Q.lines_dev = nullptr;
+  Q.total_dev = b->total_dev;
+  Q.counter = b->counter;
+  Q.lines_per_wave = b->qtune.lines_per_wave;
+  GJ_HIP(query_launch(Q, b->qtune.block, b->query_grid, s));
+  if (e2) GJ_HIP(hipEventRecord(e2, s));
+  return GPJSON_OK;
+}
+
+static int read_back_lines(gpjson_batch* b, bool* overflow) {
+  int ov = 0;
+  GJ_HIP(hipStreamSynchronize(b->ctx->stream));
+  GJ_HIP(hipMemcpy(&ov, b->overflow_dev, sizeof(int), hipMemcpyDeviceToHost));
+  b->lines.assign(b->nfiles, 0);
+  GJ_HIP(hipMemcpy(b->lines.data(), b->lines_dev, sizeof(int64_t) * b->nfiles, hipMemcpyDeviceToHost));
+  *overflow = ov != 0;
+  return GPJSON_OK;
+}
+
+static void drop_graph(gpjson_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+extern "C" {
+
+int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, const uint8_t* const* irs,
+                        const int64_t* ir_lens, const int* nq_per_file, gpjson_batch** out) {
+  if (!ctx || !bufs || !irs || !ir_lens || !nq_per_file || !out || nfiles < 1)
+    return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  *out = nullptr;
+  GJ_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<gpjson_batch> b(new gpjson_batch());
+  b->ctx = ctx;
+  b->nfiles = nfiles;
+  b->itune = ctx->itune;
+  b->qtune = ctx->qtune;
+  b->bufs.assign(bufs, bufs + nfiles);
+  b->files.assign(nfiles, FileJob{});
+  int j = 0;
+  for (int f = 0; f < nfiles; f++) {
+    if (!bufs[f]) return fail(GPJSON_ERR_INVALID_ARG, "NULL buffer");
+    if (nq_per_file[f] < 1 || nq_per_file[f] > GPJSON_MAX_QUERIES)
+      return fail(GPJSON_ERR_INVALID_ARG, "each file needs 1..255 queries");
+    b->file_first_job.push_back(j);
+    int L = 0;
+    for (int q = 0; q < nq_per_file[f]; q++, j++) {
+      int R = 1;
+      int l = levels_of(irs[j], ir_lens[j], &R);
+      L = std::max(L, l);
+      b->job_file.push_back(f);
+      b->job_q.push_back(q);
+      b->job_R.push_back(R);
+    }
+    if (L > GPJSON_MAX_LEVELS) return fail(GPJSON_ERR_CAPACITY, "JSON path deeper than 64 levels");
+    b->files[f].L = L;
+  }
+  b->file_first_job.push_back(j);
+  b->njobs = j;
+  b->qjobs.assign(j, QueryJob{});
+  for (int k = 0; k < 4; k++) GJ_HIP(hipEventCreate(&b->ev[k]));
+  std::vector<int64_t> caps(nfiles);
+  for (int f = 0; f < nfiles; f++) caps[f] = bufs[f]->n / 256 + 4096;
+  int rc = layout(b.get(), caps, irs, ir_lens);
+  if (rc) return rc;
+  // first run sizes the newline arrays exactly (grows them if a file overflowed)
+  rc = enqueue_batch(b.get(), nullptr, nullptr, nullptr);
+  bool ov = false;
+  if (!rc) rc = read_back_lines(b.get(), &ov);
+  if (!rc && ov) {
+    // the lookback states hold every file's true line count even when its
+    // newline array overflowed: lay out again with exact capacities
+    for (int f = 0; f < nfiles; f++) caps[f] = std::max<int64_t>(b->lines[f], caps[f]);
+    rc = layout(b.get(), caps, irs, ir_lens);
+    if (!rc) rc = enqueue_batch(b.get(), nullptr, nullptr, nullptr);
+    if (!rc) rc = read_back_lines(b.get(), &ov);
+    if (!rc && ov) rc = fail(GPJSON_ERR_CAPACITY, "newline capacity still exceeded");
+  }
+  if (rc) return rc;
+  *out = b.release();
+  return GPJSON_OK;
+}
+
+int gpjson_batch_run(gpjson_batch* b, int use_graph, int timed) {
+  if (!b) return fail(GPJSON_ERR_INVALID_ARG, "batch is NULL");
+  gpjson_ctx* ctx = b->ctx;
+  GJ_HIP(hipSetDevice(ctx->device));
+  if (!use_graph) {
+    b->timed_graph = false;
+    return enqueue_batch(b, timed ? b->ev[0] : nullptr, timed ? b->ev[1] : nullptr, timed ? b->ev[2] : nullptr);
+  }
+  if (!b->exec) {
+    GJ_HIP(hipStreamSynchronize(ctx->stream));
+    GJ_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch(b, nullptr, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(b);
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  if (timed) GJ_HIP(hipEventRecord(b->ev[0], ctx->stream));
+  GJ_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  if (timed) {
+    GJ_HIP(hipEventRecord(b->ev[2], ctx->stream));
+    b->timed_graph = true;
+  }
+  return GPJSON_OK;
+}
+
+int gpjson_batch_timing(gpjson_batch* b, double* index_ms, double* total_ms) {
+  if (!b) return fail(GPJSON_ERR_INVALID_ARG, "batch is NULL");
+  GJ_HIP(hipEventSynchronize(b->ev[2]));
+  float a = -1, t = 0;
+  if (!b->timed_graph) GJ_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+  GJ_HIP(hipEventElapsedTime(&t, b->ev[0], b->ev[2]));
+  if (index_ms) *index_ms = a;
+  if (total_ms) *total_ms = t;
+  return GPJSON_OK;
+}
+
+int gpjson_batch_lines(gpjson_batch* b, int file, int64_t* lines) {
+  if (!b || !lines || file < 0 || file >= b->nfiles) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  bool ov = false;
+  int rc = read_back_lines(b, &ov);
+  if (rc) return rc;
+  if (ov) return fail(GPJSON_ERR_CAPACITY, "input changed: more lines than the batch holds");
+  *lines = b->lines[file];
+  return GPJSON_OK;
+}
+
+int gpjson_batch_spans(gpjson_batch* b, int file, int q, int64_t* spans, int64_t cap_elems) {
+  if (!b || !spans || file < 0 || file >= b->nfiles) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  const int j = b->file_first_job[file] + q;
+  if (q < 0 || j >= b->file_first_job[file + 1]) return fail(GPJSON_ERR_INVALID_ARG, "query out of range");
+  int64_t lines = 0;
+  int rc = gpjson_batch_lines(b, file, &lines);
+  if (rc) return rc;
+  const int64_t need = lines * b->job_R[j] * 2;
+  if (cap_elems < need) return fail(GPJSON_ERR_CAPACITY, "span buffer too small");
+  GJ_HIP(hipMemcpy(spans, (int64_t*)b->arena_spans.p + b->job_span_off[j], (size_t)need * 8,
+                   hipMemcpyDeviceToHost));
+  return GPJSON_OK;
+}
+
+int gpjson_batch_results_per_line(const gpjson_batch* b, int file, int q) {
+  if (!b || file < 0 || file >= b->nfiles) return -1;
+  const int j = b->file_first_job[file] + q;
+  if (q < 0 || j >= b->file_first_job[file + 1]) return -1;
+  return b->job_R[j];
+}
+
+void gpjson_batch_free(gpjson_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  drop_graph(b);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  DevBuf* bufs[] = {&b->arena_s, &b->arena_lb, &b->arena_nl, &b->arena_spans, &b->dev_files,
+                    &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states};
+  for (DevBuf* d : bufs) d->release();
+  delete b;
+}
+
+}  // extern "C"

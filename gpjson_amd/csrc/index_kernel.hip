@@ -427,7 +427,11 @@ __global__ void index_lines_kernel(const FileJob* jobs, FileJob one, int64_t njo
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= njobs) return;
   const FileJob& J = jobs ? jobs[i] : one;
-  if (J.ntiles == 0) { lines_out[i] = 1; return; }
+  if (J.ntiles == 0) {  // empty file: one line starting at 0
+    if (J.nl_cap > 0) J.NL[0] = 0;
+    lines_out[i] = 1;
+    return;
+  }
   unsigned long long v = states[J.tile0 + J.ntiles - 1].inc1;
   lines_out[i] = (int64_t)(v & ~kValid);
 }

@@ -42,6 +42,9 @@ struct QueryLaunch {
   // line g % lines of job g / lines, and job q writes its spans at
   // jobs[q].out + q * lines * R * 2.
   const int64_t* lines_dev;
+  // When set (and lines_dev is null), the job table's lines / item0 were
+  // written on the device and the item count is read from here.
+  const int64_t* total_dev;
   unsigned long long* counter;    // zeroed before launch
   int lines_per_wave;             // lines a wavefront claims per atomic
 };

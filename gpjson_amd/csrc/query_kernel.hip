@@ -15,6 +15,7 @@
 // (MOVE_DOWN) are found the same way.  Wavefronts claim lines_per_wave lines
 // at a time from an atomic counter.
 #include <hip/hip_runtime.h>
+#include "index_common.h"
 #include "query_common.h"
 
 namespace gj {
@@ -306,7 +307,8 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch) {
   const int lane = threadIdx.x & 63;
   const int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
-  const int64_t total = launch.lines_dev ? common * launch.njobs : launch.total_items;
+  const int64_t total = launch.lines_dev ? common * launch.njobs
+                                         : (launch.total_dev ? *launch.total_dev : launch.total_items);
   for (;;) {
     unsigned long long g0 = 0;
     if (lane == 0) g0 = atomicAdd(launch.counter, (unsigned long long)lpw);
@@ -348,6 +350,40 @@ hipError_t query_launch(const QueryLaunch& L, int block, int grid, hipStream_t s
     case 256: hipLaunchKernelGGL(query_kernel<256>, dim3(grid), dim3(256), 0, stream, L); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// Batched runs: fills each query job's line count and first work item from the
+// index launch's per-file line counts (last tile's inclusive prefix), so the
+// interpreter launch needs no host round trip.  One block.
+__global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs,
+                                  const int* job_file, int njobs, int64_t* total_out, int64_t* lines_out,
+                                  int* overflow) {
+  if (threadIdx.x != 0) return;
+  int64_t item = 0;
+  for (int f = 0; f < nfiles; f++) {
+    const FileJob& J = files[f];
+    int64_t lines = 1;
+    if (J.ntiles > 0) lines = (int64_t)(states[J.tile0 + J.ntiles - 1].inc1 & ~(1ull << 63));
+    else if (J.nl_cap > 0) J.NL[0] = 0;
+    if (lines > J.nl_cap) *overflow = 1;
+    lines_out[f] = lines;  // true count; jobs are clamped to what the newline array holds
+  }
+  for (int j = 0; j < njobs; j++) {
+    const FileJob& J = files[job_file[j]];
+    int64_t lines = lines_out[job_file[j]];
+    if (lines > J.nl_cap) lines = J.nl_cap;
+    qjobs[j].lines = lines;
+    qjobs[j].item0 = item;
+    item += lines;
+  }
+  *total_out = item;
+}
+
+hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
+                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, states, qjobs, job_file, njobs,
+                     total_out, lines_out, overflow);
   return hipGetLastError();
 }
 

@@ -176,6 +176,43 @@ int gpjson_plan_spans(gpjson_plan *plan, int64_t *spans, int64_t cap_elems);
 const gpjson_index *gpjson_plan_index(const gpjson_plan *plan);
 void gpjson_plan_free(gpjson_plan *plan);
 
+/* ---- batched execution (no counterpart in the reference) ------------------ */
+
+/* Prepares a batch of nfiles resident buffers.  File f runs nq_per_file[f]
+ * queries; their IRs are irs[k..k+nq_per_file[f]) in file order.  Each file's
+ * index has the depth of its deepest query.  The first run happens here and
+ * sizes every array; later runs allocate nothing. */
+int gpjson_batch_create(gpjson_ctx *ctx, gpjson_buf *const *bufs, int nfiles, const uint8_t *const *irs,
+                        const int64_t *ir_lens, const int *nq_per_file, gpjson_batch **out);
+/* Enqueues one run: one persistent index launch over the tiles of every file
+ * (atomic tile queue), a device-side line scan, and one interpreter launch over
+ * every (file, query, line).  With use_graph, the sequence is captured into a
+ * hipGraph on first use and replayed.  No host wait. */
+int gpjson_batch_run(gpjson_batch *batch, int use_graph, int timed);
+/* Waits; index_ms (-1 for graph runs) and total_ms of the last timed run. */
+int gpjson_batch_timing(gpjson_batch *batch, double *index_ms, double *total_ms);
+int gpjson_batch_lines(gpjson_batch *batch, int file, int64_t *lines);
+int gpjson_batch_results_per_line(const gpjson_batch *batch, int file, int q);
+/* Waits, then copies int64[lines][R][2] spans of (file, query q) to host memory. */
+int gpjson_batch_spans(gpjson_batch *batch, int file, int q, int64_t *spans, int64_t cap_elems);
+void gpjson_batch_free(gpjson_batch *batch);
+
+/* ---- launch shapes and the autotuner -------------------------------------- */
+
+/* Current launch shapes of the context: index tile = index_block threads x
+ * index_wpt 64-byte words; interpreter block size, persistent grid in
+ * wavefronts per CU, and lines a wavefront claims at a time.  The reference
+ * hard-codes 8x1024 (stages/CombinedIndex.java:11-14) and 512x1024
+ * (functions/QueryFunction.java:229). */
+int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *query_block, int *query_waves_per_cu,
+                      int *lines_per_wave);
+int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query_block, int query_waves_per_cu,
+                      int lines_per_wave);
+/* Times every supported shape on buf with the given queries and installs the
+ * fastest; report (optional) gets one text line per candidate. */
+int gpjson_autotune(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens, int nq,
+                    int iterations, char *report, size_t report_cap);
+
 /* ---- measurement --------------------------------------------------------- */
 
 /* Times `iterations` passes of index + nq queries on a resident buffer with

@@ -1,0 +1,61 @@
+"""Batched execution equals the plain path bit for bit (every file, every query),
+both for direct runs and for hipGraph replays with eager work in between."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWEET_Q = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name",
+           "$.entities.hashtags[0].text", "$.user.profile.colors.link", "$.lang"]
+ADV_Q = ["$.chain.c1.c2.c3", "$.id", "$.arr[0]", "$.arr[999]", "$.deep.d.d", "$.s1",
+         "$.chain" + "".join(f".c{i}" for i in range(1, 10)), "$.arr[3].v"]
+
+
+def _files(seed, nfiles):
+    from gpjson_amd import datagen
+    rng = np.random.default_rng(seed)
+    out = []
+    for f in range(nfiles):
+        kind = "tweets" if f % 3 else "adversarial"
+        size = int(rng.integers(1, 600)) * 4096 + int(rng.integers(0, 4096))
+        if f == 5:
+            size = 17  # tiny file: one partial tile
+        out.append((kind, datagen.generate(kind, 1000 + f, size).tobytes()))
+    return out
+
+
+def test_batch_equals_plain(ctx):
+    from gpjson_amd import compile_path
+    files = _files(7, 12)
+    bufs = [ctx.stage(d) for _, d in files]
+    qsets = [[compile_path(p) for p in (TWEET_Q if k == "tweets" else ADV_Q)] for k, _ in files]
+    print("[batch] building", flush=True)
+    batch = ctx.batch(bufs, [[c.ir for c in qs] for qs in qsets])
+    # plain references
+    plain = []
+    for (k, d), buf, qs in zip(files, bufs, qsets):
+        L = max(c.max_depth for c in qs)
+        idx = ctx.index(buf, L)
+        plain.append([ctx.run_queries(idx, [c.ir], c.num_results)[0] for c in qs])
+        idx.free()
+
+    def check(tag):
+        for f in range(len(files)):
+            for q in range(len(qsets[f])):
+                got = batch.spans(f, q)
+                assert np.array_equal(got, plain[f][q]), (tag, f, q)
+
+    check("first run")
+    batch.run(graph=False)
+    check("direct run")
+    for r in range(3):
+        print(f"[batch] graph replay {r}", flush=True)
+        batch.run(graph=True, timed=True)
+        ctx.synchronize()
+        check(f"graph replay {r}")
+        # eager work between replays
+        idx = ctx.index(bufs[0], 2)
+        idx.free()
+    batch.free()
