@@ -82,6 +82,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=512 << 20)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
+    ap.add_argument("--config", default="", help="index_block,index_wpt,query_block,query_group,wpc,lpg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,6 +110,15 @@ def main() -> int:
     log(f"staged into HBM in {stage_s:.2f} s ({data.size / stage_s / 1e9:.1f} GB/s host->device incl. pinned copy)")
 
     cps = [compile_path(p) for p in paths]
+    if args.config:
+        ctx.set_config(*[int(x) for x in args.config.split(",")])
+    if args.autotune:
+        t = time.perf_counter()
+        report = ctx.autotune(buf, [cps[0].ir], 2)
+        for line in report.strip().splitlines():
+            log("autotune: " + line)
+        log(f"autotune took {time.perf_counter() - t:.1f} s")
+    log(f"launch shapes: {ctx.get_config()}")
     plans = [ctx.plan(buf, [cp.ir]) for cp in cps]
     lines = plans[0].lines
     log(f"plans ready: {lines} lines, levels {[cp.max_depth for cp in cps]}")
@@ -175,6 +186,7 @@ def main() -> int:
         "index_ms": [round(x, 4) for x in idx_ms],
         "query_ms": [round(x, 4) for x in q_ms],
         "staging_GBps": round(data.size / stage_s / 1e9, 2),
+        "launch_shapes": ctx.get_config(),
     }
     for p in plans:
         p.free()

@@ -188,6 +188,27 @@ class Context:
         _check(self._lib.gpjson_batch_create(self._h, hb, len(bufs), ptrs, lens, nq, ctypes.byref(h)))
         return self._adopt(Batch(self, h, list(bufs), [len(irs) for irs in irs_per_file]))
 
+    def get_config(self) -> dict:
+        v = [ctypes.c_int() for _ in range(6)]
+        _check(self._lib.gpjson_get_config(self._h, *[ctypes.byref(x) for x in v]))
+        keys = ("index_block", "index_wpt", "query_block", "query_group", "query_waves_per_cu", "lines_per_group")
+        return dict(zip(keys, [x.value for x in v]))
+
+    def set_config(self, index_block: int, index_wpt: int, query_block: int, query_group: int,
+                   query_waves_per_cu: int, lines_per_group: int) -> None:
+        _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_block, query_group,
+                                           query_waves_per_cu, lines_per_group))
+
+    def autotune(self, buf: "Buffer", irs: Sequence[bytes], iterations: int = 3) -> str:
+        """Times every launch shape on buf and installs the fastest; returns the report."""
+        nq = len(irs)
+        bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
+        ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
+        lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
+        rep = ctypes.create_string_buffer(16384)
+        _check(self._lib.gpjson_autotune(self._h, buf.handle, ptrs, lens, nq, iterations, rep, 16384))
+        return rep.value.decode()
+
     def time_plain(self, buf: "Buffer", irs: Sequence[bytes], iterations: int):
         """(ms per index+query pass, index ms, query ms) measured with HIP events on the library stream."""
         nq = len(irs)

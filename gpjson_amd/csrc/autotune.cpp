@@ -25,32 +25,33 @@ struct IdxCand {
 const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
 
 struct QCand {
-  int block, wpc, lpw;
+  int block, group, wpc, lpw;
 };
 
 }  // namespace
 
 extern "C" {
 
-int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_block, int* query_waves_per_cu,
-                      int* lines_per_wave) {
+int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_block, int* query_group,
+                      int* query_waves_per_cu, int* lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
   if (index_block) *index_block = ctx->itune.block;
   if (index_wpt) *index_wpt = ctx->itune.wpt;
   if (query_block) *query_block = ctx->qtune.block;
+  if (query_group) *query_group = ctx->qtune.group;
   if (query_waves_per_cu) *query_waves_per_cu = ctx->qtune.waves_per_cu;
   if (lines_per_wave) *lines_per_wave = ctx->qtune.lines_per_wave;
   return GPJSON_OK;
 }
 
-int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_block, int query_waves_per_cu,
-                      int lines_per_wave) {
+int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_block, int query_group,
+                      int query_waves_per_cu, int lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
   if (!index_config_supported(index_block, index_wpt))
     return fail(GPJSON_ERR_INVALID_ARG, "unsupported index shape " + std::to_string(index_block) + "x" +
                                             std::to_string(index_wpt));
-  if (query_block != 64 && query_block != 128 && query_block != 256)
-    return fail(GPJSON_ERR_INVALID_ARG, "query block must be 64, 128 or 256");
+  if (!query_config_supported(query_block, query_group))
+    return fail(GPJSON_ERR_INVALID_ARG, "query block must be 64, 128 or 256 and group 1, 4, 16 or 64");
   if (query_waves_per_cu < 1 || query_waves_per_cu > 64 || lines_per_wave < 1 || lines_per_wave > 1024)
     return fail(GPJSON_ERR_INVALID_ARG, "query shape out of range");
   GJ_HIP(hipSetDevice(ctx->device));
@@ -58,6 +59,7 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
   ctx->itune.block = index_block;
   ctx->itune.wpt = index_wpt;
   ctx->qtune.block = query_block;
+  ctx->qtune.group = query_group;
   ctx->qtune.waves_per_cu = query_waves_per_cu;
   ctx->qtune.lines_per_wave = lines_per_wave;
   return GPJSON_OK;
@@ -76,7 +78,6 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   const QueryTune keep_q = ctx->qtune;
   double best = 1e30;
   IndexTune best_i = keep_i;
-  gpjson_plan* plan = nullptr;
   char line[256];
   for (const IdxCand& c : kIdx) {
     ctx->itune.block = c.block;
@@ -113,18 +114,16 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   }
   ctx->itune = best_i;
   std::vector<QCand> qc;
-  for (int block : {64, 128, 256})
-    for (int wpc : {8, 16, 32})
-      for (int lpw : {1, 2, 4}) qc.push_back({block, wpc, lpw});
-  int rc = gpjson_plan_create(ctx, buf, irs, ir_lens, nq, &plan);
-  if (rc) {
-    ctx->itune = keep_i;
-    return rc;
-  }
+  for (int group : {1, 4, 16, 64})
+    for (int block : {64, 256})
+      for (int wpc : {16, 32})
+        for (int lpw : {1, 4}) qc.push_back({block, group, wpc, lpw});
+  int rc = GPJSON_OK;
   double bestq = 1e30;
   QueryTune best_q = keep_q;
   for (const QCand& c : qc) {
     ctx->qtune.block = c.block;
+    ctx->qtune.group = c.group;
     ctx->qtune.waves_per_cu = c.wpc;
     ctx->qtune.lines_per_wave = c.lpw;
     // a plan caches the launch shape: prepare one per candidate
@@ -142,23 +141,23 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     gpjson_plan_free(p);
     if (rc) break;
     double ms = sum / iterations;
-    std::snprintf(line, sizeof(line), "query block=%d waves_per_cu=%d lines_per_wave=%d: %.4f ms\n", c.block, c.wpc,
-                  c.lpw, ms);
+    std::snprintf(line, sizeof(line), "query block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
+                  c.block, c.group, c.wpc, c.lpw, ms);
     rep += line;
     if (ms < bestq) {
       bestq = ms;
       best_q = ctx->qtune;
     }
   }
-  gpjson_plan_free(plan);
   if (rc) {
     ctx->itune = keep_i;
     ctx->qtune = keep_q;
     return rc;
   }
   ctx->qtune = best_q;
-  std::snprintf(line, sizeof(line), "best index %dx%d (%.4f ms), query %d/%d/%d (%.4f ms)\n", best_i.block,
-                best_i.wpt, best, best_q.block, best_q.waves_per_cu, best_q.lines_per_wave, bestq);
+  std::snprintf(line, sizeof(line), "best index %dx%d (%.4f ms), query block %d group %d wpc %d lpg %d (%.4f ms)\n",
+                best_i.block, best_i.wpt, best, best_q.block, best_q.group, best_q.waves_per_cu, best_q.lines_per_wave,
+                bestq);
   rep += line;
   if (report && report_cap) {
     size_t n = std::min(report_cap - 1, rep.size());

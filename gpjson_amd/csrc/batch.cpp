@@ -198,7 +198,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.total_dev = b->total_dev;
   Q.counter = b->counter;
   Q.lines_per_wave = b->qtune.lines_per_wave;
-  GJ_HIP(query_launch(Q, b->qtune.block, b->query_grid, s));
+  GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s));
   if (e2) GJ_HIP(hipEventRecord(e2, s));
   return GPJSON_OK;
 }

@@ -84,7 +84,7 @@ struct TileAgg {
 };
 
 __device__ __forceinline__ Carry apply_agg(const TileAgg& a, const Carry& c) {
-  const M4& m = a.m[c.p & 1];
+  const M4 m = (c.p & 1) ? a.m[1] : a.m[0];
   Carry r;
   r.p = a.esc ? (a.esc - 2) : c.p;
   r.s = c.s ^ m.q;
@@ -96,11 +96,11 @@ __device__ __forceinline__ Carry apply_agg(const TileAgg& a, const Carry& c) {
 __device__ __forceinline__ TileAgg agg_cat(const TileAgg& x, const TileAgg& y) {
   TileAgg z;
   z.esc = y.esc ? y.esc : x.esc;
-#pragma unroll
-  for (int P = 0; P < 2; P++) {
-    int px = x.esc ? (x.esc - 2) : P;
-    z.m[P] = m_cat(x.m[P], y.m[px]);
-  }
+  // carry out of x for carry-in P: fixed by x unless x passes it through
+  const M4 y0 = x.esc == 3 ? y.m[1] : y.m[0];
+  const M4 y1 = x.esc == 2 ? y.m[0] : y.m[1];
+  z.m[0] = m_cat(x.m[0], y0);
+  z.m[1] = m_cat(x.m[1], y1);
   return z;
 }
 
@@ -138,38 +138,114 @@ __device__ void publish_inc(TileState* st, const Carry& c) {
   st_state(&st->inc1, w1);
 }
 
-// Exclusive carry of local tile lt of a file whose tiles start at st0.
-__device__ Carry lookback(TileState* st0, int64_t lt) {
-  TileAgg acc;
-  acc.esc = 0;
-  acc.m[0] = m_id();
-  acc.m[1] = m_id();
-  int64_t k = lt - 1;
+__device__ __forceinline__ TileAgg agg_id() {
+  TileAgg a;
+  a.esc = 0;
+  a.m[0] = m_id();
+  a.m[1] = m_id();
+  return a;
+}
+
+__device__ __forceinline__ TileAgg shfl_down_agg(const TileAgg& a, int off) {
+  TileAgg b;
+  b.esc = __shfl_down(a.esc, off);
+#pragma unroll
+  for (int P = 0; P < 2; P++) {
+    b.m[P].q = __shfl_down(a.m[P].q, off);
+    b.m[P].d0 = __shfl_down(a.m[P].d0, off);
+    b.m[P].tot = __shfl_down(a.m[P].tot, off);
+    b.m[P].nl = __shfl_down(a.m[P].nl, off);
+  }
+  return b;
+}
+
+__device__ __forceinline__ TileAgg shfl_agg(const TileAgg& a, int src) {
+  TileAgg b;
+  b.esc = __shfl(a.esc, src);
+#pragma unroll
+  for (int P = 0; P < 2; P++) {
+    b.m[P].q = __shfl(a.m[P].q, src);
+    b.m[P].d0 = __shfl(a.m[P].d0, src);
+    b.m[P].tot = __shfl(a.m[P].tot, src);
+    b.m[P].nl = __shfl(a.m[P].nl, src);
+  }
+  return b;
+}
+
+__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
+  unsigned lo = (unsigned)(uint64_t)v, hi = (unsigned)((uint64_t)v >> 32);
+  lo = __shfl(lo, src);
+  hi = __shfl(hi, src);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Lane l holds the aggregate of tile k_hi - l; returns (to every lane) the
+// composition of lanes [0, 64) in tile order, i.e. x_63 then ... then x_0.
+__device__ __forceinline__ TileAgg reduce_tiles(TileAgg x, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    TileAgg y = shfl_down_agg(x, off);
+    if (lane + off < 64) x = agg_cat(y, x);
+  }
+  return shfl_agg(x, 0);
+}
+
+// Exclusive carry of local tile lt of a file whose tiles start at st0,
+// resolved by one wavefront: each lane reads one predecessor's published state
+// per round trip, so up to 64 tiles are folded per step.  Tile 0 of a file
+// always publishes its inclusive prefix, so the walk ends there at the latest.
+__device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
+  TileAgg acc = agg_id();  // tiles (k_hi, lt) already folded
+  int64_t k_hi = lt - 1;
   for (;;) {
-    TileState* s = st0 + k;
-    unsigned long long i0 = ld_state(&s->inc0), i1 = ld_state(&s->inc1);
-    if ((i0 & kValid) && (i1 & kValid)) {
-      Carry c;
-      c.p = (int)((i0 >> 62) & 1);
-      c.s = (int)((i0 >> 61) & 1);
-      c.d = sext(i0 & 0xFFFFFFFFFFFFull, 48);
-      c.lines = (int64_t)(i1 & ~kValid);
-      return apply_agg(acc, c);
+    const int64_t k = k_hi - lane;
+    int status = 0;  // 0 not ready, 1 aggregate, 2 inclusive
+    TileAgg a = agg_id();
+    Carry c{0, 0, 0, 0};
+    if (k >= 0) {
+      TileState* s = st0 + k;
+      const unsigned long long i0 = ld_state(&s->inc0), i1 = ld_state(&s->inc1);
+      if ((i0 & kValid) && (i1 & kValid)) {
+        status = 2;
+        c.p = (int)((i0 >> 62) & 1);
+        c.s = (int)((i0 >> 61) & 1);
+        c.d = sext(i0 & 0xFFFFFFFFFFFFull, 48);
+        c.lines = (int64_t)(i1 & ~kValid);
+      } else {
+        const unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
+        if ((a0 & kValid) && (a1 & kValid) && (a2 & kValid)) {
+          status = 1;
+          a.esc = ((a0 >> 62) & 1) ? 0 : (2 + (int)((a0 >> 61) & 1));
+          const int nl = (int)(a0 & 0xFFFFFFFFu);
+          const int tot = (int)sext((a1 >> 31) & 0x7FFFFFFF, 31);
+          a.m[0] = M4{(int)((a0 >> 60) & 1), (int)sext(a1 & 0x7FFFFFFF, 31), tot, nl};
+          a.m[1] = M4{(int)((a0 >> 59) & 1), (int)sext(a2 & 0x7FFFFFFF, 31), tot, nl};
+        }
+      }
     }
-    unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
-    if ((a0 & kValid) && (a1 & kValid) && (a2 & kValid)) {
-      TileAgg a;
-      int allb = (int)((a0 >> 62) & 1);
-      a.esc = allb ? 0 : (2 + (int)((a0 >> 61) & 1));
-      int nl = (int)(a0 & 0xFFFFFFFFu);
-      int tot = (int)sext((a1 >> 31) & 0x7FFFFFFF, 31);
-      a.m[0] = M4{(int)((a0 >> 60) & 1), (int)sext(a1 & 0x7FFFFFFF, 31), tot, nl};
-      a.m[1] = M4{(int)((a0 >> 59) & 1), (int)sext(a2 & 0x7FFFFFFF, 31), tot, nl};
-      acc = agg_cat(a, acc);
-      k--;
-      continue;  // k >= 0: tile 0 of a file always publishes its inclusive prefix
+    const unsigned long long m_inc = __ballot(status == 2);
+    const unsigned long long m_nr = __ballot(status == 0);
+    const int a_inc = m_inc ? __builtin_ctzll(m_inc) : 64;
+    const int a_nr = m_nr ? __builtin_ctzll(m_nr) : 64;
+    if (a_inc < a_nr) {
+      // lanes [0, a_inc) are aggregates, lane a_inc the nearest inclusive prefix
+      if (lane >= a_inc) a = agg_id();
+      const TileAgg part = reduce_tiles(a, lane);
+      Carry base;
+      base.p = __shfl(c.p, a_inc);
+      base.s = __shfl(c.s, a_inc);
+      base.d = shfl_i64(c.d, a_inc);
+      base.lines = shfl_i64(c.lines, a_inc);
+      return apply_agg(agg_cat(part, acc), base);
     }
-    __builtin_amdgcn_s_sleep(1);
+    if (a_nr == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    // fold the ready aggregates [0, a_nr) and continue below them
+    if (lane >= a_nr) a = agg_id();
+    acc = agg_cat(reduce_tiles(a, lane), acc);
+    k_hi -= a_nr;
   }
 }
 
@@ -240,26 +316,43 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
     __syncthreads();
 
     // ---- classify this thread's words ----
-    WordClass wc[WPT];
+    // Each word's six masks replace its (already consumed) raw bytes in this
+    // thread's own LDS slots, so they do not occupy registers across the scan
+    // and lookback phases.
+    auto slot = [&](int c) { return tid * CH + (c ^ (tid & (CH - 1))); };
+    auto lo32 = [](uint64_t x) { return (uint32_t)x; };
+    auto hi32 = [](uint64_t x) { return (uint32_t)(x >> 32); };
+    auto u64 = [](uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; };
+    int code[WPT];  // escape summary per word: 0 all backslashes, else 2 + carry out
+    int tex = 0;
 #pragma unroll
     for (int k = 0; k < WPT; k++) {
       uint32_t d[16];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        uint4 v = tile[tid * CH + ((4 * k + q) ^ (tid & (CH - 1)))];
+        uint4 v = tile[slot(4 * k + q)];
         d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
       }
-      wc[k] = classify64(d);
-    }
-
-    // ---- escape carry: "last word that is not all backslashes" scan ----
-    int code[WPT];  // 0: all backslashes, else 2 + carry out
-    int tex = 0;
-#pragma unroll
-    for (int k = 0; k < WPT; k++) {
-      code[k] = (wc[k].B == ~0ull) ? 0 : (2 + (int)escape_carry_out(wc[k].B, 0));
+      const WordClass w = classify64(d);
+      tile[slot(4 * k + 0)] = make_uint4(lo32(w.B), hi32(w.B), lo32(w.Q), hi32(w.Q));
+      tile[slot(4 * k + 1)] = make_uint4(lo32(w.NL), hi32(w.NL), lo32(w.O), hi32(w.O));
+      tile[slot(4 * k + 2)] = make_uint4(lo32(w.C), hi32(w.C), lo32(w.P), hi32(w.P));
+      code[k] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
       if (code[k]) tex = code[k];
     }
+    auto load_word = [&](int k) {
+      const uint4 a = tile[slot(4 * k + 0)], b = tile[slot(4 * k + 1)], c = tile[slot(4 * k + 2)];
+      WordClass w;
+      w.B = u64(a.x, a.y);
+      w.Q = u64(a.z, a.w);
+      w.NL = u64(b.x, b.y);
+      w.O = u64(b.z, b.w);
+      w.C = u64(c.x, c.y);
+      w.P = u64(c.z, c.w);
+      return w;
+    };
+
+    // ---- escape carry: "last word that is not all backslashes" scan ----
     const unsigned long long nzm = __ballot(tex != 0);
     const unsigned long long below = nzm & ((1ull << lane) - 1ull);
     int ent = __shfl(tex, below ? (63 - __clzll(below)) : 0);
@@ -284,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       for (int k = 0; k < WPT; k++) {
         pcode[k] = run;
         int p = run ? (run - 2) : 0;
-        const WordClass& w = wc[k];
+        const WordClass w = load_word(k);
         uint64_t E = escape_word(w.B, (uint32_t)p);
         uint64_t Qe = w.Q & ~E;
         uint64_t S0 = prefix_xor(Qe);
@@ -328,29 +421,26 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
     }
     M4 texcl = m_cat(wpre, lex);
 
-    // ---- publish the tile aggregate and resolve the carry-in ----
-    if (tid == 0) {
+    // ---- publish the tile aggregate and resolve the carry-in (wave 0) ----
+    if (wave == 0) {
       TileState* st0 = launch.states + J.tile0;
-      Carry cin;
-      if (lt == 0) {
-        cin = Carry{0, 0, -1, 1};
-        if (J.nl_cap > 0) J.NL[0] = 0;
-      } else {
-        TileAgg a;
-        a.esc = tile_esc;
-        a.m[0] = mtile;
-        a.m[1] = mtile;
-        if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
-        publish_agg(st0 + lt, a, mtile.nl);
-        cin = lookback(st0, lt);
-      }
       TileAgg a;
       a.esc = tile_esc;
       a.m[0] = mtile;
       a.m[1] = mtile;
       if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
-      publish_inc(st0 + lt, apply_agg(a, cin));
-      s_carry = cin;
+      Carry cin;
+      if (lt == 0) {
+        cin = Carry{0, 0, -1, 1};
+        if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
+      } else {
+        if (lane == 0) publish_agg(st0 + lt, a, mtile.nl);
+        cin = lookback_wave(st0, lt, lane);
+      }
+      if (lane == 0) {
+        publish_inc(st0 + lt, apply_agg(a, cin));
+        s_carry = cin;
+      }
     }
     __syncthreads();
     const Carry X = s_carry;
@@ -372,7 +462,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       const int s = X.s ^ mx.q;
       const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
       const int64_t line = X.lines + mx.nl;
-      const WordClass& wd = wc[k];
+      const WordClass wd = load_word(k);
       uint64_t valid = ~0ull;
       const int64_t rem = J.N - w * 64;
       if (rem < 64) valid = (1ull << rem) - 1ull;

@@ -1,19 +1,22 @@
 // query_kernel.hip -- per-line JSONPath IR interpreter for gfx950.
 //
 // Semantics are those of the reference's find_value kernel
-// (K/find_value.cu:21-263, == J/sequential/Sequential.java:146-337): a line is
-// evaluated with a byte cursor j that only moves forward while the interpreter
-// waits for a structural event at the current level (a ':' whose key matches,
-// or the n-th ','), and IR ops run between events.
+// (K/find_value.cu:21-263, == J/sequential/Sequential.java:146-337; K/ =
+// gpjson/src/main/resources/com/koenv/gpjson/kernels/, J/ = gpjson/src/main/
+// java/com/koenv/gpjson/): a line is evaluated with a byte cursor j that only
+// moves forward while the interpreter waits for a structural event at the
+// current level (a ':' whose key matches, or the n-th ','); IR ops run between
+// events.
 //
-// Mapping: one wavefront evaluates one line.  The IR, the cursor and the
-// per-level state are wave-uniform.  The waits are the expensive part, and
-// they are done wave-parallel: each lane takes one 64-bit word of the current
-// level's bitmap (64 words = 4 KiB of text per step), checks its candidate
-// positions, and the earliest event wins by ballot -- so the first match in
-// byte order is the one taken, exactly as in the sequential loop.  Level ends
-// (MOVE_DOWN) are found the same way.  Wavefronts claim lines_per_wave lines
-// at a time from an atomic counter.
+// Mapping: a group of G lanes (G = 1, 4, 16 or 64) evaluates one line; the
+// IR, the cursor and the per-level state are uniform within the group.  A wait
+// scans the current level's bitmap G words (G*64 bytes of text) per step: each
+// lane tests the candidate positions in its word and the earliest event wins
+// by a group ballot, so the first match in byte order is the one taken, as in
+// the sequential loop.  Level ends (MOVE_DOWN) are found the same way.  Small
+// G keeps many lines in flight (latency-bound short lines); G = 64 suits long
+// lines.  Groups claim lines_per_wave lines at a time from an atomic counter.
+// Per-level state lives in LDS: levels+2 slots per group.
 #include <hip/hip_runtime.h>
 #include "index_common.h"
 #include "query_common.h"
@@ -22,29 +25,58 @@ namespace gj {
 
 namespace {
 
-constexpr int kLv = 65;  // per-level state slots (levels 0..64)
-
 __device__ __forceinline__ bool is_blank(uint8_t c) { return c == '\n' || c == '\r' || c == '\t' || c == ' '; }
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
 __device__ __forceinline__ int ctz64(unsigned long long x) { return __builtin_ctzll(x); }
+// Bits [lo_bit, 63] of a word.
+__device__ __forceinline__ uint64_t mask_from(int lo_bit) { return ~0ull << lo_bit; }
+// Bits [0, hi_bit] of a word.
+__device__ __forceinline__ uint64_t mask_upto(int hi_bit) { return hi_bit >= 63 ? ~0ull : ((2ull << hi_bit) - 1ull); }
 
-__device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int src) {
+__device__ __forceinline__ unsigned long long shfl64_lane(unsigned long long v, int src) {
   unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
   lo = __shfl(lo, src);
   hi = __shfl(hi, src);
   return ((unsigned long long)hi << 32) | lo;
 }
 
-__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
-  return (int64_t)shfl64((unsigned long long)v, src);
-}
-
-// Bits [lo_bit, 63] of a word.
-__device__ __forceinline__ uint64_t mask_from(int lo_bit) { return ~0ull << lo_bit; }
-// Bits [0, hi_bit] of a word.
-__device__ __forceinline__ uint64_t mask_upto(int hi_bit) { return hi_bit >= 63 ? ~0ull : ((2ull << hi_bit) - 1ull); }
+// Lanes [base, base+G) of the wavefront work on one line.
+template <int G>
+struct Group {
+  int lane, r, base;
+  __device__ Group() {
+    lane = __lane_id();
+    r = lane % G;
+    base = lane - r;
+  }
+  __device__ __forceinline__ unsigned long long ballot(bool p) const {
+    if (G == 1) return p ? 1ull : 0ull;
+    unsigned long long b = __ballot(p);
+    if (G == 64) return b;
+    return (b >> base) & ((1ull << G) - 1ull);
+  }
+  __device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) const {
+    if (G == 1) return v;
+    return (int64_t)shfl64_lane((unsigned long long)v, base + src);
+  }
+  __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) const {
+    if (G == 1) return v;
+    return shfl64_lane(v, base + src);
+  }
+  __device__ __forceinline__ int shfl(int v, int src) const {
+    if (G == 1) return v;
+    return __shfl(v, base + src);
+  }
+  // inclusive prefix sum over the group
+  __device__ __forceinline__ int scan_add(int v) const {
+    if (G == 1) return v;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      int y = __shfl_up(v, off, G);
+      if (r >= off) v += y;
+    }
+    return v;
+  }
+};
 
 // Word wi of a level row restricted to positions [from, to).
 __device__ __forceinline__ uint64_t window_word(const uint64_t* row, int64_t wi, int64_t wbeg, int64_t wend,
@@ -56,19 +88,19 @@ __device__ __forceinline__ uint64_t window_word(const uint64_t* row, int64_t wi,
   return v;
 }
 
-// First set bit of row in [from, to), or -1.  All lanes call with the same arguments.
-__device__ int64_t wave_first_bit(const uint64_t* row, int64_t from, int64_t to) {
+// First set bit of row in [from, to), or -1.
+template <int G>
+__device__ int64_t group_first_bit(const Group<G>& g, const uint64_t* row, int64_t from, int64_t to) {
   if (from >= to) return -1;
-  const int lane = lane_id();
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t base = wbeg; base <= wend; base += 64) {
-    int64_t wi = base + lane;
-    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
-    unsigned long long m = __ballot(v != 0);
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
+    const int64_t wi = b0 + g.r;
+    const uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    const unsigned long long m = g.ballot(v != 0);
     if (m) {
-      int l = ctz64(m);
-      uint64_t vv = shfl64(v, l);
-      return ((base + l) << 6) + (ctz64(vv));
+      const int l = ctz64(m);
+      const uint64_t vv = g.shfl_u64(v, l);
+      return ((b0 + l) << 6) + ctz64(vv);
     }
   }
   return -1;
@@ -79,12 +111,12 @@ __device__ int64_t wave_first_bit(const uint64_t* row, int64_t from, int64_t to)
 // after the line start, and those bytes must equal the key.
 __device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, int64_t p, const uint8_t* key,
                             int klen) {
-  int64_t k = p - 1;
+  const int64_t k = p - 1;
   if (k <= start) return false;
   int64_t wi = k >> 6;
   const int64_t wstart = start >> 6;
   uint64_t sw = S[wi] & mask_upto((int)(k & 63));
-  int64_t se = -1;
+  int64_t se;
   for (;;) {
     if (wi == wstart) sw &= ~mask_upto((int)(start & 63));
     if (sw) { se = (wi << 6) + 63 - __clzll(sw); break; }
@@ -92,7 +124,7 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, 
     wi--;
     sw = S[wi];
   }
-  int64_t ss = se - klen;
+  const int64_t ss = se - klen;
   if (ss < start || F[ss] != '"') return false;
   for (int i = 0; i < klen; i++)
     if (key[i] != F[ss + 1 + i]) return false;
@@ -100,69 +132,60 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, 
 }
 
 // First structural ':' in [from, to) at this level whose key matches, or -1.
-__device__ int64_t wave_find_key(const uint64_t* row, const uint8_t* F, const uint64_t* S, int64_t start,
-                                 int64_t from, int64_t to, const uint8_t* key, int klen) {
+template <int G>
+__device__ int64_t group_find_key(const Group<G>& g, const uint64_t* row, const uint8_t* F, const uint64_t* S,
+                                  int64_t start, int64_t from, int64_t to, const uint8_t* key, int klen) {
   if (from >= to) return -1;
-  const int lane = lane_id();
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t base = wbeg; base <= wend; base += 64) {
-    int64_t wi = base + lane;
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
+    const int64_t wi = b0 + g.r;
     uint64_t v = window_word(row, wi, wbeg, wend, from, to);
     int64_t found = -1;
     while (v) {
-      int b = ctz64(v);
-      int64_t p = (wi << 6) + b;
+      const int b = ctz64(v);
+      const int64_t p = (wi << 6) + b;
       if (F[p] == ':' && key_matches(F, S, start, p, key, klen)) { found = p; break; }
       v &= v - 1;
     }
-    unsigned long long m = __ballot(found >= 0);
-    if (m) return shfl_i64(found, ctz64(m));
+    const unsigned long long m = g.ballot(found >= 0);
+    if (m) return g.shfl_i64(found, ctz64(m));
   }
   return -1;
 }
 
-// Walks structural ',' in [from, to), incrementing *count per comma; returns the
-// position of the comma at which *count becomes want, or -1 (count advanced by
-// all commas in range).
-__device__ int64_t wave_find_comma(const uint64_t* row, const uint8_t* F, int64_t from, int64_t to, int* count,
-                                   int want) {
+// Walks structural ',' in [from, to), counting from *count; returns the
+// position of the comma at which the count becomes want, or -1 (count advanced
+// by every comma in range).
+template <int G>
+__device__ int64_t group_find_comma(const Group<G>& g, const uint64_t* row, const uint8_t* F, int64_t from,
+                                    int64_t to, int* count, int want) {
   if (from >= to) return -1;
-  const int lane = lane_id();
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
   int cnt0 = *count;
-  for (int64_t base = wbeg; base <= wend; base += 64) {
-    int64_t wi = base + lane;
-    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
+    const int64_t wi = b0 + g.r;
+    const uint64_t v = window_word(row, wi, wbeg, wend, from, to);
     uint64_t commas = 0;
-    uint64_t t = v;
-    while (t) {
-      int b = ctz64(t);
+    for (uint64_t t = v; t; t &= t - 1) {
+      const int b = ctz64(t);
       if (F[(wi << 6) + b] == ',') commas |= 1ull << b;
-      t &= t - 1;
     }
-    int c = __popcll(commas);
-    int incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int y = __shfl_up(incl, off);
-      if (lane >= off) incl += y;
-    }
-    int total = __shfl(incl, 63);
-    // the event is the first comma where cnt0 + (commas so far) == want
-    int need = want - cnt0;  // 1-based rank of the event comma in this window
-    bool hit = need >= 1 && need <= total && incl >= need && (incl - c) < need;
-    unsigned long long m = __ballot(hit);
+    const int c = __popcll(commas);
+    const int incl = g.scan_add(c);
+    const int total = g.shfl(incl, G - 1);
+    const int need = want - cnt0;  // 1-based rank of the event comma in this window
+    const bool hit = need >= 1 && need <= total && incl >= need && (incl - c) < need;
+    const unsigned long long m = g.ballot(hit);
     if (m) {
-      int l = ctz64(m);
+      const int l = ctz64(m);
       int64_t pos = -1;
-      if (lane == l) {
-        int rank = need - (incl - c);
+      if (g.r == l) {
         uint64_t cm = commas;
-        for (int r = 1; r < rank; r++) cm &= cm - 1;
-        pos = (wi << 6) + (ctz64(cm));
+        for (int k = 1; k < need - (incl - c); k++) cm &= cm - 1;
+        pos = (wi << 6) + ctz64(cm);
       }
       *count = want;
-      return shfl_i64(pos, l);
+      return g.shfl_i64(pos, l);
     }
     cnt0 += total;
   }
@@ -173,7 +196,7 @@ __device__ int64_t wave_find_comma(const uint64_t* row, const uint8_t* F, int64_
 __device__ __forceinline__ int read_varint(const uint8_t* ir, int64_t ir_len, int64_t* pc) {
   int v = 0, shift = 0;
   while (*pc < ir_len) {
-    uint8_t b = ir[(*pc)++];
+    const uint8_t b = ir[(*pc)++];
     v |= (int)(b & 0x7F) << shift;
     if (!(b & 0x80)) break;
     shift += 7;
@@ -182,20 +205,21 @@ __device__ __forceinline__ int read_varint(const uint8_t* ir, int64_t ir_len, in
   return v;
 }
 
-// Evaluates one line of one job.  le/ci are this wavefront's LDS slots.
-__device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* out, int64_t* le, int* ci) {
-  const int lane = lane_id();
+// Evaluates line i of job J with group g.  le/ci: this group's nslots level
+// slots in LDS, written by the group's first lane and read by all of them (LDS
+// operations of one wavefront complete in program order).
+template <int G>
+__device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, int64_t i, int64_t* out, int64_t* le,
+                          int* ci, int nslots) {
+  const bool lead = g.r == 0;
   const uint8_t* F = J.F;
   const int64_t n = J.N;
   const int64_t start = J.NL[i];
   const int64_t end = (i + 1 < lines) ? J.NL[i + 1] : n;
-  if (lane == 0)
+  if (lead)
     for (int k = 0; k < J.R * 2; k++) out[k] = -1;
-  // Per-level state lives in this wavefront's LDS slots.  They are written by
-  // lane 0 (lane-parallel for the reset) and read by every lane; LDS operations
-  // of one wavefront complete in program order, so a later read sees the write.
-  for (int k = lane; k < kLv; k += 64) { le[k] = -1; ci[k] = -1; }
-  if (lane == 0) le[0] = end;
+  for (int k = g.r; k < nslots; k += G) { le[k] = -1; ci[k] = -1; }
+  if (lead) le[0] = end;
 
   int lvl = 0;
   int64_t j = start;
@@ -208,8 +232,7 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
 
   for (;;) {
     if (lvl < 0) return;
-    int64_t lim = le[lvl];
-    if (!(j < lim && j < n)) return;
+    if (!(j < le[lvl] && j < n)) return;
     while (exec) {
       if (pc >= J.ir_len) return;
       op = J.ir[pc++];
@@ -218,7 +241,7 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
           return;
         case kStoreResult: {
           while (j < n && is_blank(F[j])) j++;
-          if (r < J.R && lane == 0) {
+          if (r < J.R && lead) {
             out[r * 2] = j;
             out[r * 2 + 1] = le[lvl];
           }
@@ -227,17 +250,17 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
         }
         case kMoveUp:
           j = le[lvl];
-          if (lane == 0) le[lvl] = -1;
+          if (lead) le[lvl] = -1;
           lvl--;
           if (lvl < 0) return;
           break;
         case kMoveDown: {
           lvl++;
-          if (lvl >= kLv) return;
+          if (lvl >= nslots) return;
           if (le[lvl] == -1) {
             int64_t e = -1;
-            if (lvl - 1 < J.L) e = wave_first_bit(J.LB + (int64_t)(lvl - 1) * J.Wp, j + 1, le[lvl - 1]);
-            if (lane == 0) le[lvl] = e;
+            if (lvl - 1 < J.L) e = group_first_bit(g, J.LB + (int64_t)(lvl - 1) * J.Wp, j + 1, le[lvl - 1]);
+            if (lead) le[lvl] = e;
           }
           break;
         }
@@ -255,7 +278,7 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
               j++;
             }
             if (j >= n) return;
-            if (lane == 0) ci[lvl] = 0;
+            if (lead) ci[lvl] = 0;
           }
           if (want_idx == 0) { j++; exec = true; } else { exec = false; }
           break;
@@ -265,8 +288,7 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
           want = J.ir + pc;
           pc += want_len;
           while (j < n && is_blank(F[j])) j++;
-          int64_t actual = le[lvl] - j;
-          if (actual != want_len) return;
+          if (le[lvl] - j != want_len) return;
           for (int k = 0; k < want_len; k++)
             if (want[k] != F[j + k]) return;
           break;
@@ -277,17 +299,17 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
     }
     if (!(j < n)) return;
     // wait for the next event at this level
-    lim = le[lvl];
-    int64_t to = lim < n ? lim : n;
+    const int64_t lim = le[lvl];
+    const int64_t to = lim < n ? lim : n;
     int64_t p = -1;
     if (lvl < J.L) {
       const uint64_t* row = J.LB + (int64_t)lvl * J.Wp;
       if (op == kMoveToKey) {
-        p = wave_find_key(row, F, J.S, start, j, to, want, want_len);
+        p = group_find_key(g, row, F, J.S, start, j, to, want, want_len);
       } else if (op == kMoveToIndex) {
         int cnt = ci[lvl];
-        p = wave_find_comma(row, F, j, to, &cnt, want_idx);
-        if (lane == 0) ci[lvl] = cnt;
+        p = group_find_comma(g, row, F, j, to, &cnt, want_idx);
+        if (lead) ci[lvl] = cnt;
       }
     }
     if (p < 0) return;
@@ -298,59 +320,44 @@ __device__ void eval_line(const QueryJob& J, int64_t lines, int64_t i, int64_t* 
 
 }  // namespace
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch) {
-  constexpr int NW = BLOCK / 64;
-  __shared__ int64_t s_le[NW][kLv];
-  __shared__ int s_ci[NW][kLv];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+template <int BLOCK, int G>
+__global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int nslots) {
+  constexpr int NG = BLOCK / G;
+  extern __shared__ int64_t q_lds[];
+  const Group<G> g;
+  const int grp = threadIdx.x / G;
+  int64_t* le = q_lds + (int64_t)grp * nslots;
+  int* ci = reinterpret_cast<int*>(q_lds + (int64_t)NG * nslots) + (int64_t)grp * nslots;
   const int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
   const int64_t total = launch.lines_dev ? common * launch.njobs
                                          : (launch.total_dev ? *launch.total_dev : launch.total_items);
   for (;;) {
     unsigned long long g0 = 0;
-    if (lane == 0) g0 = atomicAdd(launch.counter, (unsigned long long)lpw);
-    g0 = shfl64(g0, 0);
+    if (g.r == 0) g0 = atomicAdd(launch.counter, (unsigned long long)lpw);
+    g0 = (unsigned long long)g.shfl_i64((int64_t)g0, 0);
     if ((int64_t)g0 >= total) break;
     if (launch.lines_dev) {
-      for (int64_t g = (int64_t)g0; g < (int64_t)g0 + lpw && g < total; g++) {
-        const int64_t q = g / common, i = g - q * common;
+      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+        const int64_t q = it / common, i = it - q * common;
         const QueryJob& J = launch.jobs[q];
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(J, common, i, out, s_le[wave], s_ci[wave]);
+        eval_line(g, J, common, i, out, le, ci, nslots);
       }
       continue;
     }
-    // job of the first item
     int lo = 0, hi = (int)launch.njobs - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
+    while (lo < hi) {  // job of the first item
+      const int mid = (lo + hi + 1) >> 1;
       if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
     }
-    for (int64_t g = (int64_t)g0; g < (int64_t)g0 + lpw && g < total; g++) {
-      while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= g) lo++;
+    for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+      while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
       const QueryJob& J = launch.jobs[lo];
-      const int64_t i = g - J.item0;
-      eval_line(J, J.lines, i, J.out + i * (int64_t)J.R * 2, s_le[wave], s_ci[wave]);
+      const int64_t i = it - J.item0;
+      eval_line(g, J, J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots);
     }
   }
-}
-
-__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-hipError_t query_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream) {
-  switch (block) {
-    case 64: hipLaunchKernelGGL(query_kernel<64>, dim3(grid), dim3(64), 0, stream, L); break;
-    case 128: hipLaunchKernelGGL(query_kernel<128>, dim3(grid), dim3(128), 0, stream, L); break;
-    case 256: hipLaunchKernelGGL(query_kernel<256>, dim3(grid), dim3(256), 0, stream, L); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
 }
 
 // Batched runs: fills each query job's line count and first work item from the
@@ -385,6 +392,45 @@ hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states,
   hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, states, qjobs, job_file, njobs,
                      total_out, lines_out, overflow);
   return hipGetLastError();
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+bool query_config_supported(int block, int group) {
+  return (block == 64 || block == 128 || block == 256) && (group == 1 || group == 4 || group == 16 || group == 64);
+}
+
+template <int BLOCK, int G>
+static hipError_t launch_bg(const QueryLaunch& L, int nslots, int grid, hipStream_t stream) {
+  const size_t lds = (size_t)(BLOCK / G) * nslots * 12;
+  hipLaunchKernelGGL((query_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), lds, stream, L, nslots);
+  return hipGetLastError();
+}
+
+template <int BLOCK>
+static hipError_t launch_b(const QueryLaunch& L, int group, int nslots, int grid, hipStream_t stream) {
+  switch (group) {
+    case 1: return launch_bg<BLOCK, 1>(L, nslots, grid, stream);
+    case 4: return launch_bg<BLOCK, 4>(L, nslots, grid, stream);
+    case 16: return launch_bg<BLOCK, 16>(L, nslots, grid, stream);
+    case 64: return launch_bg<BLOCK, 64>(L, nslots, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// levels: the deepest level any job's IR reaches (slots levels+2 per group).
+hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream) {
+  int nslots = levels + 2;
+  if (nslots > 66) nslots = 66;
+  switch (block) {
+    case 64: return launch_b<64>(L, group, nslots, grid, stream);
+    case 128: return launch_b<128>(L, group, nslots, grid, stream);
+    case 256: return launch_b<256>(L, group, nslots, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream) {

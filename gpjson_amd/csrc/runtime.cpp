@@ -239,6 +239,9 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.counter = (unsigned long long*)((uint8_t*)ctx->ticket.p + 64);
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   out->launch = L;
+  out->block = ctx->qtune.block;
+  out->group = ctx->qtune.group;
+  out->levels = idx->L;
   const int waves_per_block = ctx->qtune.block / 64;
   out->grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   return GPJSON_OK;
@@ -247,7 +250,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel, hipEvent_t after_kernel) {
   GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  GJ_HIP(query_launch(pq.launch, ctx->qtune.block, pq.grid, ctx->stream));
+  GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
   return GPJSON_OK;
 }

@@ -19,7 +19,8 @@ hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, 
 hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
 bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
-hipError_t query_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream);
+hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream);
+bool query_config_supported(int block, int group);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
                       int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream);
@@ -41,8 +42,9 @@ struct IndexTune {
 
 struct QueryTune {
   int block = 256;
-  int waves_per_cu = 16;  // persistent grid size factor
-  int lines_per_wave = 1;
+  int group = 4;          // lanes per line
+  int waves_per_cu = 32;  // persistent grid size factor
+  int lines_per_wave = 1; // lines a group claims per atomic
 };
 
 int nlv_for_levels(int L);
@@ -102,6 +104,9 @@ namespace gj {
 struct PreparedQueries {
   QueryLaunch launch{};
   int grid = 0;
+  int block = 256;
+  int group = 4;
+  int levels = 0;
 };
 int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs, const int64_t* ir_lens,
                     int nq, int R, int64_t* spans_dev, PreparedQueries* out);
