@@ -83,7 +83,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
-    ap.add_argument("--config", default="", help="index_block,index_wpt,query_block,query_group,wpc,lpg")
+    ap.add_argument("--config", default="", help="index_block,index_wpt,query_tiled,query_block,query_group,wpc,lpg")
     args = ap.parse_args()
 
     if args.gpus != 1:

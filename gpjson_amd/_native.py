@@ -82,8 +82,8 @@ SIGNATURES = {
     "gpjson_batch_results_per_line": (c_int, [c_void_p, c_int, c_int]),
     "gpjson_batch_spans": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64]),
     "gpjson_batch_free": (None, [c_void_p]),
-    "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
-    "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int]),
+    "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
+    "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),
     "gpjson_time_plain": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, P(c_double),
                                   P(c_double), P(c_double)]),

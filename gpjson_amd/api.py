@@ -189,14 +189,15 @@ class Context:
         return self._adopt(Batch(self, h, list(bufs), [len(irs) for irs in irs_per_file]))
 
     def get_config(self) -> dict:
-        v = [ctypes.c_int() for _ in range(6)]
+        v = [ctypes.c_int() for _ in range(7)]
         _check(self._lib.gpjson_get_config(self._h, *[ctypes.byref(x) for x in v]))
-        keys = ("index_block", "index_wpt", "query_block", "query_group", "query_waves_per_cu", "lines_per_group")
+        keys = ("index_block", "index_wpt", "query_tiled", "query_block", "query_group", "query_waves_per_cu",
+                "lines_per_group")
         return dict(zip(keys, [x.value for x in v]))
 
-    def set_config(self, index_block: int, index_wpt: int, query_block: int, query_group: int,
+    def set_config(self, index_block: int, index_wpt: int, query_tiled: int, query_block: int, query_group: int,
                    query_waves_per_cu: int, lines_per_group: int) -> None:
-        _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_block, query_group,
+        _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_tiled, query_block, query_group,
                                            query_waves_per_cu, lines_per_group))
 
     def autotune(self, buf: "Buffer", irs: Sequence[bytes], iterations: int = 3) -> str:

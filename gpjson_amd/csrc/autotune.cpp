@@ -25,16 +25,17 @@ struct IdxCand {
 const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
 
 struct QCand {
-  int block, group, wpc, lpw;
+  int tiled, block, group, wpc, lpw;
 };
 
 }  // namespace
 
 extern "C" {
 
-int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_block, int* query_group,
-                      int* query_waves_per_cu, int* lines_per_wave) {
+int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_tiled, int* query_block,
+                      int* query_group, int* query_waves_per_cu, int* lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
+  if (query_tiled) *query_tiled = ctx->qtune.tiled;
   if (index_block) *index_block = ctx->itune.block;
   if (index_wpt) *index_wpt = ctx->itune.wpt;
   if (query_block) *query_block = ctx->qtune.block;
@@ -44,8 +45,8 @@ int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* qu
   return GPJSON_OK;
 }
 
-int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_block, int query_group,
-                      int query_waves_per_cu, int lines_per_wave) {
+int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_tiled, int query_block,
+                      int query_group, int query_waves_per_cu, int lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
   if (!index_config_supported(index_block, index_wpt))
     return fail(GPJSON_ERR_INVALID_ARG, "unsupported index shape " + std::to_string(index_block) + "x" +
@@ -58,6 +59,7 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
   GJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->itune.block = index_block;
   ctx->itune.wpt = index_wpt;
+  ctx->qtune.tiled = query_tiled ? 1 : 0;
   ctx->qtune.block = query_block;
   ctx->qtune.group = query_group;
   ctx->qtune.waves_per_cu = query_waves_per_cu;
@@ -114,17 +116,19 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   }
   ctx->itune = best_i;
   std::vector<QCand> qc;
+  for (int group : {1, 4, 16})
+    for (int block : {128, 256}) qc.push_back({1, block, group, 0, 1});
   for (int group : {1, 4, 16, 64})
-    for (int block : {64, 256})
-      for (int wpc : {16, 32})
-        for (int lpw : {1, 4}) qc.push_back({block, group, wpc, lpw});
+    for (int wpc : {16, 32})
+      for (int lpw : {1, 4}) qc.push_back({0, 256, group, wpc, lpw});
   int rc = GPJSON_OK;
   double bestq = 1e30;
   QueryTune best_q = keep_q;
   for (const QCand& c : qc) {
+    ctx->qtune.tiled = c.tiled;
     ctx->qtune.block = c.block;
     ctx->qtune.group = c.group;
-    ctx->qtune.waves_per_cu = c.wpc;
+    if (c.wpc) ctx->qtune.waves_per_cu = c.wpc;
     ctx->qtune.lines_per_wave = c.lpw;
     // a plan caches the launch shape: prepare one per candidate
     gpjson_plan* p = nullptr;
@@ -141,8 +145,8 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     gpjson_plan_free(p);
     if (rc) break;
     double ms = sum / iterations;
-    std::snprintf(line, sizeof(line), "query block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
-                  c.block, c.group, c.wpc, c.lpw, ms);
+    std::snprintf(line, sizeof(line), "query tiled=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
+                  c.tiled, c.block, c.group, c.wpc, c.lpw, ms);
     rep += line;
     if (ms < bestq) {
       bestq = ms;
@@ -155,9 +159,10 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     return rc;
   }
   ctx->qtune = best_q;
-  std::snprintf(line, sizeof(line), "best index %dx%d (%.4f ms), query block %d group %d wpc %d lpg %d (%.4f ms)\n",
-                best_i.block, best_i.wpt, best, best_q.block, best_q.group, best_q.waves_per_cu, best_q.lines_per_wave,
-                bestq);
+  std::snprintf(line, sizeof(line),
+                "best index %dx%d (%.4f ms), query tiled %d block %d group %d wpc %d lpg %d (%.4f ms)\n", best_i.block,
+                best_i.wpt, best, best_q.tiled, best_q.block, best_q.group, best_q.waves_per_cu,
+                best_q.lines_per_wave, bestq);
   rep += line;
   if (report && report_cap) {
     size_t n = std::min(report_cap - 1, rep.size());

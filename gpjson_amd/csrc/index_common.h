@@ -31,6 +31,7 @@ struct FileJob {
   int64_t nl_cap;
   uint64_t* E;          // escape index (verify mode, may be null)
   uint64_t* Q;          // quote index (verify mode, may be null)
+  int64_t* tile_lines;  // optional: first line id starting in each tile, [ntiles + 1]
   int64_t tile0;        // first global tile id of this file
   int64_t ntiles;
   int L;                // levels to write

@@ -440,6 +440,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       if (lane == 0) {
         publish_inc(st0 + lt, apply_agg(a, cin));
         s_carry = cin;
+        if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
       }
     }
     __syncthreads();
@@ -519,11 +520,13 @@ __global__ void index_lines_kernel(const FileJob* jobs, FileJob one, int64_t njo
   const FileJob& J = jobs ? jobs[i] : one;
   if (J.ntiles == 0) {  // empty file: one line starting at 0
     if (J.nl_cap > 0) J.NL[0] = 0;
+    if (J.tile_lines) J.tile_lines[0] = 1;
     lines_out[i] = 1;
     return;
   }
   unsigned long long v = states[J.tile0 + J.ntiles - 1].inc1;
   lines_out[i] = (int64_t)(v & ~kValid);
+  if (J.tile_lines) J.tile_lines[J.ntiles] = lines_out[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +558,7 @@ static hipError_t launch_nlv(const IndexLaunch& L, int nlv, bool verify, int gri
 }
 
 int64_t index_tile_bytes(int block, int wpt) { return (int64_t)block * wpt * 64; }
+// smallest supported tile: 128 x 2 words = 16 KiB (same as 256 x 1)
 
 bool index_config_supported(int block, int wpt) {
   return (block == 256 && (wpt == 1 || wpt == 2 || wpt == 4)) || (block == 512 && (wpt == 1 || wpt == 2)) ||

@@ -30,7 +30,11 @@ struct QueryJob {
   const uint8_t* ir;  // device copy of the IR bytes
   int64_t ir_len;
   int64_t* out;       // int64[lines][R][2]
-  int64_t item0;      // first global work item (line) of this job
+  int64_t item0;      // first global work item (line, or tile in tiled mode) of this job
+  // tiled mode: lines starting in index tile t are [tile_lines[t], tile_lines[t+1])
+  const int64_t* tile_lines;
+  int64_t ntiles;
+  int64_t tile_bytes;
 };
 
 struct QueryLaunch {
@@ -47,6 +51,9 @@ struct QueryLaunch {
   const int64_t* total_dev;
   unsigned long long* counter;    // zeroed before launch
   int lines_per_wave;             // lines a wavefront claims per atomic
+  // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
+  // tile prefix); a block stages a tile's leveled and string words in LDS.
+  int tiled;
 };
 
 }  // namespace gj

@@ -78,24 +78,44 @@ struct Group {
   }
 };
 
-// Word wi of a level row restricted to positions [from, to).
-__device__ __forceinline__ uint64_t window_word(const uint64_t* row, int64_t wi, int64_t wbeg, int64_t wend,
+// Where the leveled and string words come from: device memory, or an LDS copy
+// of the words [wlo, wlo + maxw) of the current tile.
+struct GlobalWords {
+  const uint64_t* LB;
+  int64_t Wp;
+  const uint64_t* S;
+  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return LB[(int64_t)lvl * Wp + wi]; }
+  __device__ __forceinline__ uint64_t s(int64_t wi) const { return S[wi]; }
+};
+
+struct LdsWords {
+  const uint64_t* w;  // [L + 1][maxw]: levels, then the string index
+  int64_t wlo;
+  int maxw;
+  int L;
+  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return w[lvl * maxw + (int)(wi - wlo)]; }
+  __device__ __forceinline__ uint64_t s(int64_t wi) const { return w[L * maxw + (int)(wi - wlo)]; }
+};
+
+// Word wi of level lvl restricted to positions [from, to).
+template <class Words>
+__device__ __forceinline__ uint64_t window_word(const Words& W, int lvl, int64_t wi, int64_t wbeg, int64_t wend,
                                                 int64_t from, int64_t to) {
   if (wi > wend) return 0;
-  uint64_t v = row[wi];
+  uint64_t v = W.lb(lvl, wi);
   if (wi == wbeg) v &= mask_from((int)(from & 63));
   if (wi == wend) v &= mask_upto((int)((to - 1) & 63));
   return v;
 }
 
-// First set bit of row in [from, to), or -1.
-template <int G>
-__device__ int64_t group_first_bit(const Group<G>& g, const uint64_t* row, int64_t from, int64_t to) {
+// First set bit of level lvl in [from, to), or -1.
+template <int G, class Words>
+__device__ int64_t group_first_bit(const Group<G>& g, const Words& W, int lvl, int64_t from, int64_t to) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
   for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
     const int64_t wi = b0 + g.r;
-    const uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    const uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
     const unsigned long long m = g.ballot(v != 0);
     if (m) {
       const int l = ctz64(m);
@@ -109,20 +129,21 @@ __device__ int64_t group_first_bit(const Group<G>& g, const uint64_t* row, int64
 // Key test at a ':' at position p (K/find_value.cu:227-250): the last in-string
 // byte before p must end a string of exactly klen bytes that starts with '"'
 // after the line start, and those bytes must equal the key.
-__device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, int64_t p, const uint8_t* key,
+template <class Words>
+__device__ bool key_matches(const uint8_t* F, const Words& W, int64_t start, int64_t p, const uint8_t* key,
                             int klen) {
   const int64_t k = p - 1;
   if (k <= start) return false;
   int64_t wi = k >> 6;
   const int64_t wstart = start >> 6;
-  uint64_t sw = S[wi] & mask_upto((int)(k & 63));
+  uint64_t sw = W.s(wi) & mask_upto((int)(k & 63));
   int64_t se;
   for (;;) {
     if (wi == wstart) sw &= ~mask_upto((int)(start & 63));
     if (sw) { se = (wi << 6) + 63 - __clzll(sw); break; }
     if (wi <= wstart) return false;
     wi--;
-    sw = S[wi];
+    sw = W.s(wi);
   }
   const int64_t ss = se - klen;
   if (ss < start || F[ss] != '"') return false;
@@ -132,19 +153,19 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* S, int64_t start, 
 }
 
 // First structural ':' in [from, to) at this level whose key matches, or -1.
-template <int G>
-__device__ int64_t group_find_key(const Group<G>& g, const uint64_t* row, const uint8_t* F, const uint64_t* S,
-                                  int64_t start, int64_t from, int64_t to, const uint8_t* key, int klen) {
+template <int G, class Words>
+__device__ int64_t group_find_key(const Group<G>& g, const Words& W, int lvl, const uint8_t* F, int64_t start,
+                                  int64_t from, int64_t to, const uint8_t* key, int klen) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
   for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
     const int64_t wi = b0 + g.r;
-    uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
     int64_t found = -1;
     while (v) {
       const int b = ctz64(v);
       const int64_t p = (wi << 6) + b;
-      if (F[p] == ':' && key_matches(F, S, start, p, key, klen)) { found = p; break; }
+      if (F[p] == ':' && key_matches(F, W, start, p, key, klen)) { found = p; break; }
       v &= v - 1;
     }
     const unsigned long long m = g.ballot(found >= 0);
@@ -156,15 +177,15 @@ __device__ int64_t group_find_key(const Group<G>& g, const uint64_t* row, const 
 // Walks structural ',' in [from, to), counting from *count; returns the
 // position of the comma at which the count becomes want, or -1 (count advanced
 // by every comma in range).
-template <int G>
-__device__ int64_t group_find_comma(const Group<G>& g, const uint64_t* row, const uint8_t* F, int64_t from,
+template <int G, class Words>
+__device__ int64_t group_find_comma(const Group<G>& g, const Words& W, int lvl, const uint8_t* F, int64_t from,
                                     int64_t to, int* count, int want) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
   int cnt0 = *count;
   for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
     const int64_t wi = b0 + g.r;
-    const uint64_t v = window_word(row, wi, wbeg, wend, from, to);
+    const uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
     uint64_t commas = 0;
     for (uint64_t t = v; t; t &= t - 1) {
       const int b = ctz64(t);
@@ -208,9 +229,9 @@ __device__ __forceinline__ int read_varint(const uint8_t* ir, int64_t ir_len, in
 // Evaluates line i of job J with group g.  le/ci: this group's nslots level
 // slots in LDS, written by the group's first lane and read by all of them (LDS
 // operations of one wavefront complete in program order).
-template <int G>
-__device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, int64_t i, int64_t* out, int64_t* le,
-                          int* ci, int nslots) {
+template <int G, class Words>
+__device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, const uint8_t* ir, int64_t lines,
+                          int64_t i, int64_t* out, int64_t* le, int* ci, int nslots) {
   const bool lead = g.r == 0;
   const uint8_t* F = J.F;
   const int64_t n = J.N;
@@ -235,7 +256,7 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, i
     if (!(j < le[lvl] && j < n)) return;
     while (exec) {
       if (pc >= J.ir_len) return;
-      op = J.ir[pc++];
+      op = ir[pc++];
       switch (op) {
         case kEnd:
           return;
@@ -259,19 +280,19 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, i
           if (lvl >= nslots) return;
           if (le[lvl] == -1) {
             int64_t e = -1;
-            if (lvl - 1 < J.L) e = group_first_bit(g, J.LB + (int64_t)(lvl - 1) * J.Wp, j + 1, le[lvl - 1]);
+            if (lvl - 1 < J.L) e = group_first_bit(g, W, lvl - 1, j + 1, le[lvl - 1]);
             if (lead) le[lvl] = e;
           }
           break;
         }
         case kMoveToKey:
-          want_len = read_varint(J.ir, J.ir_len, &pc);
-          want = J.ir + pc;
+          want_len = read_varint(ir, J.ir_len, &pc);
+          want = ir + pc;
           pc += want_len;
           exec = false;
           break;
         case kMoveToIndex: {
-          want_idx = read_varint(J.ir, J.ir_len, &pc);
+          want_idx = read_varint(ir, J.ir_len, &pc);
           if (ci[lvl] == -1) {
             while (j < n && F[j] != '[') {
               if (!is_blank(F[j])) return;  // not an array: the line has no further results
@@ -284,8 +305,8 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, i
           break;
         }
         case kExpressionStringEquals: {
-          want_len = read_varint(J.ir, J.ir_len, &pc);
-          want = J.ir + pc;
+          want_len = read_varint(ir, J.ir_len, &pc);
+          want = ir + pc;
           pc += want_len;
           while (j < n && is_blank(F[j])) j++;
           if (le[lvl] - j != want_len) return;
@@ -303,12 +324,11 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, int64_t lines, i
     const int64_t to = lim < n ? lim : n;
     int64_t p = -1;
     if (lvl < J.L) {
-      const uint64_t* row = J.LB + (int64_t)lvl * J.Wp;
       if (op == kMoveToKey) {
-        p = group_find_key(g, row, F, J.S, start, j, to, want, want_len);
+        p = group_find_key(g, W, lvl, F, start, j, to, want, want_len);
       } else if (op == kMoveToIndex) {
         int cnt = ci[lvl];
-        p = group_find_comma(g, row, F, j, to, &cnt, want_idx);
+        p = group_find_comma(g, W, lvl, F, j, to, &cnt, want_idx);
         if (lead) ci[lvl] = cnt;
       }
     }
@@ -342,7 +362,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
         const int64_t q = it / common, i = it - q * common;
         const QueryJob& J = launch.jobs[q];
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(g, J, common, i, out, le, ci, nslots);
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, J.ir, common, i, out, le, ci, nslots);
       }
       continue;
     }
@@ -355,8 +375,72 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
-      eval_line(g, J, J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots);
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, J.ir, J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots);
     }
+  }
+}
+
+
+// Tiled interpreter: a block takes one index tile of one job, copies the
+// leveled words of levels [0, L) and the string words covering the tile's
+// lines (from the first line's start to the last line's end) into LDS with
+// coalesced loads, and evaluates those lines from LDS.  Input bytes are read
+// from device memory (only around candidate keys).  A tile whose lines span
+// more than maxw words is evaluated straight from device memory.
+template <int BLOCK, int G>
+__global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, int nslots, int maxw) {
+  constexpr int NG = BLOCK / G;
+  constexpr int kIrCap = 512;
+  extern __shared__ int64_t q_lds[];
+  __shared__ uint8_t s_ir[kIrCap];
+  __shared__ unsigned long long s_item;
+  const Group<G> g;
+  const int grp = threadIdx.x / G;
+  int64_t* le = q_lds + (int64_t)grp * nslots;
+  int* ci = reinterpret_cast<int*>(q_lds + (int64_t)NG * nslots) + (int64_t)grp * nslots;
+  uint64_t* words = reinterpret_cast<uint64_t*>(q_lds + (int64_t)NG * nslots + ((int64_t)NG * nslots + 1) / 2);
+  const int64_t total = launch.total_items;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(launch.counter, 1ull);
+    __syncthreads();
+    const int64_t item = (int64_t)s_item;
+    if (item >= total) break;
+    int lo = 0, hi = (int)launch.njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (launch.jobs[mid].item0 <= item) lo = mid; else hi = mid - 1;
+    }
+    const QueryJob& J = launch.jobs[lo];
+    const int64_t t = item - J.item0;
+    const int64_t la = J.tile_lines[t], lb = J.tile_lines[t + 1];
+    const int64_t lines = J.tile_lines[J.ntiles];
+    const uint8_t* ir = J.ir;
+    if (J.ir_len <= kIrCap) {
+      for (int k = threadIdx.x; k < J.ir_len; k += BLOCK) s_ir[k] = J.ir[k];
+      ir = s_ir;
+    }
+    if (la < lb) {
+      const int64_t lo_b = la == 0 ? 0 : J.NL[la];
+      const int64_t hi_b = lb < lines ? J.NL[lb] : J.N;
+      const int64_t wlo = lo_b >> 6;
+      const int64_t nw = hi_b > lo_b ? ((hi_b - 1) >> 6) - wlo + 1 : 1;
+      const bool staged = nw <= maxw;
+      if (staged) {
+        for (int lv = 0; lv <= J.L; lv++) {
+          const uint64_t* src = lv < J.L ? J.LB + (int64_t)lv * J.Wp : J.S;
+          for (int64_t k = threadIdx.x; k < nw; k += BLOCK) words[lv * maxw + k] = src[wlo + k];
+        }
+      }
+      __syncthreads();
+      const LdsWords LW{words, wlo, maxw, J.L};
+      const GlobalWords GW{J.LB, J.Wp, J.S};
+      for (int64_t i = la + grp; i < lb; i += NG) {
+        int64_t* out = J.out + i * (int64_t)J.R * 2;
+        if (staged) eval_line(g, J, LW, ir, lines, i, out, le, ci, nslots);
+        else eval_line(g, J, GW, ir, lines, i, out, le, ci, nslots);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -374,6 +458,7 @@ __global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileSt
     if (J.ntiles > 0) lines = (int64_t)(states[J.tile0 + J.ntiles - 1].inc1 & ~(1ull << 63));
     else if (J.nl_cap > 0) J.NL[0] = 0;
     if (lines > J.nl_cap) *overflow = 1;
+    if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
     lines_out[f] = lines;  // true count; jobs are clamped to what the newline array holds
   }
   for (int j = 0; j < njobs; j++) {
@@ -403,10 +488,21 @@ bool query_config_supported(int block, int group) {
   return (block == 64 || block == 128 || block == 256) && (group == 1 || group == 4 || group == 16 || group == 64);
 }
 
+// LDS of the tiled kernel: level slots plus (levels + 1) rows of maxw words.
+constexpr size_t kTiledLdsBudget = 56 * 1024;
+
 template <int BLOCK, int G>
 static hipError_t launch_bg(const QueryLaunch& L, int nslots, int grid, hipStream_t stream) {
-  const size_t lds = (size_t)(BLOCK / G) * nslots * 12;
-  hipLaunchKernelGGL((query_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), lds, stream, L, nslots);
+  const size_t slots = (size_t)(BLOCK / G) * nslots * 12 + 8;
+  if (L.tiled) {
+    const int levels = nslots - 2;
+    int maxw = (int)((kTiledLdsBudget - slots) / ((size_t)(levels + 1) * 8));
+    maxw &= ~7;
+    const size_t lds = slots + (size_t)(levels + 1) * maxw * 8;
+    hipLaunchKernelGGL((query_tiled_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), lds, stream, L, nslots, maxw);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((query_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), slots, stream, L, nslots);
   return hipGetLastError();
 }
 

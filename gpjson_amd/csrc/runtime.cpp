@@ -159,6 +159,9 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   const IndexTune& t = ctx->itune;
   const int64_t tile_bytes = index_tile_bytes(t.block, t.wpt);
   const int64_t ntiles = (idx->n + tile_bytes - 1) / tile_bytes;
+  if (ntiles + 1 > idx->tile_lines_cap) return fail(GPJSON_ERR_CAPACITY, "tile table too small");
+  idx->ntiles = ntiles;
+  idx->tile_bytes = tile_bytes;
   int rc = ctx->states.ensure((size_t)std::max<int64_t>(ntiles, 1) * sizeof(TileState));
   if (rc) return rc;
   rc = ctx->ticket.ensure(4096);
@@ -174,6 +177,7 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   j.nl_cap = idx->nl_cap;
   j.E = idx->E;
   j.Q = idx->Q;
+  j.tile_lines = idx->tile_lines;
   j.tile0 = 0;
   j.ntiles = ntiles;
   j.L = idx->L;
@@ -211,12 +215,15 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   uint8_t* dbase = (uint8_t*)ctx->jobs.p;
   QueryJob* hj = (QueryJob*)hbase;
   int64_t off = (int64_t)jobs_bytes;
+  // Tiled mode needs the line count on the host (span offsets per query); the
+  // line-based mode reads it on the device.
+  const bool tiled = ctx->qtune.tiled && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
   for (int q = 0; q < nq; q++) {
     QueryJob j{};
     j.F = idx->buf->d;
     j.N = idx->n;
     j.NL = idx->NL;
-    j.lines = 0;
+    j.lines = tiled ? idx->lines : 0;
     j.S = idx->S;
     j.LB = idx->LB;
     j.Wp = idx->Wp;
@@ -224,8 +231,11 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.R = R;
     j.ir = dbase + off;
     j.ir_len = ir_lens[q];
-    j.out = spans_dev;
-    j.item0 = 0;
+    j.out = tiled ? spans_dev + (int64_t)q * idx->lines * R * 2 : spans_dev;
+    j.item0 = tiled ? (int64_t)q * idx->ntiles : 0;
+    j.tile_lines = idx->tile_lines;
+    j.ntiles = idx->ntiles;
+    j.tile_bytes = idx->tile_bytes;
     std::memcpy(hbase + off, irs[q], (size_t)ir_lens[q]);
     off += ir_lens[q];
     hj[q] = j;
@@ -235,15 +245,19 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.jobs = (const QueryJob*)dbase;
   L.njobs = nq;
   L.total_items = 0;
-  L.lines_dev = idx->lines_dev;
+  L.lines_dev = tiled ? nullptr : idx->lines_dev;
+  L.total_dev = nullptr;
   L.counter = (unsigned long long*)((uint8_t*)ctx->ticket.p + 64);
   L.lines_per_wave = ctx->qtune.lines_per_wave;
+  L.tiled = tiled ? 1 : 0;
+  if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
   out->block = ctx->qtune.block;
   out->group = ctx->qtune.group;
   out->levels = idx->L;
   const int waves_per_block = ctx->qtune.block / 64;
   out->grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
+  if (tiled) out->grid = (int)std::max<int64_t>(1, std::min<int64_t>(L.total_items, (int64_t)ctx->num_cus * 4));
   return GPJSON_OK;
 }
 
@@ -274,11 +288,11 @@ static int read_lines(gpjson_index* idx) {
 }
 
 static void free_index_arrays(gpjson_index* idx) {
-  void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->lines_dev};
+  void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->lines_dev, idx->tile_lines};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   idx->S = idx->LB = idx->E = idx->Q = nullptr;
-  idx->NL = idx->lines_dev = nullptr;
+  idx->NL = idx->lines_dev = idx->tile_lines = nullptr;
 }
 
 // Builds (or rebuilds) an index; grows the newline array if the first pass
@@ -485,6 +499,9 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
   alloc((void**)&idx->LB, (size_t)idx->Wp * std::max(levels, 1) * 8);
   alloc((void**)&idx->NL, (size_t)idx->nl_cap * 8);
   alloc((void**)&idx->lines_dev, 64);
+  // one entry per tile of the smallest supported tile size, plus the total
+  idx->tile_lines_cap = buf->n / index_tile_bytes(256, 1) + 2;
+  alloc((void**)&idx->tile_lines, (size_t)idx->tile_lines_cap * 8);
   if (flags & GPJSON_BUILD_VERIFY) {
     alloc((void**)&idx->E, (size_t)idx->W * 8);
     alloc((void**)&idx->Q, (size_t)idx->W * 8);

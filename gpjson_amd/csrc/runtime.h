@@ -37,10 +37,11 @@ int hip_fail(hipError_t e, const char* what);
 
 struct IndexTune {
   int block = 256;
-  int wpt = 2;
+  int wpt = 4;
 };
 
 struct QueryTune {
+  int tiled = 1;          // stage each index tile's words in LDS (levels <= 4)
   int block = 256;
   int group = 4;          // lanes per line
   int waves_per_cu = 32;  // persistent grid size factor
@@ -97,6 +98,10 @@ struct gpjson_index {
   uint64_t* Q = nullptr;
   int64_t lines = 0;
   int64_t* lines_dev = nullptr;  // device copy of the line count
+  int64_t* tile_lines = nullptr; // first line id of each index tile (+ total)
+  int64_t tile_lines_cap = 0;
+  int64_t ntiles = 0;            // tiles of the last build
+  int64_t tile_bytes = 0;
 };
 
 namespace gj {
