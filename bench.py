@@ -10,6 +10,16 @@ Prints ONE JSON line.  The roofline object uses the index kernel (the dominant
 kernel): achieved = input bytes per launch / mean index-kernel duration from HIP
 events on the library's stream.  cpu_baseline times the multithreaded CPU
 evaluator (test oracle, kind "port") on a bounded prefix of the same input.
+
+Other modes (not the default line):
+  --mode batch  256 files of lognormal size in [1 MiB, 512 MiB] (16 GiB), tweets
+                and adversarial alternating, 8 member-chain queries per file:
+                one batched hipGraph run versus 256 sequential plain runs,
+                checked bit for bit against the plain spans.
+  --mode serve  64 tweets files (100 GiB) resident under an HBM budget; closed
+                burst capacity, then Poisson open-loop arrivals at 3 rates over
+                12 member-chain paths (p50/p99 latency, GB/s), versus the plain
+                one-query-one-launch path.
 """
 from __future__ import annotations
 
@@ -70,27 +80,32 @@ def load_traffic(L_bytes_per_launch: float):
         return None
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=4 << 30, help="input bytes (default 4 GiB)")
-    ap.add_argument("--kind", default="tweets", choices=["tweets", "adversarial"])
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--paths", default=",".join(PLAIN_PATHS))
-    ap.add_argument("--cpu-sample", type=int, default=512 << 20)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
-    ap.add_argument("--config", default="", help="index_block,index_wpt,query_tiled,query_block,query_group,wpc,lpg")
-    args = ap.parse_args()
+TWEET_PATHS_8 = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name",
+                 "$.user.stats.friends", "$.retweet_count", "$.user.profile.colors.background"]
+ADVERSARIAL_PATHS_8 = ["$.id", "$.s1", "$.chain.pad", "$.chain.c1.pad", "$.chain.c1.c2", "$.chain.c1.c2.c3.c4",
+                       "$.deep.d.d", "$.s2"]
+SERVE_PATHS_12 = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name", "$.user.id",
+                  "$.user.stats.friends", "$.user.screen_name", "$.retweet_count", "$.lang",
+                  "$.place.bounding_box.type", "$.user.profile.colors.background"]
 
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the plain, batched and served "
-              "paths are single-GPU by design.", file=sys.stderr)
-        return 2
 
+def common_fields(args, value, dt):
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / max(args.steps, 1) * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+    }
+
+
+def run_plain(args) -> int:
     import numpy as np
 
     import gpjson_amd
@@ -145,24 +160,13 @@ def main() -> int:
     achieved = data.size / (mean_idx_ms / 1e3) / 1e9
     log(f"index kernel ms per path {['%.3f' % x for x in idx_ms]}; query kernel ms {['%.3f' % x for x in q_ms]}")
 
-    # spot check: the last pass of the first plan agrees with the oracle on a prefix of lines
     spans = plans[0].spans()
     ok_lines = int(np.sum(spans[0, :, 0, 0] >= 0))
     log(f"{paths[0]}: {ok_lines} of {lines} lines have a result")
 
     traffic = load_traffic(data.size)
-    result = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
+    result = common_fields(args, value, dt)
+    result.update({
         "data": f"synthetic seeded {args.kind} NDJSON (seed {args.seed}), resident in HBM",
         "config": {
             "workload": f"plain path, {args.kind} NDJSON {args.size} bytes ({lines} lines); one step = "
@@ -187,7 +191,7 @@ def main() -> int:
         "query_ms": [round(x, 4) for x in q_ms],
         "staging_GBps": round(data.size / stage_s / 1e9, 2),
         "launch_shapes": ctx.get_config(),
-    }
+    })
     for p in plans:
         p.free()
     buf.free()
@@ -200,6 +204,268 @@ def main() -> int:
     print(json.dumps(result), flush=True)
     ctx.close()
     return 0
+
+
+def batch_sizes(n: int, total: int, seed: int, lo: int = 1 << 20, hi: int = 512 << 20):
+    """n lognormal sizes clipped to [lo, hi], scaled so they sum to about total."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    raw = rng.lognormal(mean=0.0, sigma=1.5, size=n)
+    scale = total / raw.sum()
+    for _ in range(50):  # rescale until the clipped sizes sum to the target
+        s = np.clip(raw * scale, lo, hi)
+        scale *= total / s.sum()
+    return [int(x) for x in np.clip(raw * scale, lo, hi)]
+
+
+def run_batch(args) -> int:
+    """256 files of uneven size (tweets and adversarial mixed), 8 member-chain queries
+    per file: one batched run (persistent index launch + one interpreter launch,
+    replayed as a hipGraph) against 256 sequential plain runs (index + query per file)."""
+    import numpy as np
+
+    import gpjson_amd
+    from gpjson_amd import compile_path, datagen
+
+    sizes = batch_sizes(args.files, args.batch_total, args.seed)
+    kinds = ["tweets" if i % 2 == 0 else "adversarial" for i in range(args.files)]
+    total = sum(sizes)
+    log(f"batch: {args.files} files, {total} bytes (min {min(sizes)}, max {max(sizes)})")
+    ctx = gpjson_amd.Context(0)
+    bufs, irs, nbytes = [], [], 0
+    t = time.perf_counter()
+    for i, (kind, n) in enumerate(zip(kinds, sizes)):
+        d = datagen.generate(kind, args.seed * 1000 + i, n)
+        bufs.append(ctx.stage(d))
+        nbytes += d.size
+        del d
+        ps = TWEET_PATHS_8 if kind == "tweets" else ADVERSARIAL_PATHS_8
+        irs.append([compile_path(p).ir for p in ps])
+        if i % 32 == 31:
+            log(f"staged {i + 1} files, {nbytes / 1e9:.1f} GB, {time.perf_counter() - t:.1f} s")
+    ctx.synchronize()
+    if args.config:
+        ctx.set_config(*[int(x) for x in args.config.split(",")])
+    if args.autotune:
+        big = int(np.argmax(sizes))
+        t = time.perf_counter()
+        report = ctx.autotune(bufs[big], irs[big], 2)
+        for line in report.strip().splitlines():
+            log("autotune: " + line)
+        log(f"autotune on file {big} took {time.perf_counter() - t:.1f} s")
+    log(f"launch shapes: {ctx.get_config()}")
+
+    # plain: one index + one interpreter launch (8 queries) per file, file after file
+    plans = [ctx.plan(b, q) for b, q in zip(bufs, irs)]
+    for p in plans:
+        p.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in plans:
+            p.run()
+    ctx.synchronize()
+    plain_dt = (time.perf_counter() - t0) / args.steps
+    plain_gbps = nbytes / plain_dt / 1e9
+    log(f"plain: {plain_dt * 1e3:.2f} ms per pass over all files, {plain_gbps:.1f} GB/s")
+
+    batch = ctx.batch(bufs, irs)
+    for _ in range(max(args.warmup, 1)):
+        batch.run(graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.run(graph=True, timed=(k == args.steps - 1))
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    batch_dt = dt / args.steps
+    value = nbytes / batch_dt / 1e9
+    _, total_ms = batch.timing()
+    log(f"batched: {batch_dt * 1e3:.2f} ms per run, {value:.1f} GB/s (events {total_ms:.2f} ms)")
+
+    # bit-exact: every (file, query) of the batch equals the plain plan
+    mism = 0
+    for f, p in enumerate(plans):
+        ps = p.spans()
+        for q in range(len(irs[f])):
+            if not np.array_equal(batch.spans(f, q), ps[q]):
+                mism += 1
+    log(f"batched vs plain: {mism} mismatching (file, query) pairs of {sum(len(q) for q in irs)}")
+    if mism:
+        return 1
+
+    result = common_fields(args, value, dt)
+    result.update({
+        "data": f"synthetic seeded tweets + adversarial NDJSON (seed {args.seed}), resident in HBM",
+        "config": {
+            "workload": f"batched path: {args.files} files, lognormal sizes in [1 MiB, 512 MiB], {nbytes} bytes, "
+                        "alternating tweets/adversarial; 8 member-chain queries per file; one step = one "
+                        "hipGraph replay (persistent index launch + one interpreter launch)",
+            "input_bytes": nbytes,
+            "tweet_paths": TWEET_PATHS_8,
+            "adversarial_paths": ADVERSARIAL_PATHS_8,
+        },
+        "plain_GBps": round(plain_gbps, 3),
+        "plain_ms_per_pass": round(plain_dt * 1e3, 3),
+        "batched_over_plain": round(value / plain_gbps, 3),
+        "frac_of_hbm_peak_end_to_end": round(value / HBM_PEAK_GBPS, 4),
+        "bit_exact_vs_plain": True,
+        "launch_shapes": ctx.get_config(),
+    })
+    print(json.dumps(result), flush=True)
+    batch.free()
+    for p in plans:
+        p.free()
+    ctx.close()
+    return 0
+
+
+def run_serve(args) -> int:
+    """Resident files under an HBM budget; Poisson open-loop queries at 3 rates,
+    coalesced into one interpreter launch per batch, against the plain
+    one-query-one-launch path (index + query + result readback per request)."""
+    import numpy as np
+
+    import gpjson_amd
+    from gpjson_amd import compile_path, datagen
+    from gpjson_amd.serve import Server, open_loop
+
+    per_file = args.serve_total // args.files
+    levels = max(compile_path(p).max_depth for p in SERVE_PATHS_12)
+    rng = np.random.default_rng(args.seed)
+
+    # plain one-query-one-launch on a few files first (their HBM is released before serving)
+    ctx = gpjson_amd.Context(0)
+    sample_files = min(4, args.files)
+    plain_t, plain_b = 0.0, 0
+    for i in range(sample_files):
+        d = datagen.generate("tweets", args.seed * 1000 + i, per_file)
+        buf = ctx.stage(d)
+        for p in SERVE_PATHS_12:
+            plan = ctx.plan(buf, [compile_path(p).ir])
+            plan.run()
+            plan.spans()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                plan.run()
+                plan.spans()
+            plain_t += time.perf_counter() - t0
+            plain_b += 2 * d.size
+            plan.free()
+        buf.free()
+        del d
+    plain_gbps = plain_b / plain_t / 1e9
+    plain_ms = plain_t / (sample_files * len(SERVE_PATHS_12) * 2) * 1e3
+    log(f"plain one-query-one-launch: {plain_ms:.2f} ms per request, {plain_gbps:.1f} GB/s")
+    ctx.close()
+
+    srv = Server(args.serve_budget, levels)
+    t = time.perf_counter()
+    for i in range(args.files):
+        d = datagen.generate("tweets", args.seed * 1000 + i, per_file)
+        srv.add(f"file{i:03d}", d)
+        del d
+        if i % 8 == 7:
+            log(f"registered {i + 1} files ({(i + 1) * per_file / 1e9:.1f} GB), {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    for i in range(args.files):
+        srv.admit(f"file{i:03d}")
+    log(f"admitted {args.files} files in {time.perf_counter() - t:.1f} s; stats {srv.stats()}")
+
+    def draw(n):
+        fs = rng.integers(0, args.files, size=n)
+        ps = rng.integers(0, len(SERVE_PATHS_12), size=n)
+        return [(f"file{int(f):03d}", SERVE_PATHS_12[int(p)]) for f, p in zip(fs, ps)]
+
+    # capacity: everything submitted at once (closed burst)
+    burst = draw(args.serve_burst)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(k, p) for k, p in burst]
+    for tk in tickets:
+        srv.wait(tk)
+    cap_dt = time.perf_counter() - t0
+    capacity = len(burst) / cap_dt
+    cap_gbps = len(burst) * per_file / cap_dt / 1e9
+    log(f"burst of {len(burst)}: {capacity:.0f} requests/s, {cap_gbps:.1f} GB/s; stats {srv.stats()}")
+
+    rates = [float(x) for x in args.serve_rates.split(",")] if args.serve_rates else \
+        [round(capacity * f, 1) for f in (0.25, 0.5, 0.8)]
+    per_rate = []
+    for r in rates:
+        n = max(100, int(r * args.serve_seconds))
+        st0 = srv.stats()
+        lat, wall, nb, _ = open_loop(srv, draw(n), r, seed=args.seed + int(r))
+        st1 = srv.stats()
+        launches = st1["launches"] - st0["launches"]
+        per_rate.append({
+            "rate_per_s": r,
+            "requests": n,
+            "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
+            "GBps": round(nb / wall / 1e9, 3),
+            "mean_batch": round(n / max(launches, 1), 2),
+        })
+        log(f"rate {r}/s: {per_rate[-1]}")
+    st = srv.stats()
+    srv.close()
+
+    value = cap_gbps
+    result = common_fields(args, value, cap_dt)
+    result.update({
+        "data": f"synthetic seeded tweets NDJSON (seed {args.seed}), registered in host memory, resident in HBM",
+        "config": {
+            "workload": f"serving: {args.files} tweets files of {per_file} bytes ({args.files * per_file} bytes), "
+                        f"index levels {levels}, HBM budget {args.serve_budget} bytes; 12 member-chain paths over "
+                        "uniformly chosen files; value = closed-burst capacity in input GB/s",
+            "paths": SERVE_PATHS_12,
+            "hbm_budget_bytes": args.serve_budget,
+        },
+        "capacity_requests_per_s": round(capacity, 1),
+        "rates": per_rate,
+        "plain_GBps": round(plain_gbps, 3),
+        "plain_ms_per_request": round(plain_ms, 3),
+        "served_over_plain": round(value / plain_gbps, 3),
+        "server_stats": st,
+    })
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--size", type=int, default=4 << 30, help="plain: input bytes (default 4 GiB)")
+    ap.add_argument("--kind", default="tweets", choices=["tweets", "adversarial"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--paths", default=",".join(PLAIN_PATHS))
+    ap.add_argument("--cpu-sample", type=int, default=512 << 20)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
+    ap.add_argument("--config", default="", help="index_block,index_wpt,query_tiled,query_block,query_group,wpc,lpg")
+    ap.add_argument("--files", type=int, default=0, help="batch: 256 files; serve: 64 files")
+    ap.add_argument("--batch-total", type=int, default=16 << 30)
+    ap.add_argument("--serve-total", type=int, default=100 << 30)
+    ap.add_argument("--serve-budget", type=int, default=192 << 30)
+    ap.add_argument("--serve-burst", type=int, default=512)
+    ap.add_argument("--serve-rates", default="", help="requests/s, comma separated (default: 25/50/80%% of capacity)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the plain, batched and served "
+              "paths are single-GPU by design.", file=sys.stderr)
+        return 2
+    if args.mode == "batch":
+        args.files = args.files or 256
+        return run_batch(args)
+    if args.mode == "serve":
+        args.files = args.files or 64
+        return run_serve(args)
+    return run_plain(args)
 
 
 if __name__ == "__main__":

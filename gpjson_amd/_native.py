@@ -82,6 +82,14 @@ SIGNATURES = {
     "gpjson_batch_results_per_line": (c_int, [c_void_p, c_int, c_int]),
     "gpjson_batch_spans": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64]),
     "gpjson_batch_free": (None, [c_void_p]),
+    "gpjson_server_create": (c_int, [c_int, c_int64, c_int, P(c_void_p)]),
+    "gpjson_server_add": (c_int, [c_void_p, c_char_p, c_void_p, c_int64]),
+    "gpjson_server_admit": (c_int, [c_void_p, c_char_p]),
+    "gpjson_server_submit": (c_int, [c_void_p, c_char_p, c_char_p, P(ctypes.c_uint64)]),
+    "gpjson_server_wait": (c_int, [c_void_p, ctypes.c_uint64, c_int64, P(c_int64), P(c_int), P(c_double), c_void_p,
+                                   c_int64]),
+    "gpjson_server_stats": (c_int, [c_void_p, P(c_int64)]),
+    "gpjson_server_destroy": (None, [c_void_p]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),

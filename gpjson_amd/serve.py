@@ -1,0 +1,119 @@
+"""Query serving over resident indexes (gpjson_server_* in include/gpjson.h).
+
+Server keeps registered files and their indexes resident in HBM within a byte
+budget (least-recently-used files are evicted; the host copy stays) and
+coalesces every pending query on resident files into one interpreter launch.
+``open_loop`` drives it with Poisson arrivals and reports latency percentiles.
+"""
+from __future__ import annotations
+
+import ctypes
+import random
+import threading
+import time
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .api import _check
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes: int, levels: int, device: int = 0):
+        self._lib = N.load()
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_server_create(device, int(hbm_budget_bytes), levels, ctypes.byref(h)))
+        self._h = h
+        self.levels = levels
+        self.sizes: Dict[str, int] = {}
+
+    def add(self, key: str, data) -> None:
+        arr = np.frombuffer(memoryview(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else \
+            np.ascontiguousarray(data, dtype=np.uint8)
+        _check(self._lib.gpjson_server_add(self._h, key.encode(), arr.ctypes.data_as(ctypes.c_void_p), arr.size))
+        self.sizes[key] = int(arr.size)
+
+    def admit(self, key: str) -> None:
+        _check(self._lib.gpjson_server_admit(self._h, key.encode()))
+
+    def submit(self, key: str, path: str) -> int:
+        t = ctypes.c_uint64()
+        _check(self._lib.gpjson_server_submit(self._h, key.encode(), path.encode(), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> Tuple[np.ndarray, float]:
+        """(spans int64[lines][R][2], latency in microseconds) of a ticket."""
+        lines, R, lat = ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.gpjson_server_wait(self._h, ticket, timeout_ms, ctypes.byref(lines), ctypes.byref(R),
+                                            ctypes.byref(lat), None, 0))
+        out = np.empty((lines.value, R.value, 2), dtype=np.int64)
+        _check(self._lib.gpjson_server_wait(self._h, ticket, timeout_ms, None, None, None,
+                                            out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out, lat.value
+
+    def stats(self) -> dict:
+        v = (ctypes.c_int64 * 7)()
+        _check(self._lib.gpjson_server_stats(self._h, v))
+        keys = ("resident_bytes", "resident_files", "launches", "requests", "max_batch", "admissions", "evictions")
+        return dict(zip(keys, list(v)))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.gpjson_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: float, seed: int = 0,
+              keep_spans: bool = False, wait_timeout_ms: int = 120000):
+    """Submits `requests` with exponential inter-arrival times (Poisson arrivals
+    at rate_per_s, seeded) from one thread while another collects results.
+    Returns (latencies_us, wall_s, bytes_queried, spans or None)."""
+    rng = random.Random(seed)
+    gaps = [rng.expovariate(rate_per_s) for _ in requests]
+    tickets: List[int] = [0] * len(requests)
+    ready = [threading.Event() for _ in requests]
+    lat = [0.0] * len(requests)
+    spans = [None] * len(requests)
+    errors: List[BaseException] = []
+
+    def collector():
+        try:
+            for i in range(len(requests)):
+                ready[i].wait()
+                sp, l_us = server.wait(tickets[i], wait_timeout_ms)
+                lat[i] = l_us
+                if keep_spans:
+                    spans[i] = sp
+        except BaseException as e:  # surfaced to the caller
+            errors.append(e)
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for i, (key, path) in enumerate(requests):
+        due += gaps[i]
+        d = due - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[i] = server.submit(key, path)
+        ready[i].set()
+    th.join()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    nbytes = sum(server.sizes[k] for k, _ in requests)
+    return np.array(lat), wall, nbytes, (spans if keep_spans else None)

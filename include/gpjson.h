@@ -197,6 +197,30 @@ int gpjson_batch_results_per_line(const gpjson_batch *batch, int file, int q);
 int gpjson_batch_spans(gpjson_batch *batch, int file, int q, int64_t *spans, int64_t cap_elems);
 void gpjson_batch_free(gpjson_batch *batch);
 
+/* ---- serving (no counterpart in the reference) ---------------------------- */
+
+/* A query server on `device` that keeps files and their indexes (built for
+ * `levels` levels) resident in HBM within hbm_budget_bytes (input + index
+ * bytes), evicting least-recently-used files (the host copy stays), and
+ * coalesces concurrent queries on resident files into one interpreter launch.
+ * Owns two streams (serving, admission) and two threads. */
+int gpjson_server_create(int device, int64_t hbm_budget_bytes, int levels, gpjson_server **out);
+/* Registers n host bytes under `key` (copied; not resident until needed). */
+int gpjson_server_add(gpjson_server *server, const char *key, const void *data, int64_t n);
+/* Makes `key` resident now (stage + index), evicting as needed; waits. */
+int gpjson_server_admit(gpjson_server *server, const char *key);
+/* Queues query `path` on file `key`; returns at once with a ticket. */
+int gpjson_server_submit(gpjson_server *server, const char *key, const char *path, uint64_t *ticket);
+/* Waits for a ticket (timeout_ms < 0: no timeout); gives lines, results per
+ * line, latency from submit to completion in microseconds, and (spans not
+ * NULL) copies int64[lines][R][2] spans, after which the ticket is forgotten. */
+int gpjson_server_wait(gpjson_server *server, uint64_t ticket, int64_t timeout_ms, int64_t *lines, int *R,
+                       double *latency_us, int64_t *spans, int64_t cap_elems);
+/* out7: resident bytes, resident files, interpreter launches, requests served
+ * by them, largest batch, admissions, evictions. */
+int gpjson_server_stats(gpjson_server *server, int64_t *out7);
+void gpjson_server_destroy(gpjson_server *server);
+
 /* ---- launch shapes and the autotuner -------------------------------------- */
 
 /* Current launch shapes of the context: index tile = index_block threads x

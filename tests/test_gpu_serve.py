@@ -1,0 +1,115 @@
+"""Serving: every served result equals the plain path bit for bit, alone and in a
+concurrent mix, with coalesced launches and LRU eviction under a tight budget."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PATHS = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name",
+         "$.entities.hashtags[0]"]
+ADV_PATHS = ["$.arr[999]", "$.chain.c1.c2", "$.id"]
+
+
+@pytest.fixture(scope="module")
+def files():
+    from gpjson_amd import datagen
+    out = {}
+    for i in range(6):
+        kind = "tweets" if i % 3 else "adversarial"
+        out[f"f{i}"] = (kind, datagen.generate(kind, 500 + i, (1 + i) * (1 << 20) + 777).tobytes())
+    return out
+
+
+@pytest.fixture(scope="module")
+def plain(ctx, files):
+    from gpjson_amd import compile_path
+    ref = {}
+    for key, (kind, data) in files.items():
+        buf = ctx.stage(data)
+        idx = ctx.index(buf, 4)
+        for p in (PATHS if kind == "tweets" else ADV_PATHS):
+            cp = compile_path(p)
+            ref[(key, p)] = ctx.run_queries(idx, [cp.ir], cp.num_results)[0]
+        idx.free()
+        buf.free()
+    return ref
+
+
+def _reqs(files, n, seed):
+    rng = np.random.default_rng(seed)
+    keys = sorted(files)
+    out = []
+    for _ in range(n):
+        k = keys[int(rng.integers(0, len(keys)))]
+        ps = PATHS if files[k][0] == "tweets" else ADV_PATHS
+        out.append((k, ps[int(rng.integers(0, len(ps)))]))
+    return out
+
+
+def test_alone_and_in_mix_equal_plain(files, plain):
+    from gpjson_amd.serve import Server
+    total = sum(len(d) for _, d in files.values())
+    # budget below the total: admissions must evict
+    with Server(hbm_budget_bytes=int(total * 0.9), levels=4) as srv:
+        for key, (_, data) in files.items():
+            srv.add(key, data)
+        # alone
+        for key, p in [("f1", "$.user.lang"), ("f0", "$.arr[999]")]:
+            print(f"[serve] alone {key} {p}", flush=True)
+            sp, _ = srv.wait(srv.submit(key, p), 60000)
+            assert np.array_equal(sp, plain[(key, p)])
+        # in a concurrent mix from several threads
+        reqs = _reqs(files, 120, 3)
+        tickets = [None] * len(reqs)
+
+        def submitter(lo):
+            for i in range(lo, len(reqs), 4):
+                tickets[i] = srv.submit(*reqs[i])
+
+        ths = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+        [t.start() for t in ths]
+        [t.join() for t in ths]
+        print("[serve] mix submitted", flush=True)
+        for i, (key, p) in enumerate(reqs):
+            sp, lat = srv.wait(tickets[i], 60000)
+            assert np.array_equal(sp, plain[(key, p)]), (key, p)
+            assert lat > 0
+        st = srv.stats()
+        print(f"[serve] stats {st}", flush=True)
+        assert st["launches"] < st["requests"], st       # queries were coalesced
+        assert st["evictions"] > 0, st                    # the budget forced LRU eviction
+        assert st["resident_bytes"] <= int(total * 0.9)
+
+
+def test_open_loop_poisson(files, plain):
+    from gpjson_amd.serve import Server, open_loop
+    with Server(hbm_budget_bytes=1 << 34, levels=4) as srv:
+        for key, (_, data) in files.items():
+            srv.add(key, data)
+            srv.admit(key)
+        reqs = _reqs(files, 200, 9)
+        lat, wall, nbytes, spans = open_loop(srv, reqs, rate_per_s=2000.0, seed=1, keep_spans=True)
+        for (key, p), sp in zip(reqs, spans):
+            assert np.array_equal(sp, plain[(key, p)])
+        print(f"[serve] p50 {np.percentile(lat, 50):.0f} us p99 {np.percentile(lat, 99):.0f} us "
+              f"{nbytes / wall / 1e9:.2f} GB/s stats {srv.stats()}", flush=True)
+
+
+def test_errors(files):
+    from gpjson_amd import JSONPathException, UnsupportedJSONPathException
+    from gpjson_amd.serve import Server
+    with Server(hbm_budget_bytes=1 << 30, levels=2) as srv:
+        srv.add("a", files["f1"][1])
+        with pytest.raises(ValueError):
+            srv.submit("missing", "$.a")
+        with pytest.raises(UnsupportedJSONPathException):
+            srv.submit("a", "$..a")
+        with pytest.raises(JSONPathException):
+            srv.submit("a", "$.a b")
+        from gpjson_amd import GPJSONException
+        with pytest.raises(GPJSONException):
+            srv.submit("a", "$.a.b.c")  # deeper than the server's 2 levels
+        with pytest.raises(ValueError):
+            srv.add("a", b"{}")
