@@ -42,7 +42,8 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(data, paths, sample_bytes: int, threads: int):
-    """Multithreaded CPU evaluator (oracle) on the first sample_bytes of data."""
+    """Multithreaded CPU evaluator (oracle) on the first sample_bytes of data:
+    (spans per path, baseline record)."""
     import numpy as np
 
     from gpjson_amd import compile_path
@@ -53,12 +54,13 @@ def cpu_baseline(data, paths, sample_bytes: int, threads: int):
     cut = int(np.flatnonzero(data[:n] == 10)[-1]) + 1 if n < data.size else n
     sample = data[:cut]
     cps = [compile_path(p) for p in paths]
+    spans = []
     t0 = time.perf_counter()
     for cp in cps:
         nl, s, lb = O.index_mt(sample, cp.max_depth, threads)
-        O.find_value_mt(sample, nl, s, lb, cp.max_depth, cp.ir, cp.num_results, threads)
+        spans.append(O.find_value_mt(sample, nl, s, lb, cp.max_depth, cp.ir, cp.num_results, threads))
     dt = time.perf_counter() - t0
-    return {
+    return spans, {
         "value": round(len(paths) * sample.size / dt / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
@@ -160,9 +162,17 @@ def run_plain(args) -> int:
     achieved = data.size / (mean_idx_ms / 1e3) / 1e9
     log(f"index kernel ms per path {['%.3f' % x for x in idx_ms]}; query kernel ms {['%.3f' % x for x in q_ms]}")
 
-    spans = plans[0].spans()
-    ok_lines = int(np.sum(spans[0, :, 0, 0] >= 0))
-    log(f"{paths[0]}: {ok_lines} of {lines} lines have a result")
+    # every plan's spans equal the one-shot path's (its own index + launch)
+    plan_spans = [p.spans()[0] for p in plans]
+    one_shot_ok = True
+    for cp, got in zip(cps, plan_spans):
+        idx = ctx.index(buf, cp.max_depth)
+        want = ctx.run_queries(idx, [cp.ir], cp.num_results)[0]
+        idx.free()
+        same = bool(np.array_equal(got, want))
+        one_shot_ok &= same
+        log(f"{cp.path}: {int(np.sum(got[:, 0, 0] >= 0))} of {lines} lines have a result; "
+            f"equal to the one-shot path: {same}")
 
     traffic = load_traffic(data.size)
     result = common_fields(args, value, dt)
@@ -191,6 +201,7 @@ def run_plain(args) -> int:
         "query_ms": [round(x, 4) for x in q_ms],
         "staging_GBps": round(data.size / stage_s / 1e9, 2),
         "launch_shapes": ctx.get_config(),
+        "bit_exact_vs_one_shot": one_shot_ok,
     })
     for p in plans:
         p.free()
@@ -198,12 +209,19 @@ def run_plain(args) -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {min(args.cpu_sample, data.size)} bytes with {threads} threads")
-        result["cpu_baseline"] = cpu_baseline(data, paths, args.cpu_sample, threads)
+        cpu_spans, result["cpu_baseline"] = cpu_baseline(data, paths, args.cpu_sample, threads)
+        # the sample ends at a newline, so all but its last (empty) line are whole lines of the input
+        oracle_ok = True
+        for got, ref in zip(plan_spans, cpu_spans):
+            k = ref.size // (2 * got.shape[1]) - 1
+            oracle_ok &= bool(np.array_equal(got[:k].reshape(-1), ref[:k * got.shape[1] * 2]))
+        result["bit_exact_vs_oracle_on_sample"] = oracle_ok
+        log(f"GPU spans equal the oracle's on the sample's lines: {oracle_ok}")
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
     ctx.close()
-    return 0
+    return 0 if one_shot_ok and result.get("bit_exact_vs_oracle_on_sample", True) else 1
 
 
 def batch_sizes(n: int, total: int, seed: int, lo: int = 1 << 20, hi: int = 512 << 20):

@@ -154,15 +154,18 @@ int upload_irs(gpjson_ctx* ctx, const uint8_t* const* irs, const int64_t* ir_len
   return GPJSON_OK;
 }
 
-int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel) {
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
   gpjson_ctx* ctx = idx->ctx;
-  const IndexTune& t = ctx->itune;
+  const IndexTune& t = tune ? *tune : ctx->itune;
   const int64_t tile_bytes = index_tile_bytes(t.block, t.wpt);
   const int64_t ntiles = (idx->n + tile_bytes - 1) / tile_bytes;
   if (ntiles + 1 > idx->tile_lines_cap) return fail(GPJSON_ERR_CAPACITY, "tile table too small");
   idx->ntiles = ntiles;
   idx->tile_bytes = tile_bytes;
-  int rc = ctx->states.ensure((size_t)std::max<int64_t>(ntiles, 1) * sizeof(TileState));
+  const size_t states_bytes = (size_t)std::max<int64_t>(ntiles, 1) * sizeof(TileState);
+  // growing the tile-state array frees the old one: earlier launches must be done
+  if (states_bytes > ctx->states.cap) GJ_HIP(hipStreamSynchronize(ctx->stream));
+  int rc = ctx->states.ensure(states_bytes);
   if (rc) return rc;
   rc = ctx->ticket.ensure(4096);
   if (rc) return rc;
@@ -200,21 +203,24 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
 }
 
 int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
-                           const int64_t* ir_lens, int nq, int R, int64_t* spans_dev, PreparedQueries* out) {
+                           const int64_t* ir_lens, int nq, int R, int64_t* spans_dev, DevBuf* table,
+                           PreparedQueries* out) {
   int64_t ir_total = 0;
   for (int q = 0; q < nq; q++) ir_total += ir_lens[q];
+  // table layout: [work counter, 64 B][QueryJob x nq][IR bytes]
+  const size_t jobs_off = 64;
   const size_t jobs_bytes = sizeof(QueryJob) * (size_t)nq;
-  const size_t bytes = jobs_bytes + (size_t)ir_total + 16;
+  const size_t bytes = jobs_off + jobs_bytes + (size_t)ir_total + 16;
+  // (also waits for the stream, so a table being replaced is no longer read)
   int rc = ensure_host_scratch(ctx, bytes);
   if (rc) return rc;
-  rc = ctx->jobs.ensure(bytes);
-  if (rc) return rc;
-  rc = ctx->ticket.ensure(4096);
+  rc = table->ensure(bytes);
   if (rc) return rc;
   uint8_t* hbase = (uint8_t*)ctx->host_scratch;
-  uint8_t* dbase = (uint8_t*)ctx->jobs.p;
-  QueryJob* hj = (QueryJob*)hbase;
-  int64_t off = (int64_t)jobs_bytes;
+  uint8_t* dbase = (uint8_t*)table->p;
+  std::memset(hbase, 0, jobs_off);
+  QueryJob* hj = (QueryJob*)(hbase + jobs_off);
+  int64_t off = (int64_t)(jobs_off + jobs_bytes);
   // Tiled mode needs the line count on the host (span offsets per query); the
   // line-based mode reads it on the device.
   const bool tiled = ctx->qtune.tiled && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
@@ -242,12 +248,12 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   }
   GJ_HIP(hipMemcpyAsync(dbase, hbase, bytes, hipMemcpyHostToDevice, ctx->stream));
   QueryLaunch L{};
-  L.jobs = (const QueryJob*)dbase;
+  L.jobs = (const QueryJob*)(dbase + jobs_off);
   L.njobs = nq;
   L.total_items = 0;
   L.lines_dev = tiled ? nullptr : idx->lines_dev;
   L.total_dev = nullptr;
-  L.counter = (unsigned long long*)((uint8_t*)ctx->ticket.p + 64);
+  L.counter = (unsigned long long*)dbase;
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   L.tiled = tiled ? 1 : 0;
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
@@ -272,7 +278,7 @@ int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t befor
 int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
                         const int64_t* ir_lens, int nq, int R, int64_t* spans_dev) {
   PreparedQueries pq;
-  int rc = prepare_queries(ctx, idx, irs, ir_lens, nq, R, spans_dev, &pq);
+  int rc = prepare_queries(ctx, idx, irs, ir_lens, nq, R, spans_dev, &ctx->jobs, &pq);
   if (rc) return rc;
   return launch_prepared(ctx, pq);
 }
@@ -732,6 +738,8 @@ struct gpjson_plan {
   int nq = 0, R = 1;
   int64_t* spans = nullptr;
   gj::PreparedQueries pq;
+  gj::DevBuf table;    // this plan's counter + job table + IR bytes
+  gj::IndexTune itune;  // index tile shape the prepared launch was sized for
   std::vector<uint8_t> irs_flat;
   std::vector<int64_t> ir_lens;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // index kernel, query kernel
@@ -755,7 +763,8 @@ int gpjson_plan_create(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* i
   int rc = gpjson_index_build(ctx, buf, L, 0, &p->idx);
   if (rc) return rc;
   rc = dev_alloc((void**)&p->spans, (size_t)nq * (size_t)std::max<int64_t>(p->idx->nl_cap, 1) * R * 2 * 8);
-  if (rc == GPJSON_OK) rc = prepare_queries(ctx, p->idx, irs, ir_lens, nq, R, p->spans, &p->pq);
+  p->itune = ctx->itune;
+  if (rc == GPJSON_OK) rc = prepare_queries(ctx, p->idx, irs, ir_lens, nq, R, p->spans, &p->table, &p->pq);
   for (int k = 0; k < 4 && rc == GPJSON_OK; k++)
     if (hipEventCreate(&p->ev[k]) != hipSuccess) rc = fail(GPJSON_ERR_HIP, "hipEventCreate");
   if (rc) {
@@ -770,7 +779,9 @@ int gpjson_plan_run(gpjson_plan* p, int timed) {
   if (!p) return fail(GPJSON_ERR_INVALID_ARG, "plan is NULL");
   gpjson_ctx* ctx = p->ctx;
   GJ_HIP(hipSetDevice(ctx->device));
-  int rc = enqueue_index(p->idx, timed ? p->ev[0] : nullptr, timed ? p->ev[1] : nullptr);
+  // the plan keeps the tile shape it was prepared with (its tiled interpreter
+  // launch is sized by that index's tiles), whatever the context's shape is now
+  int rc = enqueue_index(p->idx, timed ? p->ev[0] : nullptr, timed ? p->ev[1] : nullptr, &p->itune);
   if (rc) return rc;
   rc = launch_prepared(ctx, p->pq, timed ? p->ev[2] : nullptr, timed ? p->ev[3] : nullptr);
   if (rc) return rc;
@@ -817,6 +828,7 @@ void gpjson_plan_free(gpjson_plan* p) {
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
   if (p->spans) (void)hipFree(p->spans);
+  p->table.release();
   if (p->idx) gpjson_index_free(p->idx);
   delete p;
 }

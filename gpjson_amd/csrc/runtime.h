@@ -113,12 +113,16 @@ struct PreparedQueries {
   int group = 4;
   int levels = 0;
 };
+// The job table (work counter, jobs, IR bytes) lives in *table, which the
+// caller owns for as long as the prepared launch may run.
 int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs, const int64_t* ir_lens,
-                    int nq, int R, int64_t* spans_dev, PreparedQueries* out);
+                    int nq, int R, int64_t* spans_dev, DevBuf* table, PreparedQueries* out);
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel = nullptr,
                     hipEvent_t after_kernel = nullptr);
 // Enqueues reset + index + line count for one index (no host sync).
-int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel = nullptr, hipEvent_t after_kernel = nullptr);
+// tune: tile shape to use (NULL: the context's current shape).
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel = nullptr, hipEvent_t after_kernel = nullptr,
+                  const IndexTune* tune = nullptr);
 // Enqueues one interpreter launch of nq queries over idx with the line count
 // read on the device; spans_dev receives int64[nq][lines][R][2].
 int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
