@@ -88,7 +88,7 @@ SIGNATURES = {
     "gpjson_server_submit": (c_int, [c_void_p, c_char_p, c_char_p, P(ctypes.c_uint64)]),
     "gpjson_server_wait": (c_int, [c_void_p, ctypes.c_uint64, c_int64, P(c_int64), P(c_int), P(c_double), c_void_p,
                                    c_int64]),
-    "gpjson_server_stats": (c_int, [c_void_p, P(c_int64)]),
+    "gpjson_server_stats": (c_int, [c_void_p, P(c_int64), c_int]),
     "gpjson_server_destroy": (None, [c_void_p]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),

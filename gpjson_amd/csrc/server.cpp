@@ -14,8 +14,12 @@
 //     queries run on the serving stream;
 //   - the batching thread coalesces every pending request whose file is
 //     resident into ONE interpreter launch (continuous batching: requests that
-//     arrive while batches are on the GPU join the next batch; up to two
-//     batches are in flight, each with its own span buffers).
+//     arrive while batches are on the GPU join the next batch; up to kSlots
+//     batches are in flight, each with its own span buffers);
+//   - a finished request's spans stay in its batch's pinned buffer until the
+//     client collects them (one copy, made by the waiting thread outside the
+//     server lock); when every slot holds uncollected results and requests are
+//     pending, the oldest slot's results are moved to per-request host memory.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -53,6 +57,8 @@ struct ServedFile {
   Clock::time_point last_use{};
 };
 
+constexpr int kSlots = 4;
+
 struct Request {
   uint64_t ticket = 0;
   std::string key;
@@ -64,7 +70,10 @@ struct Request {
   std::string error;
   int64_t lines = 0;
   int R = 0;
-  std::vector<int64_t> spans;
+  std::vector<int64_t> spans;  // results moved out of a batch buffer
+  int slot = -1;               // batch slot whose pinned buffer holds the results
+  int64_t off = 0;             // element offset in that buffer
+  bool collecting = false;     // a waiter is copying from the slot
   int64_t file_bytes = 0;
 };
 
@@ -77,7 +86,9 @@ struct Batch {
   void* host_jobs = nullptr;   // pinned staging of the job table + IR bytes
   size_t host_jobs_cap = 0;
   hipEvent_t done = nullptr;
-  bool active = false;
+  bool active = false;           // launched, not yet complete
+  int outstanding = 0;           // completed requests whose spans are still here
+  uint64_t seq = 0;              // launch order (oldest is spilled first)
   std::vector<int64_t> offsets;  // element offset of each request's spans
 };
 
@@ -100,9 +111,10 @@ struct gpjson_server {
   int64_t resident_bytes = 0;
   bool stop = false;
   std::thread batcher, admitter;
-  Batch slots[2];
+  Batch slots[kSlots];
+  uint64_t batch_seq = 0;
   // stats
-  int64_t launches = 0, batched_requests = 0, admissions = 0, evictions = 0, max_batch = 0;
+  int64_t launches = 0, batched_requests = 0, admissions = 0, evictions = 0, max_batch = 0, spills = 0;
   std::string fatal;
 };
 
@@ -182,13 +194,15 @@ void admitter_main(gpjson_server* s) {
   }
 }
 
-// Completes a finished batch: copies spans into the requests.  Lock held.
-void finish_batch(gpjson_server* s, Batch& b) {
+// Completes a finished batch: its requests' spans are now in the slot's pinned
+// buffer, where waiters read them.  Lock held.
+void finish_batch(gpjson_server* s, int k) {
+  Batch& b = s->slots[k];
   const auto now = Clock::now();
   for (size_t i = 0; i < b.reqs.size(); i++) {
     Request& r = *b.reqs[i];
-    const int64_t cnt = r.lines * r.R * 2;
-    r.spans.assign((int64_t*)b.host_spans + b.offsets[i], (int64_t*)b.host_spans + b.offsets[i] + cnt);
+    r.slot = k;
+    r.off = b.offsets[i];
     r.completed = now;
     r.state = 2;
   }
@@ -196,11 +210,35 @@ void finish_batch(gpjson_server* s, Batch& b) {
     f->inflight--;
     f->last_use = now;
   }
-  b.reqs.clear();
+  b.outstanding = (int)b.reqs.size();
   b.files.clear();
   b.active = false;
+  if (b.outstanding == 0) b.reqs.clear();
   s->done_cv.notify_all();
   s->admit_cv.notify_all();
+}
+
+// Moves the uncollected results of the oldest completed slot into per-request
+// host memory so the slot can take a new batch.  Requests that a waiter is
+// copying right now keep the slot busy until that copy ends.  Lock held.
+void spill_oldest(gpjson_server* s) {
+  int k = -1;
+  for (int j = 0; j < kSlots; j++) {
+    const Batch& b = s->slots[j];
+    if (!b.active && b.outstanding > 0 && (k < 0 || b.seq < s->slots[k].seq)) k = j;
+  }
+  if (k < 0) return;
+  Batch& b = s->slots[k];
+  for (auto& rp : b.reqs) {
+    Request& r = *rp;
+    if (r.slot != k || r.collecting) continue;
+    const int64_t cnt = r.lines * r.R * 2;
+    r.spans.assign((int64_t*)b.host_spans + r.off, (int64_t*)b.host_spans + r.off + cnt);
+    r.slot = -1;
+    b.outstanding--;
+  }
+  if (b.outstanding == 0) b.reqs.clear();
+  s->spills++;
 }
 
 // Launches one interpreter over every request in `reqs` (files resident).  Lock
@@ -297,11 +335,19 @@ void batcher_main(gpjson_server* s) {
       for (;;) {
         if (s->stop) return;
         // complete finished batches
-        for (Batch& b : s->slots)
-          if (b.active && hipEventQuery(b.done) == hipSuccess) finish_batch(s, b);
+        for (int k = 0; k < kSlots; k++)
+          if (s->slots[k].active && hipEventQuery(s->slots[k].done) == hipSuccess) finish_batch(s, k);
         free_slot = -1;
-        for (int k = 0; k < 2; k++)
-          if (!s->slots[k].active) { free_slot = k; break; }
+        for (int k = 0; k < kSlots; k++)
+          if (!s->slots[k].active && s->slots[k].outstanding == 0) { free_slot = k; break; }
+        if (free_slot < 0 && !s->pending.empty()) {
+          bool any_active = false;
+          for (const Batch& b : s->slots) any_active |= b.active;
+          // slow collectors must not stall the GPU: free the oldest slot
+          if (!any_active) spill_oldest(s);
+          for (int k = 0; k < kSlots; k++)
+            if (!s->slots[k].active && s->slots[k].outstanding == 0) { free_slot = k; break; }
+        }
         // requests whose file is resident can run now; others wait for admission
         if (free_slot >= 0) {
           for (auto it = s->pending.begin(); it != s->pending.end();) {
@@ -323,7 +369,8 @@ void batcher_main(gpjson_server* s) {
           }
         }
         if (!take.empty()) break;
-        bool any_active = s->slots[0].active || s->slots[1].active;
+        bool any_active = false;
+        for (const Batch& b : s->slots) any_active |= b.active;
         if (any_active) {
           lk.unlock();
           std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -336,6 +383,7 @@ void batcher_main(gpjson_server* s) {
       b.reqs = take;
       b.files = take_files;
       b.active = true;
+      b.seq = ++s->batch_seq;
       for (auto& r : take) r->state = 1;
       s->launches++;
       s->batched_requests += (int64_t)take.size();
@@ -459,27 +507,42 @@ int gpjson_server_wait(gpjson_server* s, uint64_t ticket, int64_t timeout_ms, in
   if (R) *R = r->R;
   if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->completed - r->submitted).count();
   if (spans) {
-    if (cap_elems < (int64_t)r->spans.size()) return fail(GPJSON_ERR_CAPACITY, "span buffer too small");
-    std::memcpy(spans, r->spans.data(), r->spans.size() * 8);
+    const int64_t cnt = r->lines * r->R * 2;
+    if (cap_elems < cnt) return fail(GPJSON_ERR_CAPACITY, "span buffer too small");
+    if (r->collecting) return fail(GPJSON_ERR_INVALID_ARG, "ticket is being collected by another thread");
     s->requests.erase(ticket);
+    if (r->slot < 0) {
+      std::memcpy(spans, r->spans.data(), (size_t)cnt * 8);
+      return GPJSON_OK;
+    }
+    // the slot is not reused while this request is outstanding: copy unlocked
+    Batch& b = s->slots[r->slot];
+    r->collecting = true;
+    lk.unlock();
+    std::memcpy(spans, (const int64_t*)b.host_spans + r->off, (size_t)cnt * 8);
+    lk.lock();
+    r->collecting = false;
+    r->slot = -1;
+    if (--b.outstanding == 0) {
+      b.reqs.clear();
+      s->cv.notify_all();
+    }
   }
   return GPJSON_OK;
 }
 
-// Counters: resident bytes, resident files, interpreter launches, requests
-// served in them, largest batch, admissions, evictions.
-int gpjson_server_stats(gpjson_server* s, int64_t* out7) {
-  if (!s || !out7) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+// Counters, in this order: resident bytes, resident files, interpreter
+// launches, requests served in them, largest batch, admissions, evictions,
+// spills (slots whose uncollected results were moved to host memory).  Fills
+// min(n, 8) values.
+int gpjson_server_stats(gpjson_server* s, int64_t* out, int n) {
+  if (!s || !out || n < 0) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(s->mu);
   int64_t res_files = 0;
   for (auto& kv : s->files) res_files += kv.second->resident ? 1 : 0;
-  out7[0] = s->resident_bytes;
-  out7[1] = res_files;
-  out7[2] = s->launches;
-  out7[3] = s->batched_requests;
-  out7[4] = s->max_batch;
-  out7[5] = s->admissions;
-  out7[6] = s->evictions;
+  const int64_t v[8] = {s->resident_bytes, res_files, s->launches, s->batched_requests, s->max_batch,
+                        s->admissions, s->evictions, s->spills};
+  for (int i = 0; i < n && i < 8; i++) out[i] = v[i];
   return GPJSON_OK;
 }
 

@@ -42,6 +42,13 @@ class Server:
         _check(self._lib.gpjson_server_submit(self._h, key.encode(), path.encode(), ctypes.byref(t)))
         return t.value
 
+    def ready(self, ticket: int, timeout_ms: int = -1) -> Tuple[int, int, float]:
+        """Waits for a ticket without collecting it: (lines, results per line, latency us)."""
+        lines, R, lat = ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.gpjson_server_wait(self._h, ticket, timeout_ms, ctypes.byref(lines), ctypes.byref(R),
+                                            ctypes.byref(lat), None, 0))
+        return lines.value, R.value, lat.value
+
     def wait(self, ticket: int, timeout_ms: int = -1) -> Tuple[np.ndarray, float]:
         """(spans int64[lines][R][2], latency in microseconds) of a ticket."""
         lines, R, lat = ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
@@ -53,9 +60,10 @@ class Server:
         return out, lat.value
 
     def stats(self) -> dict:
-        v = (ctypes.c_int64 * 7)()
-        _check(self._lib.gpjson_server_stats(self._h, v))
-        keys = ("resident_bytes", "resident_files", "launches", "requests", "max_batch", "admissions", "evictions")
+        v = (ctypes.c_int64 * 8)()
+        _check(self._lib.gpjson_server_stats(self._h, v, 8))
+        keys = ("resident_bytes", "resident_files", "launches", "requests", "max_batch", "admissions", "evictions",
+                "spills")
         return dict(zip(keys, list(v)))
 
     def close(self) -> None:

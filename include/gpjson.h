@@ -216,9 +216,10 @@ int gpjson_server_submit(gpjson_server *server, const char *key, const char *pat
  * NULL) copies int64[lines][R][2] spans, after which the ticket is forgotten. */
 int gpjson_server_wait(gpjson_server *server, uint64_t ticket, int64_t timeout_ms, int64_t *lines, int *R,
                        double *latency_us, int64_t *spans, int64_t cap_elems);
-/* out7: resident bytes, resident files, interpreter launches, requests served
- * by them, largest batch, admissions, evictions. */
-int gpjson_server_stats(gpjson_server *server, int64_t *out7);
+/* Fills min(n, 8) counters: resident bytes, resident files, interpreter
+ * launches, requests served by them, largest batch, admissions, evictions,
+ * spills (batches whose uncollected results were moved to host memory). */
+int gpjson_server_stats(gpjson_server *server, int64_t *out, int n);
 void gpjson_server_destroy(gpjson_server *server);
 
 /* ---- launch shapes and the autotuner -------------------------------------- */

@@ -83,6 +83,27 @@ def test_alone_and_in_mix_equal_plain(files, plain):
         assert st["resident_bytes"] <= int(total * 0.9)
 
 
+def test_uncollected_results_spill(files, plain):
+    """Results left uncollected in every batch slot are moved to host memory
+    when new requests need a slot, and still come back bit-exact."""
+    from gpjson_amd.serve import Server
+    with Server(hbm_budget_bytes=1 << 34, levels=4) as srv:
+        for key, (_, data) in files.items():
+            srv.add(key, data)
+        reqs = _reqs(files, 12, 21)
+        tickets = []
+        for key, p in reqs:  # one request per batch, none collected yet
+            t = srv.submit(key, p)
+            srv.ready(t, 60000)
+            tickets.append(t)
+        st = srv.stats()
+        print(f"[serve] spill stats {st}", flush=True)
+        assert st["spills"] > 0, st
+        for t, (key, p) in zip(tickets, reqs):
+            sp, _ = srv.wait(t, 60000)
+            assert np.array_equal(sp, plain[(key, p)]), (key, p)
+
+
 def test_open_loop_poisson(files, plain):
     from gpjson_amd.serve import Server, open_loop
     with Server(hbm_budget_bytes=1 << 34, levels=4) as srv:
