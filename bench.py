@@ -70,16 +70,21 @@ def cpu_baseline(data, paths, sample_bytes: int, threads: int):
     }
 
 
-def load_traffic(L_bytes_per_launch: float):
-    """HBM bytes per index launch from the committed rocprofv3 PMC summary, if present."""
+def load_traffic(kind: str, size: int, seed: int, paths):
+    """HBM bytes per index launch from the committed rocprofv3 PMC summary, when it
+    was measured on this same workload (else None)."""
     path = os.path.join(ROOT, "profiles", "index_kernel_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    except Exception:
+            d = json.load(f)
+    except (OSError, ValueError):
         return None
+    w = d.get("workload", {})
+    if (w.get("kind"), w.get("size"), w.get("seed"), w.get("paths")) != (kind, size, seed, list(paths)):
+        return None
+    return d.get("bytes_per_launch")
 
 
 TWEET_PATHS_8 = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name",
@@ -174,7 +179,7 @@ def run_plain(args) -> int:
         log(f"{cp.path}: {int(np.sum(got[:, 0, 0] >= 0))} of {lines} lines have a result; "
             f"equal to the one-shot path: {same}")
 
-    traffic = load_traffic(data.size)
+    traffic = load_traffic(args.kind, args.size, args.seed, paths)
     result = common_fields(args, value, dt)
     result.update({
         "data": f"synthetic seeded {args.kind} NDJSON (seed {args.seed}), resident in HBM",
@@ -206,6 +211,10 @@ def run_plain(args) -> int:
     for p in plans:
         p.free()
     buf.free()
+    copy_ms = ctx.time_copy(1 << 30, 5)
+    result["hbm_copy_GBps"] = round(2 * (1 << 30) / (copy_ms / 1e3) / 1e9, 1)
+    result["roofline"]["frac_of_copy"] = round(achieved / result["hbm_copy_GBps"], 4)
+    log(f"streaming copy of 1 GiB: {copy_ms:.3f} ms = {result['hbm_copy_GBps']} GB/s (read + write)")
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {min(args.cpu_sample, data.size)} bytes with {threads} threads")
@@ -449,12 +458,32 @@ def run_serve(args) -> int:
     return 0
 
 
+def run_copy(args) -> int:
+    """Streaming device copy of --size bytes: the achievable HBM bandwidth, and the
+    known-traffic kernel (reads and writes --size bytes per launch) used to
+    calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import gpjson_amd
+    ctx = gpjson_amd.Context(0)
+    ms = ctx.time_copy(args.size, max(args.steps, 1))
+    gbps = 2 * args.size / (ms / 1e3) / 1e9
+    result = common_fields(args, gbps, ms / 1e3 * args.steps)
+    result.update({
+        "data": "device buffer (0x5a bytes)",
+        "config": {"workload": f"streaming copy, {args.size} bytes read and written per launch"},
+        "ms_per_copy": round(ms, 4),
+        "frac_of_hbm_peak": round(gbps / HBM_PEAK_GBPS, 4),
+    })
+    print(json.dumps(result), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy"])
     ap.add_argument("--size", type=int, default=4 << 30, help="plain: input bytes (default 4 GiB)")
     ap.add_argument("--kind", default="tweets", choices=["tweets", "adversarial"])
     ap.add_argument("--seed", type=int, default=1)
@@ -483,6 +512,8 @@ def main() -> int:
     if args.mode == "serve":
         args.files = args.files or 64
         return run_serve(args)
+    if args.mode == "copy":
+        return run_copy(args)
     return run_plain(args)
 
 

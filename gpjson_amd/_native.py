@@ -90,6 +90,7 @@ SIGNATURES = {
                                    c_int64]),
     "gpjson_server_stats": (c_int, [c_void_p, P(c_int64), c_int]),
     "gpjson_server_destroy": (None, [c_void_p]),
+    "gpjson_time_copy": (c_int, [c_void_p, c_int64, c_int, P(c_double)]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),

@@ -210,6 +210,12 @@ class Context:
         _check(self._lib.gpjson_autotune(self._h, buf.handle, ptrs, lens, nq, iterations, rep, 16384))
         return rep.value.decode()
 
+    def time_copy(self, nbytes: int, iterations: int = 5) -> float:
+        """ms per device-to-device streaming copy of nbytes (reads and writes nbytes)."""
+        ms = ctypes.c_double()
+        _check(self._lib.gpjson_time_copy(self._h, nbytes, iterations, ctypes.byref(ms)))
+        return ms.value
+
     def time_plain(self, buf: "Buffer", irs: Sequence[bytes], iterations: int):
         """(ms per index+query pass, index ms, query ms) measured with HIP events on the library stream."""
         nq = len(irs)

@@ -25,6 +25,7 @@ SOURCES = [
     "batch.cpp",
     "server.cpp",
     "autotune.cpp",
+    "diag.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

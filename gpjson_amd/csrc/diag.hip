@@ -1,0 +1,71 @@
+// diag.hip -- streaming copy used to measure the achievable HBM bandwidth of
+// the device and to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+// against a kernel whose traffic is known exactly (n bytes read, n written).
+// No counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include "runtime.h"
+
+namespace gj {
+namespace {
+
+// Grid-stride copy of 16-byte vectors; n16 vectors, then a byte tail.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                          int64_t n16, const uint8_t* __restrict__ src_b,
+                                                          uint8_t* __restrict__ dst_b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  const int64_t t = n16 * 16 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst_b[t] = src_b[t];
+}
+
+}  // namespace
+}  // namespace gj
+
+using namespace gj;
+
+extern "C" {
+
+int gpjson_time_copy(gpjson_ctx* ctx, int64_t n, int iterations, double* ms_per_copy) {
+  if (!ctx || n <= 0 || iterations < 1 || !ms_per_copy) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  GJ_HIP(hipSetDevice(ctx->device));
+  DevBuf a, b;
+  int rc = a.ensure((size_t)n);
+  if (rc == GPJSON_OK) rc = b.ensure((size_t)n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == GPJSON_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(GPJSON_ERR_HIP, "hipEventCreate");
+  const int64_t n16 = n / 16;
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>((n16 + 255) / 256, 1), (int64_t)ctx->num_cus * 32);
+  auto launch = [&]() {
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const uint4*)a.p, (uint4*)b.p,
+                       n16, (const uint8_t*)a.p, (uint8_t*)b.p, n);
+    return hipGetLastError();
+  };
+  if (rc == GPJSON_OK && hipMemsetAsync(a.p, 0x5a, (size_t)n, ctx->stream) != hipSuccess)
+    rc = fail(GPJSON_ERR_HIP, "hipMemsetAsync");
+  if (rc == GPJSON_OK && launch() != hipSuccess) rc = fail(GPJSON_ERR_HIP, "stream_copy_kernel");  // warm-up
+  if (rc == GPJSON_OK) {
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int i = 0; i < iterations && rc == GPJSON_OK; i++)
+      if (launch() != hipSuccess) rc = fail(GPJSON_ERR_HIP, "stream_copy_kernel");
+    (void)hipEventRecord(e1, ctx->stream);
+  }
+  if (rc == GPJSON_OK) {
+    hipError_t e = hipEventSynchronize(e1);
+    if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
+  }
+  if (rc == GPJSON_OK) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = ms / iterations;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(ctx->stream);
+  a.release();
+  b.release();
+  return rc;
+}
+
+}  // extern "C"

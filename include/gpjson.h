@@ -248,6 +248,12 @@ int gpjson_autotune(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs,
  * split of the last pass. */
 int gpjson_time_plain(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens,
                       int nq, int iterations, double *ms_per_pass, double *index_ms, double *query_ms);
+/* Device-to-device streaming copy of n bytes (16-byte vector loads and
+ * stores), timed with HIP events over `iterations` launches: the achievable
+ * HBM bandwidth is 2n / ms_per_copy, and the kernel is the known-traffic
+ * reference for calibrating FETCH_SIZE / WRITE_SIZE counters.  No counterpart
+ * in the reference. */
+int gpjson_time_copy(gpjson_ctx *ctx, int64_t n, int iterations, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
