@@ -25,6 +25,8 @@ GPJSON_INDEX_STRING = 1
 GPJSON_INDEX_LEVELED = 2
 GPJSON_INDEX_ESCAPE = 3
 GPJSON_INDEX_QUOTE = 4
+GPJSON_INDEX_COLON = 5
+GPJSON_INDEX_RAW_QUOTE = 6
 
 GPJSON_BUILD_VERIFY = 1
 GPJSON_QUERY_DISABLE_FALLBACK = 1

@@ -382,6 +382,14 @@ class Index:
     def quote_index(self) -> np.ndarray:
         return self.export(N.GPJSON_INDEX_QUOTE)
 
+    def colon_bitmap(self) -> np.ndarray:
+        """Every ':' byte (the interpreter's key candidates)."""
+        return self.export(N.GPJSON_INDEX_COLON)
+
+    def raw_quote_bitmap(self) -> np.ndarray:
+        """Every '"' byte, escaped or not (the interpreter's key-start test)."""
+        return self.export(N.GPJSON_INDEX_RAW_QUOTE)
+
     def free(self) -> None:
         if self._h and self.ctx._h:
             N.load().gpjson_index_free(self._h)

@@ -37,7 +37,7 @@ struct gpjson_batch {
   std::vector<int> file_first_job;
   int64_t total_tiles = 0;
   // device memory
-  DevBuf arena_s, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
+  DevBuf arena_s, arena_rc, arena_rq, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
   int64_t* total_dev = nullptr;
   int64_t* lines_dev = nullptr;
   int* overflow_dev = nullptr;
@@ -109,6 +109,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
   }
   b->total_tiles = tiles;
   int rc = b->arena_s.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
+  if (!rc) rc = b->arena_rc.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
+  if (!rc) rc = b->arena_rq.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
   if (!rc) rc = b->arena_lb.ensure((size_t)std::max<int64_t>(lb_words, 1) * 8);
   if (!rc) rc = b->arena_nl.ensure((size_t)std::max<int64_t>(nl_elems, 1) * 8);
   if (!rc) rc = b->arena_spans.ensure((size_t)std::max<int64_t>(span_elems, 1) * 8);
@@ -121,6 +123,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     FileJob& J = b->files[f];
     J.F = b->bufs[f]->d;
     J.S = (uint64_t*)b->arena_s.p + s_off[f];
+    J.RC = (uint64_t*)b->arena_rc.p + s_off[f];
+    J.RQ = (uint64_t*)b->arena_rq.p + s_off[f];
     J.LB = (uint64_t*)b->arena_lb.p + lb_off[f];
     J.NL = (int64_t*)b->arena_nl.p + nl_off[f];
     J.E = nullptr;
@@ -143,6 +147,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     q.lines = 0;
     q.S = J.S;
     q.LB = J.LB;
+    q.RC = J.RC;
+    q.RQ = J.RQ;
     q.Wp = J.Wp;
     q.L = J.L;
     q.R = b->job_R[j];
@@ -363,8 +369,8 @@ void gpjson_batch_free(gpjson_batch* b) {
   drop_graph(b);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
-  DevBuf* bufs[] = {&b->arena_s, &b->arena_lb, &b->arena_nl, &b->arena_spans, &b->dev_files,
-                    &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states};
+  DevBuf* bufs[] = {&b->arena_s, &b->arena_rc, &b->arena_rq, &b->arena_lb, &b->arena_nl, &b->arena_spans,
+                    &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states};
   for (DevBuf* d : bufs) d->release();
   delete b;
 }

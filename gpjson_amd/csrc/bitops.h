@@ -84,6 +84,7 @@ struct WordClass {
   uint64_t O;   // '{' '['
   uint64_t C;   // '}' ']'
   uint64_t P;   // ':' ','
+  uint64_t K;   // ':'
 };
 
 // Per byte of t: bit 7 set iff the byte's low seven bits equal k (k < 0x80).
@@ -112,7 +113,7 @@ GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
 // j+8, j+16, j+24; a class's per-byte marker (bit 7) then lands on its natural
 // bit position with one right shift by 7-j.
 GJ_HD void classify32(const uint32_t d[8], uint32_t& B, uint32_t& Q, uint32_t& NL, uint32_t& O,
-                      uint32_t& C, uint32_t& P) {
+                      uint32_t& C, uint32_t& P, uint32_t& K) {
   uint32_t t[8];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
@@ -126,7 +127,7 @@ GJ_HD void classify32(const uint32_t d[8], uint32_t& B, uint32_t& Q, uint32_t& N
     t[4 * h + 2] = perm_b32(u3, u1, 0x05040100u);
     t[4 * h + 3] = perm_b32(u3, u1, 0x07060302u);
   }
-  uint32_t b = 0, q = 0, nl = 0, o = 0, c = 0, p = 0;
+  uint32_t b = 0, q = 0, nl = 0, o = 0, c = 0, p = 0, k = 0;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     uint32_t x = t[j];
@@ -138,16 +139,18 @@ GJ_HD void classify32(const uint32_t d[8], uint32_t& B, uint32_t& Q, uint32_t& N
     nl |= (eq7(lo7, 0x0A0A0A0Au) >> (7 - j)) & nv;
     o |= (eq7(lo7c, 0x7B7B7B7Bu) >> (7 - j)) & nv;
     c |= (eq7(lo7c, 0x7D7D7D7Du) >> (7 - j)) & nv;
-    p |= ((eq7(lo7, 0x3A3A3A3Au) | eq7(lo7, 0x2C2C2C2Cu)) >> (7 - j)) & nv;
+    const uint32_t colon = eq7(lo7, 0x3A3A3A3Au);
+    p |= ((colon | eq7(lo7, 0x2C2C2C2Cu)) >> (7 - j)) & nv;
+    k |= (colon >> (7 - j)) & nv;
   }
-  B = b; Q = q; NL = nl; O = o; C = c; P = p;
+  B = b; Q = q; NL = nl; O = o; C = c; P = p; K = k;
 }
 
 // Classifies one 64-byte word given as 16 little-endian dwords.
 GJ_HD WordClass classify64(const uint32_t d[16]) {
-  uint32_t b0, q0, n0, o0, c0, p0, b1, q1, n1, o1, c1, p1;
-  classify32(d, b0, q0, n0, o0, c0, p0);
-  classify32(d + 8, b1, q1, n1, o1, c1, p1);
+  uint32_t b0, q0, n0, o0, c0, p0, k0, b1, q1, n1, o1, c1, p1, k1;
+  classify32(d, b0, q0, n0, o0, c0, p0, k0);
+  classify32(d + 8, b1, q1, n1, o1, c1, p1, k1);
   WordClass w;
   w.B = ((uint64_t)b1 << 32) | b0;
   w.Q = ((uint64_t)q1 << 32) | q0;
@@ -155,6 +158,7 @@ GJ_HD WordClass classify64(const uint32_t d[16]) {
   w.O = ((uint64_t)o1 << 32) | o0;
   w.C = ((uint64_t)c1 << 32) | c0;
   w.P = ((uint64_t)p1 << 32) | p0;
+  w.K = ((uint64_t)k1 << 32) | k0;
   return w;
 }
 

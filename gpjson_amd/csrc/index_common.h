@@ -31,6 +31,8 @@ struct FileJob {
   int64_t nl_cap;
   uint64_t* E;          // escape index (verify mode, may be null)
   uint64_t* Q;          // quote index (verify mode, may be null)
+  uint64_t* RC;         // raw ':' bytes, W words (key candidates for the interpreter)
+  uint64_t* RQ;         // raw '"' bytes, W words (key-start test of the interpreter)
   int64_t* tile_lines;  // optional: first line id starting in each tile, [ntiles + 1]
   int64_t tile0;        // first global tile id of this file
   int64_t ntiles;

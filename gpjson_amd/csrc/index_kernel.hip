@@ -316,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
     __syncthreads();
 
     // ---- classify this thread's words ----
-    // Each word's six masks replace its (already consumed) raw bytes in this
+    // Each word's seven masks replace its (already consumed) raw bytes in this
     // thread's own LDS slots, so they do not occupy registers across the scan
     // and lookback phases.
     auto slot = [&](int c) { return tid * CH + (c ^ (tid & (CH - 1))); };
@@ -337,6 +337,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       tile[slot(4 * k + 0)] = make_uint4(lo32(w.B), hi32(w.B), lo32(w.Q), hi32(w.Q));
       tile[slot(4 * k + 1)] = make_uint4(lo32(w.NL), hi32(w.NL), lo32(w.O), hi32(w.O));
       tile[slot(4 * k + 2)] = make_uint4(lo32(w.C), hi32(w.C), lo32(w.P), hi32(w.P));
+      tile[slot(4 * k + 3)] = make_uint4(lo32(w.K), hi32(w.K), 0u, 0u);
       code[k] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
       if (code[k]) tex = code[k];
     }
@@ -471,6 +472,11 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       uint64_t Qe = wd.Q & ~E;
       uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
       J.S[w] = S;
+      if (J.RC) {
+        const uint4 kk = tile[slot(4 * k + 3)];
+        J.RC[w] = u64(kk.x, kk.y) & valid;
+      }
+      if (J.RQ) J.RQ[w] = wd.Q & valid;
       if (VERIFY) {
         if (J.E) J.E[w] = E & valid;
         if (J.Q) J.Q[w] = Qe & valid;

@@ -24,6 +24,8 @@ struct QueryJob {
   int64_t lines;
   const uint64_t* S;  // string index
   const uint64_t* LB; // leveled bitmaps, level stride Wp
+  const uint64_t* RC; // raw ':' bytes (bit per input byte)
+  const uint64_t* RQ; // raw '"' bytes (bit per input byte)
   int64_t Wp;
   int L;              // levels present in LB
   int R;              // result slots per line

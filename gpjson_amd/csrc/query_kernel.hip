@@ -108,35 +108,44 @@ __device__ __forceinline__ uint64_t window_word(const Words& W, int lvl, int64_t
   return v;
 }
 
+// Words a lane loads per step in the scans below: independent loads issued
+// together, so one memory round trip covers kUnroll words of a line.
+constexpr int kUnroll = 4;
+
 // First set bit of level lvl in [from, to), or -1.
 template <int G, class Words>
 __device__ int64_t group_first_bit(const Group<G>& g, const Words& W, int lvl, int64_t from, int64_t to) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
-    const int64_t wi = b0 + g.r;
-    const uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
-    const unsigned long long m = g.ballot(v != 0);
-    if (m) {
-      const int l = ctz64(m);
-      const uint64_t vv = g.shfl_u64(v, l);
-      return ((b0 + l) << 6) + ctz64(vv);
-    }
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kUnroll) {
+    const int64_t w0 = b0 + (int64_t)g.r * kUnroll;
+    uint64_t v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) v[u] = window_word(W, lvl, w0 + u, wbeg, wend, from, to);
+    int64_t found = -1;
+#pragma unroll
+    for (int u = kUnroll - 1; u >= 0; u--)
+      if (v[u]) found = ((w0 + u) << 6) + ctz64(v[u]);
+    const unsigned long long m = g.ballot(found >= 0);
+    if (m) return g.shfl_i64(found, ctz64(m));
   }
   return -1;
 }
 
-// Key test at a ':' at position p (K/find_value.cu:227-250): the last in-string
-// byte before p must end a string of exactly klen bytes that starts with '"'
-// after the line start, and those bytes must equal the key.
+// Key test at a ':' at position p (K/find_value.cu:227-250): e = the last
+// in-string byte in (start, p); the key must start with a '"' at ss = e - klen
+// (ss >= start) and the klen bytes after it must equal the key.  The quote test
+// reads the raw-quote bitmap, so input bytes are read only for candidates
+// whose length already fits.  sw / qw: string and raw-quote words of p's word.
 template <class Words>
-__device__ bool key_matches(const uint8_t* F, const Words& W, int64_t start, int64_t p, const uint8_t* key,
-                            int klen) {
+__device__ bool key_matches(const uint8_t* F, const uint64_t* RQ, const Words& W, int64_t start, int64_t p,
+                            uint64_t sw_p, uint64_t qw_p, const uint8_t* key, int klen) {
   const int64_t k = p - 1;
   if (k <= start) return false;
+  const int64_t wp = p >> 6;
   int64_t wi = k >> 6;
   const int64_t wstart = start >> 6;
-  uint64_t sw = W.s(wi) & mask_upto((int)(k & 63));
+  uint64_t sw = (wi == wp ? sw_p : W.s(wi)) & mask_upto((int)(k & 63));
   int64_t se;
   for (;;) {
     if (wi == wstart) sw &= ~mask_upto((int)(start & 63));
@@ -146,27 +155,48 @@ __device__ bool key_matches(const uint8_t* F, const Words& W, int64_t start, int
     sw = W.s(wi);
   }
   const int64_t ss = se - klen;
-  if (ss < start || F[ss] != '"') return false;
+  if (ss < start) return false;
+  const int64_t qi = ss >> 6;
+  const uint64_t qw = qi == wp ? qw_p : RQ[qi];
+  if (!((qw >> (ss & 63)) & 1ull)) return false;
   for (int i = 0; i < klen; i++)
     if (key[i] != F[ss + 1 + i]) return false;
   return true;
 }
 
 // First structural ':' in [from, to) at this level whose key matches, or -1.
+// Candidates are the level's bits that are ':' bytes (colon bitmap), so no
+// input byte is read to find them.
 template <int G, class Words>
-__device__ int64_t group_find_key(const Group<G>& g, const Words& W, int lvl, const uint8_t* F, int64_t start,
-                                  int64_t from, int64_t to, const uint8_t* key, int klen) {
+__device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint64_t* RC, const uint64_t* RQ, int lvl,
+                                  const uint8_t* F, int64_t start, int64_t from, int64_t to, const uint8_t* key,
+                                  int klen) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
-    const int64_t wi = b0 + g.r;
-    uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kUnroll) {
+    const int64_t w0 = b0 + (int64_t)g.r * kUnroll;
+    uint64_t v[kUnroll], sw[kUnroll], qw[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t wi = w0 + u;
+      v[u] = 0;
+      sw[u] = 0;
+      qw[u] = 0;
+      if (wi <= wend) {
+        v[u] = window_word(W, lvl, wi, wbeg, wend, from, to) & RC[wi];
+        sw[u] = W.s(wi);
+        qw[u] = RQ[wi];
+      }
+    }
     int64_t found = -1;
-    while (v) {
-      const int b = ctz64(v);
-      const int64_t p = (wi << 6) + b;
-      if (F[p] == ':' && key_matches(F, W, start, p, key, klen)) { found = p; break; }
-      v &= v - 1;
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      uint64_t t = found < 0 ? v[u] : 0;
+      while (t) {
+        const int64_t p = ((w0 + u) << 6) + ctz64(t);
+        if (key_matches(F, RQ, W, start, p, sw[u], qw[u], key, klen)) { found = p; break; }
+        t &= t - 1;
+      }
     }
     const unsigned long long m = g.ballot(found >= 0);
     if (m) return g.shfl_i64(found, ctz64(m));
@@ -325,7 +355,7 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
     int64_t p = -1;
     if (lvl < J.L) {
       if (op == kMoveToKey) {
-        p = group_find_key(g, W, lvl, F, start, j, to, want, want_len);
+        p = group_find_key(g, W, J.RC, J.RQ, lvl, F, start, j, to, want, want_len);
       } else if (op == kMoveToIndex) {
         int cnt = ci[lvl];
         p = group_find_comma(g, W, lvl, F, j, to, &cnt, want_idx);

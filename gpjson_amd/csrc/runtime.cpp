@@ -180,6 +180,8 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   j.nl_cap = idx->nl_cap;
   j.E = idx->E;
   j.Q = idx->Q;
+  j.RC = idx->RC;
+  j.RQ = idx->RQ;
   j.tile_lines = idx->tile_lines;
   j.tile0 = 0;
   j.ntiles = ntiles;
@@ -232,6 +234,8 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.lines = tiled ? idx->lines : 0;
     j.S = idx->S;
     j.LB = idx->LB;
+    j.RC = idx->RC;
+    j.RQ = idx->RQ;
     j.Wp = idx->Wp;
     j.L = idx->L;
     j.R = R;
@@ -294,10 +298,10 @@ static int read_lines(gpjson_index* idx) {
 }
 
 static void free_index_arrays(gpjson_index* idx) {
-  void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->lines_dev, idx->tile_lines};
+  void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->RC, idx->RQ, idx->lines_dev, idx->tile_lines};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  idx->S = idx->LB = idx->E = idx->Q = nullptr;
+  idx->S = idx->LB = idx->E = idx->Q = idx->RC = idx->RQ = nullptr;
   idx->NL = idx->lines_dev = idx->tile_lines = nullptr;
 }
 
@@ -502,6 +506,8 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
     if (rc == GPJSON_OK) rc = dev_alloc(p, bytes);
   };
   alloc((void**)&idx->S, (size_t)idx->W * 8);
+  alloc((void**)&idx->RC, (size_t)idx->W * 8);
+  alloc((void**)&idx->RQ, (size_t)idx->W * 8);
   alloc((void**)&idx->LB, (size_t)idx->Wp * std::max(levels, 1) * 8);
   alloc((void**)&idx->NL, (size_t)idx->nl_cap * 8);
   alloc((void**)&idx->lines_dev, 64);
@@ -542,6 +548,8 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
     case GPJSON_INDEX_LEVELED: size = idx->W * 8 * idx->L; src = idx->LB; break;
     case GPJSON_INDEX_ESCAPE: size = idx->W * 8; src = idx->E; break;
     case GPJSON_INDEX_QUOTE: size = idx->W * 8; src = idx->Q; break;
+    case GPJSON_INDEX_COLON: size = idx->W * 8; src = idx->RC; break;
+    case GPJSON_INDEX_RAW_QUOTE: size = idx->W * 8; src = idx->RQ; break;
     default: return fail(GPJSON_ERR_INVALID_ARG, "unknown index array " + std::to_string(which));
   }
   if (size_bytes) *size_bytes = size;

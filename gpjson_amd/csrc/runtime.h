@@ -98,6 +98,8 @@ struct gpjson_index {
   int64_t nl_cap = 0;
   uint64_t* E = nullptr;
   uint64_t* Q = nullptr;
+  uint64_t* RC = nullptr;  // raw ':' bitmap (interpreter key candidates)
+  uint64_t* RQ = nullptr;  // raw '"' bitmap (interpreter key-start test)
   int64_t lines = 0;
   int64_t* lines_dev = nullptr;  // device copy of the line count
   int64_t* tile_lines = nullptr; // first line id of each index tile (+ total)

@@ -49,7 +49,9 @@ enum {
   GPJSON_INDEX_STRING = 1,  /* uint64[ceil(n/64)] */
   GPJSON_INDEX_LEVELED = 2, /* uint64[levels][ceil(n/64)], level-major */
   GPJSON_INDEX_ESCAPE = 3,  /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
-  GPJSON_INDEX_QUOTE = 4    /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
+  GPJSON_INDEX_QUOTE = 4,   /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
+  GPJSON_INDEX_COLON = 5,   /* uint64[ceil(n/64)]: every ':' byte (no reference counterpart) */
+  GPJSON_INDEX_RAW_QUOTE = 6 /* uint64[ceil(n/64)]: every '"' byte, escaped or not (no reference counterpart) */
 };
 
 /* Flags of gpjson_index_build. */

@@ -12,7 +12,7 @@ void bh_classify64(const uint8_t* bytes, uint64_t* out) {
     d[i] = (uint32_t)bytes[4 * i] | ((uint32_t)bytes[4 * i + 1] << 8) | ((uint32_t)bytes[4 * i + 2] << 16) |
            ((uint32_t)bytes[4 * i + 3] << 24);
   gj::WordClass w = gj::classify64(d);
-  out[0] = w.B; out[1] = w.Q; out[2] = w.NL; out[3] = w.O; out[4] = w.C; out[5] = w.P;
+  out[0] = w.B; out[1] = w.Q; out[2] = w.NL; out[3] = w.O; out[4] = w.C; out[5] = w.P; out[6] = w.K;
 }
 int64_t bh_leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L, uint64_t* out) {
   return gj::leveled_word<8>(O, C, P, d, lo, L, out);

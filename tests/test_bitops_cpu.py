@@ -106,7 +106,7 @@ def test_prefix_xor(lib):
 def test_classify64(lib):
     rng = np.random.default_rng(3)
     special = np.frombuffer(b'\\"\n{}[]:,', dtype=np.uint8)
-    out = np.zeros(6, dtype=np.uint64)
+    out = np.zeros(7, dtype=np.uint64)
     for t in range(4000):
         if t % 2:
             b = rng.integers(0, 256, 64).astype(np.uint8)
@@ -123,6 +123,7 @@ def test_classify64(lib):
         assert int(out[3]) == m((b == 0x7B) | (b == 0x5B))
         assert int(out[4]) == m((b == 0x7D) | (b == 0x5D))
         assert int(out[5]) == m((b == 0x3A) | (b == 0x2C))
+        assert int(out[6]) == m(b == 0x3A)
 
 
 def leveled_ref(O, C, P, d, lo, L):

@@ -24,6 +24,15 @@ def _p(msg):
     sys.stdout.flush()
 
 
+def byte_bitmap(data: bytes, c: int) -> np.ndarray:
+    """uint64 words with bit i%64 of word i/64 set where data[i] == c (closed form)."""
+    f = np.frombuffer(data, dtype=np.uint8)
+    W = (len(f) + 63) // 64
+    bits = np.zeros(W * 64, dtype=np.uint8)
+    bits[:len(f)] = f == c
+    return np.packbits(bits, bitorder="little").view("<u8").astype(np.uint64) if W else np.zeros(0, np.uint64)
+
+
 def check_index(ctx, data: bytes, levels: int, tag: str = ""):
     _p(f"[parity] index n={len(data)} L={levels} {tag}")
     buf = ctx.stage(data)
@@ -36,6 +45,8 @@ def check_index(ctx, data: bytes, levels: int, tag: str = ""):
     assert np.array_equal(idx.escape_index(), O.escape_index(data)), "escape index"
     assert np.array_equal(idx.quote_index(), O.quote_index(data)), "quote index"
     assert np.array_equal(idx.string_index(), s), "string index"
+    assert np.array_equal(idx.colon_bitmap(), byte_bitmap(data, 0x3A)), "colon bitmap"
+    assert np.array_equal(idx.raw_quote_bitmap(), byte_bitmap(data, 0x22)), "raw quote bitmap"
     got = idx.leveled_bitmaps()
     if not np.array_equal(got, lb):
         W = (len(data) + 63) // 64
