@@ -56,6 +56,12 @@ struct QueryLaunch {
   // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
   // tile prefix); a block stages a tile's leveled and string words in LDS.
   int tiled;
+  // Every job's IR lies in [ir_base, ir_base + ir_bytes).  When ir_bytes fits
+  // kIrLdsBytes, each block copies it to LDS once and interprets from there.
+  const uint8_t* ir_base;
+  int64_t ir_bytes;
 };
+
+constexpr int kIrLdsBytes = 4096;
 
 }  // namespace gj

@@ -159,8 +159,16 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* RQ, const Words& W
   const int64_t qi = ss >> 6;
   const uint64_t qw = qi == wp ? qw_p : RQ[qi];
   if (!((qw >> (ss & 63)) & 1ull)) return false;
-  for (int i = 0; i < klen; i++)
-    if (key[i] != F[ss + 1 + i]) return false;
+  // compare 8 bytes per round trip (the loads of a chunk are independent)
+  for (int i0 = 0; i0 < klen; i0 += 8) {
+    uint8_t f[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) f[i] = i0 + i < klen ? F[ss + 1 + i0 + i] : 0;
+    bool same = true;
+#pragma unroll
+    for (int i = 0; i < 8; i++) same &= !(i0 + i < klen) || f[i] == key[i0 + i];
+    if (!same) return false;
+  }
   return true;
 }
 
@@ -374,10 +382,17 @@ template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int nslots) {
   constexpr int NG = BLOCK / G;
   extern __shared__ int64_t q_lds[];
+  __shared__ uint8_t s_ir[kIrLdsBytes];
   const Group<G> g;
   const int grp = threadIdx.x / G;
   int64_t* le = q_lds + (int64_t)grp * nslots;
   int* ci = reinterpret_cast<int*>(q_lds + (int64_t)NG * nslots) + (int64_t)grp * nslots;
+  // the IR of every job, read from LDS for the rest of the kernel
+  const bool ir_lds = launch.ir_base && launch.ir_bytes <= kIrLdsBytes;
+  if (ir_lds)
+    for (int k = threadIdx.x; k < launch.ir_bytes; k += BLOCK) s_ir[k] = launch.ir_base[k];
+  __syncthreads();
+  auto ir_of = [&](const QueryJob& J) { return ir_lds ? s_ir + (J.ir - launch.ir_base) : J.ir; };
   const int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
   const int64_t total = launch.lines_dev ? common * launch.njobs
@@ -392,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
         const int64_t q = it / common, i = it - q * common;
         const QueryJob& J = launch.jobs[q];
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, J.ir, common, i, out, le, ci, nslots);
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
       }
       continue;
     }
@@ -405,7 +420,8 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, J.ir, J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots);
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+                nslots);
     }
   }
 }

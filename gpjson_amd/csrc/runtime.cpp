@@ -260,6 +260,8 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.counter = (unsigned long long*)dbase;
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   L.tiled = tiled ? 1 : 0;
+  L.ir_base = dbase + jobs_off + jobs_bytes;
+  L.ir_bytes = ir_total;
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
   out->block = ctx->qtune.block;

@@ -41,13 +41,14 @@ struct IndexTune {
 };
 
 // Defaults are the shape the autotuner measured fastest on 4 GiB of tweets
-// (line mode, one lane per line: 3.2 ms per query against 8.2 ms tiled).
+// (line mode, one lane per line, 4 lines per claim: 3.1 ms for $.user.lang
+// against 7.7 ms tiled).
 struct QueryTune {
   int tiled = 0;          // stage each index tile's words in LDS (levels <= 4)
   int block = 256;
   int group = 1;          // lanes per line
-  int waves_per_cu = 32;  // persistent grid size factor
-  int lines_per_wave = 1; // lines a group claims per atomic
+  int waves_per_cu = 16;  // persistent grid size factor
+  int lines_per_wave = 4; // lines a group claims per atomic
 };
 
 int nlv_for_levels(int L);

@@ -318,6 +318,8 @@ int launch_batch(gpjson_server* s, Batch& b) {
   L.total_items = items;
   L.counter = (unsigned long long*)b.jobs.p;
   L.lines_per_wave = ctx->qtune.lines_per_wave;
+  L.ir_base = (const uint8_t*)b.jobs.p + jobs_off + jobs_bytes;
+  L.ir_bytes = ir_total;
   GJ_HIP(zero_u64(L.counter, 1, ctx->stream));
   const int waves_per_block = ctx->qtune.block / 64;
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
