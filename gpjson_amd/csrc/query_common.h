@@ -52,7 +52,7 @@ struct QueryLaunch {
   // written on the device and the item count is read from here.
   const int64_t* total_dev;
   unsigned long long* counter;    // zeroed before launch
-  int lines_per_wave;             // lines a wavefront claims per atomic
+  int lines_per_wave;             // lines each lane group takes per wavefront claim
   // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
   // tile prefix); a block stages a tile's leveled and string words in LDS.
   int tiled;

@@ -397,11 +397,18 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
   const int64_t total = launch.lines_dev ? common * launch.njobs
                                          : (launch.total_dev ? *launch.total_dev : launch.total_items);
+  // One atomic per wavefront claims lpw lines for each of its 64/G groups: the
+  // wavefront runs until its slowest lane is done anyway, and the counter sees
+  // 64/G times fewer atomics than with per-group claims.
+  constexpr int kGroupsPerWave = 64 / G;
+  const int lane = __lane_id();
   for (;;) {
-    unsigned long long g0 = 0;
-    if (g.r == 0) g0 = atomicAdd(launch.counter, (unsigned long long)lpw);
-    g0 = (unsigned long long)g.shfl_i64((int64_t)g0, 0);
-    if ((int64_t)g0 >= total) break;
+    unsigned long long w0 = 0;
+    if (lane == 0) w0 = atomicAdd(launch.counter, (unsigned long long)kGroupsPerWave * lpw);
+    w0 = shfl64_lane(w0, 0);
+    if ((int64_t)w0 >= total) break;
+    const unsigned long long g0 = w0 + (unsigned long long)(lane / G) * lpw;
+    if ((int64_t)g0 >= total) continue;
     if (launch.lines_dev) {
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
         const int64_t q = it / common, i = it - q * common;

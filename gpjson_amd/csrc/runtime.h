@@ -41,14 +41,13 @@ struct IndexTune {
 };
 
 // Defaults are the shape the autotuner measured fastest on 4 GiB of tweets
-// (line mode, one lane per line, 4 lines per claim: 3.1 ms for $.user.lang
-// against 7.7 ms tiled).
+// (line mode, one lane per line: 2.9 ms for $.user.lang against 7.2 ms tiled).
 struct QueryTune {
   int tiled = 0;          // stage each index tile's words in LDS (levels <= 4)
   int block = 256;
   int group = 1;          // lanes per line
   int waves_per_cu = 16;  // persistent grid size factor
-  int lines_per_wave = 4; // lines a group claims per atomic
+  int lines_per_wave = 1; // lines each group takes per wavefront claim
 };
 
 int nlv_for_levels(int L);
