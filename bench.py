@@ -492,7 +492,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
-    ap.add_argument("--config", default="", help="index_block,index_wpt,query_tiled,query_block,query_group,wpc,lpg")
+    ap.add_argument("--config", default="", help="index_block,index_wpt,query_mode,query_block,query_group,wpc,lpg")
     ap.add_argument("--files", type=int, default=0, help="batch: 256 files; serve: 64 files")
     ap.add_argument("--batch-total", type=int, default=16 << 30)
     ap.add_argument("--serve-total", type=int, default=100 << 30)

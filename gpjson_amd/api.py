@@ -191,13 +191,14 @@ class Context:
     def get_config(self) -> dict:
         v = [ctypes.c_int() for _ in range(7)]
         _check(self._lib.gpjson_get_config(self._h, *[ctypes.byref(x) for x in v]))
-        keys = ("index_block", "index_wpt", "query_tiled", "query_block", "query_group", "query_waves_per_cu",
+        keys = ("index_block", "index_wpt", "query_mode", "query_block", "query_group", "query_waves_per_cu",
                 "lines_per_group")
         return dict(zip(keys, [x.value for x in v]))
 
-    def set_config(self, index_block: int, index_wpt: int, query_tiled: int, query_block: int, query_group: int,
+    def set_config(self, index_block: int, index_wpt: int, query_mode: int, query_block: int, query_group: int,
                    query_waves_per_cu: int, lines_per_group: int) -> None:
-        _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_tiled, query_block, query_group,
+        """query_mode: 0 line (lane group per line), 1 tiled, 2 member-chain evaluator (else line)."""
+        _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_mode, query_block, query_group,
                                            query_waves_per_cu, lines_per_group))
 
     def autotune(self, buf: "Buffer", irs: Sequence[bytes], iterations: int = 3) -> str:

@@ -26,6 +26,7 @@ SOURCES = [
     "server.cpp",
     "autotune.cpp",
     "diag.hip",
+    "chain_kernel.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -32,10 +32,10 @@ struct QCand {
 
 extern "C" {
 
-int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_tiled, int* query_block,
+int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* query_mode, int* query_block,
                       int* query_group, int* query_waves_per_cu, int* lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
-  if (query_tiled) *query_tiled = ctx->qtune.tiled;
+  if (query_mode) *query_mode = ctx->qtune.mode;
   if (index_block) *index_block = ctx->itune.block;
   if (index_wpt) *index_wpt = ctx->itune.wpt;
   if (query_block) *query_block = ctx->qtune.block;
@@ -45,7 +45,7 @@ int gpjson_get_config(gpjson_ctx* ctx, int* index_block, int* index_wpt, int* qu
   return GPJSON_OK;
 }
 
-int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_tiled, int query_block,
+int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query_mode, int query_block,
                       int query_group, int query_waves_per_cu, int lines_per_wave) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
   if (!index_config_supported(index_block, index_wpt))
@@ -55,11 +55,13 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
     return fail(GPJSON_ERR_INVALID_ARG, "query block must be 64, 128 or 256 and group 1, 4, 16 or 64");
   if (query_waves_per_cu < 1 || query_waves_per_cu > 64 || lines_per_wave < 1 || lines_per_wave > 1024)
     return fail(GPJSON_ERR_INVALID_ARG, "query shape out of range");
+  if (query_mode < kLineMode || query_mode > kChainMode)
+    return fail(GPJSON_ERR_INVALID_ARG, "query mode must be 0 (line), 1 (tiled) or 2 (chain)");
   GJ_HIP(hipSetDevice(ctx->device));
   GJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->itune.block = index_block;
   ctx->itune.wpt = index_wpt;
-  ctx->qtune.tiled = query_tiled ? 1 : 0;
+  ctx->qtune.mode = query_mode;
   ctx->qtune.block = query_block;
   ctx->qtune.group = query_group;
   ctx->qtune.waves_per_cu = query_waves_per_cu;
@@ -121,11 +123,14 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   for (int group : {1, 4, 16, 64})
     for (int wpc : {16, 32})
       for (int lpw : {1, 4}) qc.push_back({0, 256, group, wpc, lpw});
+  // member-chain evaluator (used only when every query is a member chain)
+  for (int block : {64, 256})
+    for (int wpc : {8, 16, 32}) qc.push_back({2, block, 1, wpc, 1});
   int rc = GPJSON_OK;
   double bestq = 1e30;
   QueryTune best_q = keep_q;
   for (const QCand& c : qc) {
-    ctx->qtune.tiled = c.tiled;
+    ctx->qtune.mode = c.tiled;
     ctx->qtune.block = c.block;
     ctx->qtune.group = c.group;
     if (c.wpc) ctx->qtune.waves_per_cu = c.wpc;
@@ -145,7 +150,7 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     gpjson_plan_free(p);
     if (rc) break;
     double ms = sum / iterations;
-    std::snprintf(line, sizeof(line), "query tiled=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
+    std::snprintf(line, sizeof(line), "query mode=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
                   c.tiled, c.block, c.group, c.wpc, c.lpw, ms);
     rep += line;
     if (ms < bestq) {
@@ -160,8 +165,8 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   }
   ctx->qtune = best_q;
   std::snprintf(line, sizeof(line),
-                "best index %dx%d (%.4f ms), query tiled %d block %d group %d wpc %d lpg %d (%.4f ms)\n", best_i.block,
-                best_i.wpt, best, best_q.tiled, best_q.block, best_q.group, best_q.waves_per_cu,
+                "best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms)\n", best_i.block,
+                best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
                 best_q.lines_per_wave, bestq);
   rep += line;
   if (report && report_cap) {

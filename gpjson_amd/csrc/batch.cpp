@@ -37,6 +37,7 @@ struct gpjson_batch {
   std::vector<int> file_first_job;
   int64_t total_tiles = 0;
   int64_t ir_total = 0;  // IR bytes of every job, contiguous in dev_ir
+  bool chain = false;    // every job is a member chain: chain evaluator
   // device memory
   DevBuf arena_s, arena_rc, arena_rq, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
   int64_t* total_dev = nullptr;
@@ -135,6 +136,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
   int64_t ir_total = 0;
   for (int j = 0; j < b->njobs; j++) ir_total += ir_lens[j];
   b->ir_total = ir_total;
+  b->chain = b->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
+  for (int j = 0; j < b->njobs && b->chain; j++) b->chain = chain_depth(irs[j], ir_lens[j]) > 0;
   rc = b->dev_ir.ensure((size_t)ir_total + 16);
   if (rc) return rc;
   std::vector<uint8_t> flat((size_t)ir_total + 16);
@@ -208,7 +211,8 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
-  GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s));
+  if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->query_grid, s));
+  else GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s));
   if (e2) GJ_HIP(hipEventRecord(e2, s));
   return GPJSON_OK;
 }

@@ -393,10 +393,15 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
     for (int k = threadIdx.x; k < launch.ir_bytes; k += BLOCK) s_ir[k] = launch.ir_base[k];
   __syncthreads();
   auto ir_of = [&](const QueryJob& J) { return ir_lds ? s_ir + (J.ir - launch.ir_base) : J.ir; };
-  const int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
   const int64_t total = launch.lines_dev ? common * launch.njobs
                                          : (launch.total_dev ? *launch.total_dev : launch.total_items);
+  // Several lines per group amortize a claim over short lines, but with few
+  // (long) lines they serialize work one group could share: halve the claim
+  // until every group of the grid gets at least one full claim.
+  int lpw = launch.lines_per_wave > 0 ? launch.lines_per_wave : 1;
+  const int64_t groups = (int64_t)gridDim.x * NG;
+  while (lpw > 1 && total < groups * lpw) lpw >>= 1;
   // One atomic per wavefront claims lpw lines for each of its 64/G groups: the
   // wavefront runs until its slowest lane is done anyway, and the counter sees
   // 64/G times fewer atomics than with per-group claims.

@@ -225,7 +225,9 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   int64_t off = (int64_t)(jobs_off + jobs_bytes);
   // Tiled mode needs the line count on the host (span offsets per query); the
   // line-based mode reads it on the device.
-  const bool tiled = ctx->qtune.tiled && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
+  bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
+  for (int q = 0; q < nq && chain; q++) chain = chain_depth(irs[q], ir_lens[q]) > 0;
+  const bool tiled = !chain && ctx->qtune.mode == kTiledMode && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
   for (int q = 0; q < nq; q++) {
     QueryJob j{};
     j.F = idx->buf->d;
@@ -264,6 +266,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.ir_bytes = ir_total;
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
+  out->chain = chain;
   out->block = ctx->qtune.block;
   out->group = ctx->qtune.group;
   out->levels = idx->L;
@@ -276,7 +279,8 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel, hipEvent_t after_kernel) {
   GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream));
+  if (pq.chain) GJ_HIP(chain_launch(pq.launch, pq.block, pq.grid, ctx->stream));
+  else GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
   return GPJSON_OK;
 }

@@ -20,6 +20,10 @@ hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t str
 bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
 hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream);
+// chain_kernel.hip: member-chain programs, (KEY k DOWN)^D STORE END; the
+// launch's IR must fit kIrLdsBytes.
+int chain_depth(const uint8_t* ir, int64_t len);  // D, or 0 if ir is not a member chain
+hipError_t chain_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream);
 bool query_config_supported(int block, int group);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
@@ -40,14 +44,22 @@ struct IndexTune {
   int wpt = 4;
 };
 
-// Defaults are the shape the autotuner measured fastest on 4 GiB of tweets
-// (line mode, one lane per line: 2.9 ms for $.user.lang against 7.2 ms tiled).
+// Interpreter modes: one lane group per line (the general interpreter); tiled
+// (a block stages one index tile's words in LDS, levels <= 4); chain (the
+// member-chain evaluator when every program of a launch is a member chain,
+// else line mode).
+enum QueryMode { kLineMode = 0, kTiledMode = 1, kChainMode = 2 };
+
+// Defaults: the line-mode shape the autotuner measured fastest on 4 GiB of
+// tweets (one lane per line, 4 lines per claim: 2.86 ms for $.user.lang against
+// 3.31 ms with 1 line per claim and 7.2 ms tiled), with member chains on the
+// chain evaluator.
 struct QueryTune {
-  int tiled = 0;          // stage each index tile's words in LDS (levels <= 4)
+  int mode = kChainMode;
   int block = 256;
   int group = 1;          // lanes per line
   int waves_per_cu = 16;  // persistent grid size factor
-  int lines_per_wave = 1; // lines each group takes per wavefront claim
+  int lines_per_wave = 4; // lines each group takes per wavefront claim (at most)
 };
 
 int nlv_for_levels(int L);
@@ -112,6 +124,7 @@ namespace gj {
 // Device job table + launch descriptor for nq queries over one index.
 struct PreparedQueries {
   QueryLaunch launch{};
+  bool chain = false;  // run on the member-chain evaluator
   int grid = 0;
   int block = 256;
   int group = 4;

@@ -227,15 +227,17 @@ void gpjson_server_destroy(gpjson_server *server);
 /* ---- launch shapes and the autotuner -------------------------------------- */
 
 /* Current launch shapes of the context: index tile = index_block threads x
- * index_wpt 64-byte words; interpreter mode (query_tiled: a block stages one
- * index tile's leveled and string words in LDS, used for up to 4 levels),
- * block size, lanes per line (query_group: 1, 4, 16 or 64), persistent grid in
- * wavefronts per CU and lines a lane group claims at a time (line mode).  The
- * reference hard-codes 8x1024 (stages/CombinedIndex.java:11-14) and 512x1024
- * threads with one thread per line (functions/QueryFunction.java:229). */
-int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *query_tiled, int *query_block,
+ * index_wpt 64-byte words; interpreter mode (query_mode: 0 = one lane group
+ * per line; 1 = tiled, a block stages one index tile's leveled and string words
+ * in LDS, up to 4 levels; 2 = member-chain evaluator when every query of a
+ * launch is a member chain $.a.b..., else mode 0), block size, lanes per line
+ * (query_group: 1, 4, 16 or 64; modes 0 and 1), persistent grid in wavefronts
+ * per CU and lines a lane group claims at a time (mode 0).  The reference
+ * hard-codes 8x1024 (stages/CombinedIndex.java:11-14) and 512x1024 threads with
+ * one thread per line (functions/QueryFunction.java:229). */
+int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *query_mode, int *query_block,
                       int *query_group, int *query_waves_per_cu, int *lines_per_wave);
-int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query_tiled, int query_block,
+int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query_mode, int query_block,
                       int query_group, int query_waves_per_cu, int lines_per_wave);
 /* Times every supported shape on buf with the given queries and installs the
  * fastest; report (optional) gets one text line per candidate. */

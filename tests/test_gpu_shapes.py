@@ -1,6 +1,6 @@
 """Every launch shape the autotuner may pick gives the oracle's bit-exact answers:
 all index tile shapes, and the interpreter in tiled and line mode with every
-lane-group size."""
+lane-group size, and the member-chain evaluator."""
 import numpy as np
 import pytest
 
@@ -9,8 +9,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 INDEX_SHAPES = [(256, 1), (256, 2), (256, 4), (512, 1), (512, 2), (128, 2), (128, 4)]
+# (mode, block, group); mode 2 with these mixed query sets runs in line mode
 QUERY_SHAPES = [(1, 256, 1), (1, 256, 4), (1, 128, 16), (1, 256, 64), (0, 256, 1), (0, 256, 4), (0, 64, 16),
-                (0, 256, 64)]
+                (0, 256, 64), (2, 256, 1)]
 
 
 @pytest.fixture(scope="module")
@@ -71,6 +72,43 @@ def test_query_shapes(ctx, inputs, tiled, block, group):
             for q, c in enumerate(cps):
                 ref = O.find_value(data, nl, s, lb, L, c.ir, R).reshape(len(nl), R, 2)
                 assert np.array_equal(got[q], ref), (name, PATHS[name][q])
+            idx.free()
+            buf.free()
+    finally:
+        ctx.set_config(*cfg.values())
+
+
+CHAIN_PATHS = {
+    "tweets": ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.profile.colors.background",
+               "$.nokey", "$.user.nokey"],
+    "adversarial": ["$.id", "$.s1", "$.chain.c1.c2", "$.chain.c1.c2.c3.c4", "$.deep.d.d", "$.s2",
+                    "$.chain.c1.c2.c3.c4.c5.c6.c7.c8.c9.c10.c11.c12.c13.c14.c15.c16.c17.c18.c19"],
+    "soup": ["$.a", "$.a.b", "$.x.y.z"],
+}
+
+
+@pytest.mark.parametrize("block,wpc", [(64, 8), (256, 16), (256, 32)])
+def test_chain_mode(ctx, inputs, block, wpc):
+    """The member-chain evaluator (every query a member chain) equals the
+    oracle, one query per launch and all queries in one launch."""
+    from gpjson_amd import compile_path
+    cfg = ctx.get_config()
+    try:
+        ctx.set_config(cfg["index_block"], cfg["index_wpt"], 2, block, 1, wpc, 1)
+        for name, data in inputs.items():
+            cps = [compile_path(p) for p in CHAIN_PATHS[name]]
+            L = max(c.max_depth for c in cps)
+            print(f"[shapes] chain {block}/{wpc} {name}", flush=True)
+            buf = ctx.stage(data)
+            idx = ctx.index(buf, L)
+            together = ctx.run_queries(idx, [c.ir for c in cps], 1)
+            nl, s = O.newline_index(data), O.string_index(data)
+            lb = O.leveled_index(data, s, L)
+            for q, c in enumerate(cps):
+                ref = O.find_value(data, nl, s, lb, L, c.ir, 1).reshape(len(nl), 1, 2)
+                alone = ctx.run_queries(idx, [c.ir], 1)[0]
+                assert np.array_equal(alone, ref), (name, c.path)
+                assert np.array_equal(together[q], ref), (name, c.path)
             idx.free()
             buf.free()
     finally:
