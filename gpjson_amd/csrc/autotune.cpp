@@ -124,8 +124,8 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     for (int wpc : {16, 32})
       for (int lpw : {1, 4}) qc.push_back({0, 256, group, wpc, lpw});
   // member-chain evaluator (used only when every query is a member chain)
-  for (int block : {64, 256})
-    for (int wpc : {8, 16, 32}) qc.push_back({2, block, 1, wpc, 1});
+  for (int group : {1, 4, 16})
+    for (int wpc : {8, 16, 32}) qc.push_back({2, 256, group, wpc, 1});
   int rc = GPJSON_OK;
   double bestq = 1e30;
   QueryTune best_q = keep_q;

@@ -211,7 +211,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
-  if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->query_grid, s));
+  if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->qtune.group, b->query_grid, s));
   else GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s));
   if (e2) GJ_HIP(hipEventRecord(e2, s));
   return GPJSON_OK;

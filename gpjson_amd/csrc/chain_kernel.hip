@@ -1,17 +1,17 @@
 // chain_kernel.hip -- member-chain evaluator for gfx950.
 //
-// The general interpreter (query_kernel.hip) runs one line per lane through a
-// switch over opcodes, so the 64 lanes of a wavefront sit at different opcodes
-// and scan positions, the wavefront executes every branch in turn, and each
-// branch waits for its own memory round trip.  For member chains -- the
-// programs (KEY k DOWN)^D STORE END of $.k0.k1...k(D-1) -- this kernel runs
-// the per-lane state machine of chain_step.h instead: every iteration each
-// lane issues one batch of independent loads chosen by its phase (the newline
-// pair, kChainWords words of its level with the colon, string and raw-quote
-// words, or 8 input bytes), the wavefront waits once, and every lane advances.
-// A lane that finishes a line takes a new one at the next iteration (one
-// atomic per wavefront for all idle lanes), so lanes never wait for each
-// other's lines.
+// The general interpreter (query_kernel.hip) runs lines through a switch over
+// opcodes; lanes at different opcodes and scan positions serialize, and every
+// lane's scattered 8-byte loads pull whole cache lines that are evicted before
+// the rest of them is used.  For member chains -- the programs
+// (KEY k DOWN)^D STORE END of $.k0.k1...k(D-1) -- this kernel runs the state
+// machine of chain_step.h with a group of G lanes per line: every iteration
+// each lane loads kChainWords consecutive words of the bitmaps its group's
+// phase needs (a group step reads whole cache lines of each bitmap), the
+// group combines the lanes' partial results with a ballot, and every lane
+// applies the same update.  A group that finishes a line takes a new one at
+// the next iteration (one atomic per wavefront for all idle groups), so
+// groups never wait for each other's lines.
 #include <hip/hip_runtime.h>
 
 #include "chain_step.h"
@@ -32,7 +32,7 @@ __device__ __forceinline__ unsigned long long shfl64(unsigned long long v, int s
 
 }  // namespace
 
-template <int BLOCK>
+template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void chain_kernel(QueryLaunch launch) {
   __shared__ uint8_t s_ir[kIrLdsBytes];
   __shared__ int64_t s_item0[kItemLdsJobs];
@@ -43,7 +43,9 @@ __global__ __launch_bounds__(BLOCK) void chain_kernel(QueryLaunch launch) {
   __syncthreads();
 
   const int lane = __lane_id();
-  const unsigned long long lanes_below = (1ull << lane) - 1ull;
+  const int r = lane % G;         // lane within the group
+  const int gbase = lane - r;     // first lane of the group
+  const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase);
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
   const int64_t total = launch.lines_dev ? common * launch.njobs
                                          : (launch.total_dev ? *launch.total_dev : launch.total_items);
@@ -55,15 +57,15 @@ __global__ __launch_bounds__(BLOCK) void chain_kernel(QueryLaunch launch) {
   int64_t item0 = 0;
 
   for (;;) {
-    // ---- claim lines for idle lanes: one atomic per wavefront ----
-    const unsigned long long idle = __ballot(s.phase == kNeedLine);
+    // ---- claim lines for idle groups: one atomic per wavefront ----
+    const unsigned long long idle = __ballot(s.phase == kNeedLine && r == 0);
     if (idle) {
       const int leader = __builtin_ctzll(idle);
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(launch.counter, (unsigned long long)__popcll(idle));
       base = shfl64(base, leader);
       if (s.phase == kNeedLine) {
-        const int64_t item = (int64_t)(base + __popcll(idle & lanes_below));
+        const int64_t item = (int64_t)(base + __popcll(idle & ((1ull << gbase) - 1ull)));
         if (item >= total) {
           s.phase = kDone;
         } else {
@@ -105,10 +107,24 @@ __global__ __launch_bounds__(BLOCK) void chain_kernel(QueryLaunch launch) {
       }
     }
     if (__ballot(s.phase != kDone) == 0) break;
-    if (s.phase == kDone) continue;
+
+    // ---- one step: loads, this lane's part, group combine, update ----
     ChainLoads ld;
-    chain_load(s, J, ld);
-    chain_advance(s, J, ld, s_ir);
+    ChainPart part{-1, 0, 0, true};
+    if (s.phase != kDone) {
+      chain_load(s, J, r, ld);
+      part = chain_lane(s, J, r, ld, s_ir);
+    }
+    const unsigned long long has = __ballot(part.pos >= 0) & gmask;
+    const unsigned long long differ = __ballot(!part.same) & gmask;
+    ChainPart p{-1, 0, 0, differ == 0};
+    if (has) {
+      const int src = __builtin_ctzll(has);
+      p.pos = (int64_t)shfl64((unsigned long long)part.pos, src);
+      p.ss = (int64_t)shfl64((unsigned long long)part.ss, src);
+      p.kind = __shfl(part.kind, src);
+    }
+    if (s.phase != kDone) chain_update(s, J, ld, p, G, s_ir, r == 0);
   }
 }
 
@@ -136,14 +152,26 @@ int chain_depth(const uint8_t* ir, int64_t len) {
   return D;
 }
 
-hipError_t chain_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream) {
-  switch (block) {
-    case 64: hipLaunchKernelGGL(chain_kernel<64>, dim3(grid), dim3(64), 0, stream, L); break;
-    case 128: hipLaunchKernelGGL(chain_kernel<128>, dim3(grid), dim3(128), 0, stream, L); break;
-    case 256: hipLaunchKernelGGL(chain_kernel<256>, dim3(grid), dim3(256), 0, stream, L); break;
+template <int BLOCK>
+static hipError_t chain_launch_b(const QueryLaunch& L, int group, int grid, hipStream_t stream) {
+  switch (group) {
+    case 1: hipLaunchKernelGGL((chain_kernel<BLOCK, 1>), dim3(grid), dim3(BLOCK), 0, stream, L); break;
+    case 4: hipLaunchKernelGGL((chain_kernel<BLOCK, 4>), dim3(grid), dim3(BLOCK), 0, stream, L); break;
+    case 16: hipLaunchKernelGGL((chain_kernel<BLOCK, 16>), dim3(grid), dim3(BLOCK), 0, stream, L); break;
+    case 64: hipLaunchKernelGGL((chain_kernel<BLOCK, 64>), dim3(grid), dim3(BLOCK), 0, stream, L); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// group: lanes per line (1, 4, 16 or 64).
+hipError_t chain_launch(const QueryLaunch& L, int block, int group, int grid, hipStream_t stream) {
+  switch (block) {
+    case 64: return chain_launch_b<64>(L, group, grid, stream);
+    case 128: return chain_launch_b<128>(L, group, grid, stream);
+    case 256: return chain_launch_b<256>(L, group, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace gj

@@ -1,6 +1,6 @@
-// chain_step.h -- one lane's state machine of the member-chain evaluator.
-// Header-only, compiled by hipcc into chain_kernel.hip and by g++ into the CPU
-// harness that checks it line by line against the oracle.
+// chain_step.h -- the member-chain evaluator's state machine, one step at a
+// time.  Header-only: compiled by hipcc into chain_kernel.hip and by g++ into
+// the CPU harness that checks it line by line against the oracle.
 //
 // A member chain $.k0.k1...k(D-1) compiles to (KEY k DOWN)^D STORE END
 // (J/jsonpath/IRBuilder.java:19-81).  For such a program the interpreter of
@@ -21,9 +21,12 @@
 // ss = e - len, ss >= start, F[ss] == '"' (raw-quote bitmap), then the len
 // bytes after ss equal the key.
 //
-// Every step a lane issues one batch of independent loads chosen by its phase
-// (chain_load) and then advances (chain_advance); the kernel interleaves the
-// steps of all lanes so a wavefront never waits on one lane's branch.
+// A group of G lanes evaluates one line.  Each step every lane of the group
+// loads kChainWords consecutive words of each bitmap it needs (lane r the words
+// wbase + r*kChainWords ...), so a group step reads whole cache lines; each
+// lane then computes its partial result (chain_lane), the group combines them
+// (first lane with a result, or AND for a byte compare), and every lane applies
+// the same update (chain_update).  The state is uniform within the group.
 #pragma once
 #include <stdint.h>
 
@@ -32,7 +35,7 @@
 
 namespace gj {
 
-constexpr int kChainWords = 4;  // level words a lane loads per step
+constexpr int kChainWords = 4;  // bitmap words a lane loads per step
 
 enum ChainPhase : int {
   kNeedLine = 0,  // idle: claim a line
@@ -44,7 +47,7 @@ enum ChainPhase : int {
   kDone = 6,      // every line claimed
 };
 
-// The arrays of the job a lane is working on.
+// The arrays of the job a group is working on.
 struct ChainJobView {
   const uint8_t* F;
   const int64_t* NL;
@@ -66,10 +69,18 @@ struct ChainLane {
 };
 
 struct ChainLoads {
-  int64_t nl0, nl1, wbase;
+  int64_t nl0, nl1, w0;  // w0: first word this lane loaded
   uint64_t lw[kChainWords], cw[kChainWords];
-  uint64_t sw[kChainWords + 1], qw[kChainWords + 1];  // [0]: the word before wbase
+  uint64_t sw[kChainWords + 1], qw[kChainWords + 1];  // [0]: the word before w0
   uint8_t fb[8];
+};
+
+// A lane's contribution to the group's step.
+struct ChainPart {
+  int64_t pos;  // kKey: candidate ':'; kDelim: first bit; kStore: first non-blank; -1: none
+  int64_t ss;   // kKey: the candidate's quote position
+  int kind;     // kKey: 1 match (empty key), 2 needs a byte compare
+  bool same;    // kVerify: this lane's key bytes are equal
 };
 
 GJ_HD uint64_t chain_mask_from(int lo_bit) { return ~0ull << lo_bit; }
@@ -96,9 +107,12 @@ GJ_HD int chain_varint(const uint8_t* ir, int* pc) {
   return v;
 }
 
-// Issues the loads of the lane's current phase.
-GJ_HD void chain_load(const ChainLane& s, const ChainJobView& J, ChainLoads& ld) {
-  ld.nl0 = ld.nl1 = ld.wbase = 0;
+// First word of the group's window this step.
+GJ_HD int64_t chain_wbase(const ChainLane& s) { return (s.phase == kKey ? s.j : s.dpos) >> 6; }
+
+// Lane r's loads for the group's current phase.
+GJ_HD void chain_load(const ChainLane& s, const ChainJobView& J, int r, ChainLoads& ld) {
+  ld.nl0 = ld.nl1 = ld.w0 = 0;
 #pragma unroll
   for (int u = 0; u < kChainWords; u++) ld.lw[u] = ld.cw[u] = 0;
 #pragma unroll
@@ -109,14 +123,13 @@ GJ_HD void chain_load(const ChainLane& s, const ChainJobView& J, ChainLoads& ld)
     ld.nl0 = J.NL[s.i];
     ld.nl1 = s.i + 1 < J.lines ? J.NL[s.i + 1] : J.N;
   } else if (s.phase == kKey || s.phase == kDelim) {
-    const int64_t from = s.phase == kKey ? s.j : s.dpos;
-    const int64_t wbase = from >> 6;
-    ld.wbase = wbase;
+    const int64_t w0 = chain_wbase(s) + (int64_t)r * kChainWords;
+    ld.w0 = w0;
     const uint64_t* lv = J.LB + (int64_t)s.d * J.Wp;
     const int64_t wlast = ((s.hi < J.N ? s.hi : J.N) - 1) >> 6;  // last word of the window
 #pragma unroll
     for (int u = 0; u < kChainWords; u++) {
-      const int64_t w = wbase + u;
+      const int64_t w = w0 + u;
       if (w <= wlast) {
         ld.lw[u] = lv[w];
         if (s.phase == kKey) {
@@ -126,58 +139,40 @@ GJ_HD void chain_load(const ChainLane& s, const ChainJobView& J, ChainLoads& ld)
         }
       }
     }
-    if (s.phase == kKey && wbase > (s.start >> 6)) {
-      ld.sw[0] = J.S[wbase - 1];
-      ld.qw[0] = J.RQ[wbase - 1];
+    if (s.phase == kKey && w0 > (s.start >> 6) && w0 - 1 <= wlast) {
+      ld.sw[0] = J.S[w0 - 1];
+      ld.qw[0] = J.RQ[w0 - 1];
     }
   } else if (s.phase == kVerify) {
+    const int b0 = s.vdone + 8 * r;
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (s.vdone + u < s.klen) ld.fb[u] = J.F[s.ss + 1 + s.vdone + u];
+      if (b0 + u < s.klen) ld.fb[u] = J.F[s.ss + 1 + b0 + u];
   } else if (s.phase == kStore) {
+    const int64_t b0 = s.j + 8 * (int64_t)r;
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (s.j + u < J.N) ld.fb[u] = J.F[s.j + u];
+      if (b0 + u < J.N) ld.fb[u] = J.F[b0 + u];
   }
 }
 
-// Sets up the KEY op at IR offset kpc (ir[kpc] == KEY) or the final STORE.
-GJ_HD void chain_next_op(ChainLane& s, const uint8_t* ir, int kpc) {
-  if (ir[kpc] == kStoreResult) {
-    s.klen = -1;
-    return;
-  }
-  int pc = kpc + 1;
-  s.klen = chain_varint(ir, &pc);
-  s.koff = pc;
-}
-
-// Advances the lane's state with the loads of this step.  Writes the result
-// (and the -1 fill of a new line) to J.out_base.
-GJ_HD void chain_advance(ChainLane& s, const ChainJobView& J, const ChainLoads& ld, const uint8_t* ir) {
+// Lane r's partial result of the step.
+GJ_HD ChainPart chain_lane(const ChainLane& s, const ChainJobView& J, int r, const ChainLoads& ld,
+                           const uint8_t* ir) {
+  ChainPart out{-1, 0, 0, true};
   const int64_t N = J.N;
-  if (s.phase == kInit) {
-    s.start = ld.nl0;
-    int64_t* out = J.out_base + s.i * (int64_t)J.R * 2;
-    for (int k = 0; k < J.R * 2; k++) out[k] = -1;
-    s.j = s.start;
-    s.hi = ld.nl1;
-    s.d = 0;
-    chain_next_op(s, ir, J.ir0);
-    s.phase = (s.start < s.hi && s.start < N && s.d < J.L && s.klen >= 0) ? kKey : kNeedLine;
-  } else if (s.phase == kKey) {
+  if (s.phase == kKey) {
     const int64_t to = s.hi < N ? s.hi : N;
     const int64_t wlast = (to - 1) >> 6;
     const int64_t wstart = s.start >> 6;
-    int64_t found = -1, verify = -1;
 #pragma unroll
     for (int u = 0; u < kChainWords; u++) {
-      const int64_t w = ld.wbase + u;
+      const int64_t w = ld.w0 + u;
       uint64_t v = ld.lw[u] & ld.cw[u];
       if (w == (s.j >> 6)) v &= chain_mask_from((int)(s.j & 63));
       if (w == wlast) v &= chain_mask_upto((int)((to - 1) & 63));
       if (w > wlast) v = 0;
-      while (v && found < 0 && verify < 0) {
+      while (v && out.pos < 0) {
         const int64_t p = (w << 6) + ctz64(v);
         v &= v - 1;
         const int64_t k = p - 1;
@@ -185,7 +180,7 @@ GJ_HD void chain_advance(ChainLane& s, const ChainJobView& J, const ChainLoads& 
         // e = the last string bit in (start, k]: the loaded words first
         int64_t e = -1;
         int64_t wi = k >> 6;
-        int idx = (int)(wi - ld.wbase) + 1;  // sw[] slot of word wi
+        int idx = (int)(wi - ld.w0) + 1;  // sw[] slot of word wi
         bool slow = idx < 0 || idx > kChainWords;
         uint64_t sv = slow ? 0 : chain_pick(ld.sw, idx);
         sv &= chain_mask_upto((int)(k & 63));
@@ -223,39 +218,94 @@ GJ_HD void chain_advance(ChainLane& s, const ChainJobView& J, const ChainLoads& 
         const int64_t sc = e - s.klen;
         if (sc < s.start) continue;
         const int64_t qi = sc >> 6;
-        const int qidx = (int)(qi - ld.wbase) + 1;
+        const int qidx = (int)(qi - ld.w0) + 1;
         const uint64_t q = (qidx >= 0 && qidx <= kChainWords) ? chain_pick(ld.qw, qidx) : J.RQ[qi];
         if (!((q >> (sc & 63)) & 1ull)) continue;
-        if (s.klen == 0) {
-          found = p;
-        } else {
-          verify = p;
-          s.ss = sc;
-        }
+        out.pos = p;
+        out.ss = sc;
+        out.kind = s.klen == 0 ? 1 : 2;
       }
     }
-    if (found >= 0) {
-      s.j = found + 1;
+  } else if (s.phase == kVerify) {
+    const int b0 = s.vdone + 8 * r;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const bool in = b0 + u < s.klen;
+      const uint8_t kb = ir[in ? s.koff + b0 + u : s.koff];
+      out.same = out.same && (!in || ld.fb[u] == kb);
+    }
+  } else if (s.phase == kDelim) {
+    // first level-d bit in [dpos, hi); hi > j >= 0 here
+    const int64_t whi = (s.hi - 1) >> 6;
+#pragma unroll
+    for (int u = kChainWords - 1; u >= 0; u--) {
+      const int64_t w = ld.w0 + u;
+      uint64_t v = ld.lw[u];
+      if (w == (s.dpos >> 6)) v &= chain_mask_from((int)(s.dpos & 63));
+      if (w < (s.dpos >> 6)) v = 0;
+      if (w == whi) v &= chain_mask_upto((int)((s.hi - 1) & 63));
+      if (w > whi) v = 0;
+      if (v) out.pos = (w << 6) + ctz64(v);
+    }
+  } else if (s.phase == kStore) {
+    const int64_t b0 = s.j + 8 * (int64_t)r;
+#pragma unroll
+    for (int u = 7; u >= 0; u--)
+      if (b0 + u < N && !chain_blank(ld.fb[u])) out.pos = b0 + u;
+  }
+  return out;
+}
+
+// Sets up the KEY op at IR offset kpc (ir[kpc] == KEY) or the final STORE.
+GJ_HD void chain_next_op(ChainLane& s, const uint8_t* ir, int kpc) {
+  if (ir[kpc] == kStoreResult) {
+    s.klen = -1;
+    return;
+  }
+  int pc = kpc + 1;
+  s.klen = chain_varint(ir, &pc);
+  s.koff = pc;
+}
+
+// The group's update (identical in every lane).  `p` is the combined result:
+// the part of the first lane (in lane order) with pos >= 0 (all -1 if none),
+// and for kVerify the AND of every lane's `same`.  G: lanes in the group.
+// ld: this lane's loads (kInit uses the newline pair, loaded alike by all
+// lanes).  Writes the -1 fill of a new line and the result to J.out_base;
+// write_out selects the one lane that writes.
+GJ_HD void chain_update(ChainLane& s, const ChainJobView& J, const ChainLoads& ld, const ChainPart& p, int G,
+                        const uint8_t* ir, bool write_out) {
+  const int64_t N = J.N;
+  const int64_t span = (int64_t)G * kChainWords;  // words the group loaded
+  if (s.phase == kInit) {
+    s.start = ld.nl0;
+    if (write_out) {
+      int64_t* out = J.out_base + s.i * (int64_t)J.R * 2;
+      for (int k = 0; k < J.R * 2; k++) out[k] = -1;
+    }
+    s.j = s.start;
+    s.hi = ld.nl1;
+    s.d = 0;
+    chain_next_op(s, ir, J.ir0);
+    s.phase = (s.start < s.hi && s.start < N && s.d < J.L && s.klen >= 0) ? kKey : kNeedLine;
+  } else if (s.phase == kKey) {
+    const int64_t to = s.hi < N ? s.hi : N;
+    if (p.pos >= 0 && p.kind == 1) {
+      s.j = p.pos + 1;
       s.dpos = s.j + 1;
       s.phase = (s.j < s.hi && s.j < N) ? kDelim : kNeedLine;
-    } else if (verify >= 0) {
-      s.cand = verify;
+    } else if (p.pos >= 0) {
+      s.cand = p.pos;
+      s.ss = p.ss;
       s.vdone = 0;
       s.phase = kVerify;
     } else {
-      s.j = (ld.wbase + kChainWords) << 6;
+      s.j = (chain_wbase(s) + span) << 6;
       if (s.j >= to) s.phase = kNeedLine;  // no matching key in the window
     }
   } else if (s.phase == kVerify) {
-    bool same = true;
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const bool in = s.vdone + u < s.klen;
-      const uint8_t kb = ir[in ? s.koff + s.vdone + u : s.koff];
-      same = same && (!in || ld.fb[u] == kb);
-    }
-    s.vdone += 8;
-    if (!same) {
+    s.vdone += 8 * G;
+    if (!p.same) {
       s.j = s.cand + 1;  // keep scanning after the candidate
       s.phase = kKey;
     } else if (s.vdone >= s.klen) {
@@ -264,42 +314,29 @@ GJ_HD void chain_advance(ChainLane& s, const ChainJobView& J, const ChainLoads& 
       s.phase = (s.j < s.hi && s.j < N) ? kDelim : kNeedLine;
     }
   } else if (s.phase == kDelim) {
-    // first level-d bit in (j, hi), scanned from dpos; hi > j >= 0 here
-    int64_t e = -1;
-    const int64_t from = s.dpos;
-    const int64_t whi = (s.hi - 1) >> 6;
-#pragma unroll
-    for (int u = kChainWords - 1; u >= 0; u--) {
-      const int64_t w = ld.wbase + u;
-      uint64_t v = ld.lw[u];
-      if (w == (from >> 6)) v &= chain_mask_from((int)(from & 63));
-      if (w == whi) v &= chain_mask_upto((int)((s.hi - 1) & 63));
-      if (w > whi) v = 0;
-      if (v) e = (w << 6) + ctz64(v);
-    }
     const int64_t lim = s.hi < N ? s.hi : N;  // words at or beyond lim were not loaded
-    const bool resolved = e >= 0 || from >= s.hi || ((ld.wbase + kChainWords) << 6) >= lim;
+    const int64_t next = (chain_wbase(s) + span) << 6;
+    const bool resolved = p.pos >= 0 || s.dpos >= s.hi || next >= lim;
     if (!resolved) {
-      s.dpos = (ld.wbase + kChainWords) << 6;
+      s.dpos = next;
     } else {
-      s.hi = e;  // -1: no delimiter at this level
+      s.hi = p.pos;  // -1: no delimiter at this level
       s.d++;
       chain_next_op(s, ir, s.koff + s.klen + 1);  // skip the DOWN byte
       if (s.klen < 0) s.phase = kStore;
       else s.phase = (s.d < J.L && s.hi > s.j && s.j < N) ? kKey : kNeedLine;
     }
   } else if (s.phase == kStore) {
-    int adv = 8;
-#pragma unroll
-    for (int u = 7; u >= 0; u--)
-      if (s.j + u < N && !chain_blank(ld.fb[u])) adv = u;
-    if (adv < 8 || s.j + 8 >= N) {
-      int64_t* out = J.out_base + s.i * (int64_t)J.R * 2;
-      out[0] = adv < 8 ? s.j + adv : N;
-      out[1] = s.hi;
+    const int64_t next = s.j + 8 * (int64_t)G;
+    if (p.pos >= 0 || next >= N) {
+      if (write_out) {
+        int64_t* out = J.out_base + s.i * (int64_t)J.R * 2;
+        out[0] = p.pos >= 0 ? p.pos : N;
+        out[1] = s.hi;
+      }
       s.phase = kNeedLine;
     } else {
-      s.j += 8;
+      s.j = next;
     }
   }
 }

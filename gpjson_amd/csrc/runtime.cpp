@@ -279,7 +279,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel, hipEvent_t after_kernel) {
   GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  if (pq.chain) GJ_HIP(chain_launch(pq.launch, pq.block, pq.grid, ctx->stream));
+  if (pq.chain) GJ_HIP(chain_launch(pq.launch, pq.block, pq.group, pq.grid, ctx->stream));
   else GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
   return GPJSON_OK;

@@ -23,7 +23,7 @@ hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, 
 // chain_kernel.hip: member-chain programs, (KEY k DOWN)^D STORE END; the
 // launch's IR must fit kIrLdsBytes.
 int chain_depth(const uint8_t* ir, int64_t len);  // D, or 0 if ir is not a member chain
-hipError_t chain_launch(const QueryLaunch& L, int block, int grid, hipStream_t stream);
+hipError_t chain_launch(const QueryLaunch& L, int block, int group, int grid, hipStream_t stream);
 bool query_config_supported(int block, int group);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,

@@ -326,7 +326,7 @@ int launch_batch(gpjson_server* s, Batch& b) {
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
-  if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, grid, ctx->stream));
+  if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, ctx->stream));
   else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream));
   GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, ctx->stream));
   GJ_HIP(hipEventRecord(b.done, ctx->stream));

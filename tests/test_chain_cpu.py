@@ -1,7 +1,7 @@
 """The member-chain evaluator's state machine (gpjson_amd/csrc/chain_step.h),
-built for the CPU with g++, against the oracle's interpreter (the reference's
-find_value semantics) on tweets, adversarial lines, fixtures and random
-structural soups."""
+built for the CPU with g++ and run with groups of 1, 4 and 16 lanes, against
+the oracle's interpreter (the reference's find_value semantics) on tweets,
+adversarial lines, fixtures and random structural soups."""
 import ctypes
 import os
 import subprocess
@@ -31,7 +31,7 @@ def lib(tmp_path_factory):
     L.ch_run.restype = ctypes.c_int64
     L.ch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
+                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
     return L
 
 
@@ -62,6 +62,9 @@ def is_chain(ir: bytes) -> bool:
     return d > 0 and ir[pc:] == b"\x01\x00"
 
 
+GROUPS = (1, 4, 16)
+
+
 def check(lib, data: bytes, paths, levels=None):
     from gpjson_amd import compile_path
     f = np.frombuffer(data, dtype=np.uint8)
@@ -79,14 +82,15 @@ def check(lib, data: bytes, paths, levels=None):
         assert is_chain(cp.ir), cp.path
         ir = np.frombuffer(cp.ir, dtype=np.uint8)
         want = O.find_value(data, nl, s, lb, L, cp.ir, 1)
-        got = np.full(len(nl) * 2, -7, dtype=np.int64)
-        steps = lib.ch_run(fbuf.ctypes.data, len(f), nl.ctypes.data, len(nl), s1.ctypes.data, lb1.ctypes.data,
-                           max(W, 1), L, rc.ctypes.data, rq.ctypes.data, ir.ctypes.data, 1, got.ctypes.data,
-                           1 << 22)
-        assert steps >= 0, cp.path
-        if not np.array_equal(got, want):
-            bad = np.flatnonzero(got != want)[:4]
-            raise AssertionError(f"{cp.path}: differs at {bad.tolist()}: got {got[bad]} want {want[bad]}")
+        for G in GROUPS:
+            got = np.full(len(nl) * 2, -7, dtype=np.int64)
+            steps = lib.ch_run(fbuf.ctypes.data, len(f), nl.ctypes.data, len(nl), s1.ctypes.data, lb1.ctypes.data,
+                               max(W, 1), L, rc.ctypes.data, rq.ctypes.data, ir.ctypes.data, 1, got.ctypes.data,
+                               G, 1 << 22)
+            assert steps >= 0, cp.path
+            if not np.array_equal(got, want):
+                bad = np.flatnonzero(got != want)[:4]
+                raise AssertionError(f"{cp.path} G={G}: differs at {bad.tolist()}: got {got[bad]} want {want[bad]}")
 
 
 def test_tweets(lib):

@@ -87,18 +87,18 @@ CHAIN_PATHS = {
 }
 
 
-@pytest.mark.parametrize("block,wpc", [(64, 8), (256, 16), (256, 32)])
-def test_chain_mode(ctx, inputs, block, wpc):
+@pytest.mark.parametrize("block,group,wpc", [(64, 1, 8), (256, 1, 16), (256, 4, 16), (256, 16, 8), (128, 64, 16)])
+def test_chain_mode(ctx, inputs, block, group, wpc):
     """The member-chain evaluator (every query a member chain) equals the
     oracle, one query per launch and all queries in one launch."""
     from gpjson_amd import compile_path
     cfg = ctx.get_config()
     try:
-        ctx.set_config(cfg["index_block"], cfg["index_wpt"], 2, block, 1, wpc, 1)
+        ctx.set_config(cfg["index_block"], cfg["index_wpt"], 2, block, group, wpc, 1)
         for name, data in inputs.items():
             cps = [compile_path(p) for p in CHAIN_PATHS[name]]
             L = max(c.max_depth for c in cps)
-            print(f"[shapes] chain {block}/{wpc} {name}", flush=True)
+            print(f"[shapes] chain {block}/{group}/{wpc} {name}", flush=True)
             buf = ctx.stage(data)
             idx = ctx.index(buf, L)
             together = ctx.run_queries(idx, [c.ir for c in cps], 1)
