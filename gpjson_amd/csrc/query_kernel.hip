@@ -109,22 +109,31 @@ __device__ __forceinline__ uint64_t window_word(const Words& W, int lvl, int64_t
 }
 
 // Words a lane loads per step in the scans below: independent loads issued
-// together, so one memory round trip covers kUnroll words of a line.
-constexpr int kUnroll = 4;
+// together, so one memory round trip covers that many words of a line (key
+// search: level, colon, string and raw-quote words; level-bit search: level
+// words only).
+#ifndef GJ_KEY_WORDS
+#define GJ_KEY_WORDS 4
+#endif
+#ifndef GJ_BIT_WORDS
+#define GJ_BIT_WORDS 4
+#endif
+constexpr int kKeyWords = GJ_KEY_WORDS;
+constexpr int kBitWords = GJ_BIT_WORDS;
 
 // First set bit of level lvl in [from, to), or -1.
 template <int G, class Words>
 __device__ int64_t group_first_bit(const Group<G>& g, const Words& W, int lvl, int64_t from, int64_t to) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kUnroll) {
-    const int64_t w0 = b0 + (int64_t)g.r * kUnroll;
-    uint64_t v[kUnroll];
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kBitWords) {
+    const int64_t w0 = b0 + (int64_t)g.r * kBitWords;
+    uint64_t v[kBitWords];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) v[u] = window_word(W, lvl, w0 + u, wbeg, wend, from, to);
+    for (int u = 0; u < kBitWords; u++) v[u] = window_word(W, lvl, w0 + u, wbeg, wend, from, to);
     int64_t found = -1;
 #pragma unroll
-    for (int u = kUnroll - 1; u >= 0; u--)
+    for (int u = kBitWords - 1; u >= 0; u--)
       if (v[u]) found = ((w0 + u) << 6) + ctz64(v[u]);
     const unsigned long long m = g.ballot(found >= 0);
     if (m) return g.shfl_i64(found, ctz64(m));
@@ -181,11 +190,11 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
                                   int klen) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kUnroll) {
-    const int64_t w0 = b0 + (int64_t)g.r * kUnroll;
-    uint64_t v[kUnroll], sw[kUnroll], qw[kUnroll];
+  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kKeyWords) {
+    const int64_t w0 = b0 + (int64_t)g.r * kKeyWords;
+    uint64_t v[kKeyWords], sw[kKeyWords], qw[kKeyWords];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kKeyWords; u++) {
       const int64_t wi = w0 + u;
       v[u] = 0;
       sw[u] = 0;
@@ -198,7 +207,7 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
     }
     int64_t found = -1;
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < kKeyWords; u++) {
       uint64_t t = found < 0 ? v[u] : 0;
       while (t) {
         const int64_t p = ((w0 + u) << 6) + ctz64(t);

@@ -50,12 +50,13 @@ struct IndexTune {
 // else line mode).
 enum QueryMode { kLineMode = 0, kTiledMode = 1, kChainMode = 2 };
 
-// Defaults: the line-mode shape the autotuner measured fastest on 4 GiB of
-// tweets (one lane per line, 4 lines per claim: 2.86 ms for $.user.lang against
-// 3.31 ms with 1 line per claim and 7.2 ms tiled), with member chains on the
-// chain evaluator.
+// Defaults: the shape the autotuner measured fastest on 4 GiB of tweets for
+// $.user.lang: line mode, one lane per line, 4 lines per claim (2.86 ms; 3.31 ms
+// with 1 line per claim, 7.2 ms tiled, 7.0 ms on the chain evaluator, which
+// wins only for chains that end near the line start, e.g. $.id 0.74 ms against
+// 1.05 ms).
 struct QueryTune {
-  int mode = kChainMode;
+  int mode = kLineMode;
   int block = 256;
   int group = 1;          // lanes per line
   int waves_per_cu = 16;  // persistent grid size factor
