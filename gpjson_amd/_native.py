@@ -9,7 +9,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpjson_hip.so")
+# GPJSON_LIB selects another build of the library (for comparing builds); the
+# in-tree build is the default.  There is no fallback: a missing library raises.
+LIB_PATH = os.environ.get("GPJSON_LIB") or os.path.join(HERE, "libgpjson_hip.so")
 
 GPJSON_OK = 0
 GPJSON_ERR_HIP = 1

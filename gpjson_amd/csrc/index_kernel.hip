@@ -508,6 +508,275 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   }
 }
 
+// Row layout of a tile: ROWS rows of BLOCK consecutive 64-byte words, lane t of
+// the block owning word t of every row.  Lanes of a wavefront own consecutive
+// words, so every store of the indexes is coalesced, and each lane loads its
+// words straight into registers (no LDS staging of the input: occupancy is set
+// by registers alone).  The word order of the scans is row-major.
+template <int BLOCK, int ROWS, int NLV, bool VERIFY>
+__global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
+  constexpr int NW = BLOCK / 64;
+  constexpr int TILE_WORDS = BLOCK * ROWS;
+
+  __shared__ unsigned s_ticket;
+  __shared__ int s_job;
+  __shared__ int s_esc[ROWS][NW];  // last escape code that is not 0 of each (row, wave)
+  __shared__ M4 s_m[ROWS][NW];     // transfer function of each (row, wave)
+  __shared__ int s_w0;             // sequence index of the tile's first word that is not all '\\'
+  __shared__ int s_w0_flip;
+  __shared__ M4 s_w0_m;
+  __shared__ Carry s_carry;
+  // the seven masks of every word, parked here (field-major, so the lanes of a
+  // wavefront touch consecutive 8-byte slots) while the scans and the
+  // lookback run, so they occupy no registers there
+  __shared__ uint64_t s_wc[ROWS][7][BLOCK];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const unsigned long long lanes_below = (1ull << lane) - 1ull;
+
+  for (;;) {
+    if (tid == 0) {
+      s_ticket = atomicAdd(launch.ticket, 1u);
+      s_w0 = -1;
+      s_w0_flip = 0;
+      unsigned t = s_ticket;
+      int lo = 0, hi = (int)launch.njobs - 1;
+      if (t < launch.total_tiles && launch.jobs) {
+        while (lo < hi) {  // last job with tile0 <= t
+          int mid = (lo + hi + 1) >> 1;
+          if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
+        }
+      }
+      s_job = lo;
+    }
+    __syncthreads();
+    const unsigned ticket = s_ticket;
+    if (ticket >= launch.total_tiles) break;
+    const FileJob J = launch.jobs ? launch.jobs[s_job] : launch.one;
+    const int64_t lt = (int64_t)ticket - J.tile0;
+    const int64_t word0 = lt * (int64_t)TILE_WORDS;
+
+    // ---- load and classify this lane's word of every row ----
+    int code[ROWS];     // escape summary: 0 all backslashes, else 2 + carry out
+    M4 wm0[ROWS], wm1[ROWS];  // the word's transfer function for escape carry-in 0 / 1
+    int flip[ROWS];     // the byte after the leading backslash run is a quote
+    {
+      uint4 raw[ROWS][4];
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        const int64_t off = (word0 + (int64_t)r * BLOCK + tid) * 64;
+        if (off + 64 <= J.N) {
+          const uint4* src = reinterpret_cast<const uint4*>(J.F + off);
+#pragma unroll
+          for (int q = 0; q < 4; q++) raw[r][q] = src[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            uint32_t w[4] = {0, 0, 0, 0};
+            for (int b = 0; b < 16; b++) {
+              const int64_t o = off + 16 * q + b;
+              if (o < J.N) w[b >> 2] |= (uint32_t)J.F[o] << (8 * (b & 3));
+            }
+            raw[r][q] = make_uint4(w[0], w[1], w[2], w[3]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        uint32_t d[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          d[4 * q + 0] = raw[r][q].x;
+          d[4 * q + 1] = raw[r][q].y;
+          d[4 * q + 2] = raw[r][q].z;
+          d[4 * q + 3] = raw[r][q].w;
+        }
+        const WordClass w = classify64(d);
+        code[r] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
+        flip[r] = code[r] ? (int)((w.Q >> ctz64(~w.B)) & 1) : 0;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const uint64_t E = escape_word(w.B, (uint32_t)c);
+          const uint64_t Qe = w.Q & ~E;
+          const uint64_t S0 = prefix_xor(Qe);
+          M4 m;
+          m.q = popc64(Qe) & 1;
+          m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
+          m.tot = popc64(w.O) - popc64(w.C);
+          m.nl = popc64(w.NL);
+          if (c == 0) wm0[r] = m; else wm1[r] = m;
+        }
+        s_wc[r][0][tid] = w.B;
+        s_wc[r][1][tid] = w.Q;
+        s_wc[r][2][tid] = w.NL;
+        s_wc[r][3][tid] = w.O;
+        s_wc[r][4][tid] = w.C;
+        s_wc[r][5][tid] = w.P;
+        s_wc[r][6][tid] = w.K;
+      }
+    }
+
+    // ---- escape carry: "last word that is not all backslashes" scan ----
+    int ent[ROWS];  // code of the nearest such word before this one in the tile (0: none)
+    {
+      unsigned long long nzm[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        nzm[r] = __ballot(code[r] != 0);
+        const int last = nzm[r] ? __shfl(code[r], 63 - __clzll(nzm[r])) : 0;
+        if (lane == 0) s_esc[r][wave] = last;
+        const unsigned long long below = nzm[r] & lanes_below;
+        const int e = __shfl(code[r], below ? (63 - __clzll(below)) : 0);
+        ent[r] = below ? e : -1;  // -1: resolve from earlier waves / rows
+      }
+      __syncthreads();
+      int run = 0;  // last code that is not 0 before the current (row, wave)
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          if (w == wave && ent[r] < 0) ent[r] = run;
+          const int e = s_esc[r][w];
+          if (e) run = e;
+        }
+      }
+    }
+    int tile_esc = 0;
+#pragma unroll
+    for (int r = 0; r < ROWS; r++)
+#pragma unroll
+      for (int w = 0; w < NW; w++)
+        if (s_esc[r][w]) tile_esc = s_esc[r][w];
+
+    // ---- per-word transfer functions (escape carry 0 where it comes from P) ----
+    M4 wm[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      wm[r] = (ent[r] == 3) ? wm1[r] : wm0[r];
+      if (ent[r] == 0 && code[r] != 0) {
+        // the tile's first word that is not all backslashes
+        s_w0 = r * BLOCK + tid;
+        s_w0_flip = flip[r];
+        s_w0_m = wm[r];
+      }
+    }
+
+    // ---- block scan of the transfer functions in row-major order ----
+    M4 lex[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      M4 inc = wm[r];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        M4 y = shfl_up_m(inc, off);
+        if (lane >= off) inc = m_cat(y, inc);
+      }
+      lex[r] = shfl_up_m(inc, 1);
+      if (lane == 0) lex[r] = m_id();
+      if (lane == 63) s_m[r][wave] = inc;
+    }
+    __syncthreads();
+    M4 texcl[ROWS];
+    M4 mtile = m_id();
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        if (w == wave) texcl[r] = m_cat(mtile, lex[r]);
+        mtile = m_cat(mtile, s_m[r][w]);
+      }
+    }
+
+    // ---- publish the tile aggregate and resolve the carry-in (wave 0) ----
+    if (wave == 0) {
+      TileState* st0 = launch.states + J.tile0;
+      TileAgg a;
+      a.esc = tile_esc;
+      a.m[0] = mtile;
+      a.m[1] = mtile;
+      if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
+      Carry cin;
+      if (lt == 0) {
+        cin = Carry{0, 0, -1, 1};
+        if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
+      } else {
+        if (lane == 0) publish_agg(st0 + lt, a, mtile.nl);
+        cin = lookback_wave(st0, lt, lane);
+      }
+      if (lane == 0) {
+        publish_inc(st0 + lt, apply_agg(a, cin));
+        s_carry = cin;
+        if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
+      }
+    }
+    __syncthreads();
+    const Carry X = s_carry;
+    const int w0 = s_w0;
+    const bool fix = (X.p == 1) && w0 >= 0 && s_w0_flip;
+    const M4 w0m = s_w0_m;
+
+    // ---- emit the indexes (lane t writes word t of each row: coalesced) ----
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int sidx = r * BLOCK + tid;
+      const int64_t w = word0 + sidx;
+      if (w >= J.W) continue;
+      M4 mx = texcl[r];
+      if (fix && sidx > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
+      const int p = ent[r] ? (ent[r] - 2) : X.p;
+      const int s = X.s ^ mx.q;
+      const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
+      const int64_t line = X.lines + mx.nl;
+      WordClass wd;
+      wd.B = s_wc[r][0][tid];
+      wd.Q = s_wc[r][1][tid];
+      wd.NL = s_wc[r][2][tid];
+      wd.O = s_wc[r][3][tid];
+      wd.C = s_wc[r][4][tid];
+      wd.P = s_wc[r][5][tid];
+      wd.K = s_wc[r][6][tid];
+      uint64_t valid = ~0ull;
+      const int64_t rem = J.N - w * 64;
+      if (rem < 64) valid = (1ull << rem) - 1ull;
+      const uint64_t E = escape_word(wd.B, (uint32_t)p);
+      const uint64_t Qe = wd.Q & ~E;
+      const uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
+      J.S[w] = S;
+      if (J.RC) J.RC[w] = wd.K & valid;
+      if (J.RQ) J.RQ[w] = wd.Q & valid;
+      if (VERIFY) {
+        if (J.E) J.E[w] = E & valid;
+        if (J.Q) J.Q[w] = Qe & valid;
+      }
+      const uint64_t O = wd.O & ~S, C = wd.C & ~S, P = wd.P & ~S;
+      const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
+      for (int lo = 0; lo < J.L; lo += NLV) {
+        uint64_t out[NLV];
+        if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
+#pragma unroll
+          for (int i = 0; i < NLV; i++) out[i] = 0;
+        } else {
+          leveled_word<NLV>(O, C, P, d, lo, J.L, out);
+        }
+#pragma unroll
+        for (int i = 0; i < NLV; i++)
+          if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+      }
+      uint64_t nlm = wd.NL;
+      int64_t idx = line;
+      while (nlm) {
+        const int b = ctz64(nlm);
+        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
+        idx++;
+        nlm &= nlm - 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Zeroes the lookback states and the ticket (a kernel, not hipMemsetAsync, so
 // the index pipeline can be captured into a graph).
 __global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket) {
@@ -545,8 +814,19 @@ struct IndexConfig {
   int nlv;
 };
 
+#ifndef GJ_INDEX_ROWS
+#define GJ_INDEX_ROWS 0
+#endif
+
 template <int BLOCK, int WPT, int NLV>
 static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
+  if (GJ_INDEX_ROWS) {
+    if (verify)
+      hipLaunchKernelGGL((index_rows_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
+    else
+      hipLaunchKernelGGL((index_rows_kernel<BLOCK, WPT, NLV, false>), dim3(grid), dim3(BLOCK), 0, stream, L);
+    return hipGetLastError();
+  }
   if (verify)
     hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
   else
