@@ -7,14 +7,17 @@
 //   newline index   int64[lines]   ([0] = 0, then every '\n')
 //   string index    uint64[W]      (in-string bits)
 //   leveled bitmaps uint64[L][Wp]  (structural chars per depth level)
+//   colon / quote   uint64[W]      (raw ':' and '"' bytes, for the interpreter)
 //   escape / quote  uint64[W]      (only in verify mode)
 //
-// Work decomposition.  A tile is BLOCK*WPT 64-byte words.  Tiles are claimed in
-// order through an atomic ticket, so every tile's predecessors are already
-// running.  The block stages its tile into LDS with coalesced 16-byte loads,
-// then each thread classifies its WPT consecutive words with byte-parallel
-// (SWAR) compares into 64-bit masks -- one lane owns one word, so all further
-// bit arithmetic is word-parallel across the wavefront.
+// Work decomposition.  A tile is ROWS rows of BLOCK consecutive 64-byte words;
+// lane t of the block owns word t of every row, so the lanes of a wavefront own
+// consecutive words: each lane loads its 64 bytes straight into registers
+// (16-byte loads), classifies them with byte-parallel (SWAR) compares into
+// 64-bit masks, and every index store of the wavefront is coalesced.  All
+// further bit arithmetic is word-parallel across the wavefront.  Tiles are
+// claimed in order through an atomic ticket, so every tile's predecessors are
+// already running.  Scans run in row-major word order.
 //
 // Cross-word state is a transfer function, not a value: a word's escape carry
 // out is fixed unless the word is all backslashes, and given the escape carry,
@@ -251,270 +254,45 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
 
 }  // namespace
 
-template <int BLOCK, int WPT, int NLV, bool VERIFY>
-__global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
-  constexpr int CH = WPT * 4;  // 16-byte chunks per thread
-  constexpr int TILE_CHUNKS = BLOCK * CH;
-  constexpr int64_t TILE_BYTES = (int64_t)TILE_CHUNKS * 16;
-  constexpr int NW = BLOCK / 64;
-
-  __shared__ uint4 tile[TILE_CHUNKS];
-  __shared__ unsigned s_ticket;
-  __shared__ int s_job;
-  __shared__ int s_wave_esc[NW];
-  __shared__ M4 s_wave_m[NW];
-  __shared__ int s_w0;        // tile word index of the first word that is not all '\\'
-  __shared__ int s_w0_flip;
-  __shared__ M4 s_w0_m;
-  __shared__ Carry s_carry;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-
-  for (;;) {
-    if (tid == 0) {
-      s_ticket = atomicAdd(launch.ticket, 1u);
-      s_w0 = -1;
-      s_w0_flip = 0;
-      unsigned t = s_ticket;
-      int lo = 0, hi = (int)launch.njobs - 1;
-      if (t < launch.total_tiles && launch.jobs) {
-        while (lo < hi) {  // last job with tile0 <= t
-          int mid = (lo + hi + 1) >> 1;
-          if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
-        }
-      }
-      s_job = lo;
+// The index words of word w -- string index, raw colon and quote bitmaps,
+// verify-mode escape/quote words, leveled bitmaps -- from its masks (row r,
+// lane tid of `wc`), the escape carry p, string state s and depth d before its
+// first byte.
+template <int NLV, bool VERIFY, int ROWS, int BLOCK>
+__device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uint64_t (&wc)[ROWS][7][BLOCK], int r,
+                                          int tid, int p, int s, int64_t d) {
+  uint64_t valid = ~0ull;
+  const int64_t rem = J.N - w * 64;
+  if (rem < 64) valid = (1ull << rem) - 1ull;
+  const uint64_t Bm = wc[r][0][tid], Qm = wc[r][1][tid];
+  const uint64_t E = escape_word(Bm, (uint32_t)p);
+  const uint64_t Qe = Qm & ~E;
+  const uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
+  J.S[w] = S;
+  if (J.RC) J.RC[w] = wc[r][6][tid] & valid;
+  if (J.RQ) J.RQ[w] = Qm & valid;
+  if (VERIFY) {
+    if (J.E) J.E[w] = E & valid;
+    if (J.Q) J.Q[w] = Qe & valid;
+  }
+  const uint64_t O = wc[r][3][tid] & ~S, C = wc[r][4][tid] & ~S, P = wc[r][5][tid] & ~S;
+  const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
+  for (int lo = 0; lo < J.L; lo += NLV) {
+    uint64_t out[NLV];
+    if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
+#pragma unroll
+      for (int i = 0; i < NLV; i++) out[i] = 0;
+    } else {
+      leveled_word<NLV>(O, C, P, d, lo, J.L, out);
     }
-    __syncthreads();
-    const unsigned ticket = s_ticket;
-    if (ticket >= launch.total_tiles) break;
-    const FileJob J = launch.jobs ? launch.jobs[s_job] : launch.one;
-    const int64_t lt = (int64_t)ticket - J.tile0;
-    const int64_t byte0 = lt * TILE_BYTES;
-    const int64_t word0 = lt * (int64_t)(BLOCK * WPT);
-
-    // ---- stage the tile into LDS (coalesced 16-byte loads, swizzled slots) ----
-    for (int c = tid; c < TILE_CHUNKS; c += BLOCK) {
-      int64_t off = byte0 + 16 * (int64_t)c;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (off + 16 <= J.N) {
-        v = *reinterpret_cast<const uint4*>(J.F + off);
-      } else if (off < J.N) {
-        uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
 #pragma unroll
-        for (int b = 0; b < 16; b++) {
-          uint32_t x = (off + b < J.N) ? ((uint32_t)J.F[off + b] << (8 * (b & 3))) : 0u;
-          if (b < 4) w0 |= x; else if (b < 8) w1 |= x; else if (b < 12) w2 |= x; else w3 |= x;
-        }
-        v = make_uint4(w0, w1, w2, w3);
-      }
-      int th = c / CH, j = c % CH;
-      tile[th * CH + (j ^ (th & (CH - 1)))] = v;
-    }
-    __syncthreads();
-
-    // ---- classify this thread's words ----
-    // Each word's seven masks replace its (already consumed) raw bytes in this
-    // thread's own LDS slots, so they do not occupy registers across the scan
-    // and lookback phases.
-    auto slot = [&](int c) { return tid * CH + (c ^ (tid & (CH - 1))); };
-    auto lo32 = [](uint64_t x) { return (uint32_t)x; };
-    auto hi32 = [](uint64_t x) { return (uint32_t)(x >> 32); };
-    auto u64 = [](uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; };
-    int code[WPT];  // escape summary per word: 0 all backslashes, else 2 + carry out
-    int tex = 0;
-#pragma unroll
-    for (int k = 0; k < WPT; k++) {
-      uint32_t d[16];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        uint4 v = tile[slot(4 * k + q)];
-        d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
-      }
-      const WordClass w = classify64(d);
-      tile[slot(4 * k + 0)] = make_uint4(lo32(w.B), hi32(w.B), lo32(w.Q), hi32(w.Q));
-      tile[slot(4 * k + 1)] = make_uint4(lo32(w.NL), hi32(w.NL), lo32(w.O), hi32(w.O));
-      tile[slot(4 * k + 2)] = make_uint4(lo32(w.C), hi32(w.C), lo32(w.P), hi32(w.P));
-      tile[slot(4 * k + 3)] = make_uint4(lo32(w.K), hi32(w.K), 0u, 0u);
-      code[k] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
-      if (code[k]) tex = code[k];
-    }
-    auto load_word = [&](int k) {
-      const uint4 a = tile[slot(4 * k + 0)], b = tile[slot(4 * k + 1)], c = tile[slot(4 * k + 2)];
-      WordClass w;
-      w.B = u64(a.x, a.y);
-      w.Q = u64(a.z, a.w);
-      w.NL = u64(b.x, b.y);
-      w.O = u64(b.z, b.w);
-      w.C = u64(c.x, c.y);
-      w.P = u64(c.z, c.w);
-      return w;
-    };
-
-    // ---- escape carry: "last word that is not all backslashes" scan ----
-    const unsigned long long nzm = __ballot(tex != 0);
-    const unsigned long long below = nzm & ((1ull << lane) - 1ull);
-    int ent = __shfl(tex, below ? (63 - __clzll(below)) : 0);
-    if (!below) ent = 0;
-    const int wlast = __shfl(tex, nzm ? (63 - __clzll(nzm)) : 0);
-    if (lane == 0) s_wave_esc[wave] = nzm ? wlast : 0;
-    __syncthreads();
-    int tile_esc = 0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      int e = s_wave_esc[w];
-      if (w < wave && e && !below) ent = e;
-      if (e) tile_esc = e;
-    }
-
-    // ---- per-word transfer functions (escape carry assumed 0 where it comes from P) ----
-    int pcode[WPT];
-    M4 wm[WPT];
-    {
-      int run = ent;
-#pragma unroll
-      for (int k = 0; k < WPT; k++) {
-        pcode[k] = run;
-        int p = run ? (run - 2) : 0;
-        const WordClass w = load_word(k);
-        uint64_t E = escape_word(w.B, (uint32_t)p);
-        uint64_t Qe = w.Q & ~E;
-        uint64_t S0 = prefix_xor(Qe);
-        int opens = popc64(w.O), closes = popc64(w.C);
-        M4 m;
-        m.q = popc64(Qe) & 1;
-        m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
-        m.tot = opens - closes;
-        m.nl = popc64(w.NL);
-        wm[k] = m;
-        if (run == 0 && code[k] != 0) {
-          // first word of the tile that is not all backslashes
-          int r = ctz64(~w.B);
-          s_w0 = tid * WPT + k;
-          s_w0_flip = (int)((w.Q >> r) & 1);
-          s_w0_m = m;
-        }
-        if (code[k]) run = code[k];
-      }
-    }
-
-    // ---- block scan of the word transfer functions ----
-    M4 tin = m_id();
-#pragma unroll
-    for (int k = 0; k < WPT; k++) tin = m_cat(tin, wm[k]);
-    M4 inc = tin;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      M4 y = shfl_up_m(inc, off);
-      if (lane >= off) inc = m_cat(y, inc);
-    }
-    M4 lex = shfl_up_m(inc, 1);
-    if (lane == 0) lex = m_id();
-    if (lane == 63) s_wave_m[wave] = inc;
-    __syncthreads();
-    M4 wpre = m_id(), mtile = m_id();
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      if (w < wave) wpre = m_cat(wpre, s_wave_m[w]);
-      mtile = m_cat(mtile, s_wave_m[w]);
-    }
-    M4 texcl = m_cat(wpre, lex);
-
-    // ---- publish the tile aggregate and resolve the carry-in (wave 0) ----
-    if (wave == 0) {
-      TileState* st0 = launch.states + J.tile0;
-      TileAgg a;
-      a.esc = tile_esc;
-      a.m[0] = mtile;
-      a.m[1] = mtile;
-      if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
-      Carry cin;
-      if (lt == 0) {
-        cin = Carry{0, 0, -1, 1};
-        if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
-      } else {
-        if (lane == 0) publish_agg(st0 + lt, a, mtile.nl);
-        cin = lookback_wave(st0, lt, lane);
-      }
-      if (lane == 0) {
-        publish_inc(st0 + lt, apply_agg(a, cin));
-        s_carry = cin;
-        if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
-      }
-    }
-    __syncthreads();
-    const Carry X = s_carry;
-    const int w0 = s_w0;
-    const bool fix = (X.p == 1) && w0 >= 0 && s_w0_flip;
-    const M4 w0m = s_w0_m;
-
-    // ---- emit the indexes ----
-    M4 run = texcl;
-#pragma unroll
-    for (int k = 0; k < WPT; k++) {
-      const int tw = tid * WPT + k;
-      const int64_t w = word0 + tw;
-      M4 mx = run;
-      if (fix && tw > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
-      run = m_cat(run, wm[k]);
-      if (w >= J.W) continue;
-      const int p = pcode[k] ? (pcode[k] - 2) : X.p;
-      const int s = X.s ^ mx.q;
-      const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-      const int64_t line = X.lines + mx.nl;
-      const WordClass wd = load_word(k);
-      uint64_t valid = ~0ull;
-      const int64_t rem = J.N - w * 64;
-      if (rem < 64) valid = (1ull << rem) - 1ull;
-      uint64_t E = escape_word(wd.B, (uint32_t)p);
-      uint64_t Qe = wd.Q & ~E;
-      uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
-      J.S[w] = S;
-      if (J.RC) {
-        const uint4 kk = tile[slot(4 * k + 3)];
-        J.RC[w] = u64(kk.x, kk.y) & valid;
-      }
-      if (J.RQ) J.RQ[w] = wd.Q & valid;
-      if (VERIFY) {
-        if (J.E) J.E[w] = E & valid;
-        if (J.Q) J.Q[w] = Qe & valid;
-      }
-      const uint64_t O = wd.O & ~S, C = wd.C & ~S, P = wd.P & ~S;
-      const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
-      for (int lo = 0; lo < J.L; lo += NLV) {
-        uint64_t out[NLV];
-        if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
-#pragma unroll
-          for (int i = 0; i < NLV; i++) out[i] = 0;
-        } else {
-          leveled_word<NLV>(O, C, P, d, lo, J.L, out);
-        }
-#pragma unroll
-        for (int i = 0; i < NLV; i++)
-          if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
-      }
-      uint64_t nlm = wd.NL;
-      int64_t idx = line;
-      while (nlm) {
-        int b = ctz64(nlm);
-        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
-        idx++;
-        nlm &= nlm - 1;
-      }
-    }
-    __syncthreads();
+    for (int i = 0; i < NLV; i++)
+      if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
   }
 }
 
-// Row layout of a tile: ROWS rows of BLOCK consecutive 64-byte words, lane t of
-// the block owning word t of every row.  Lanes of a wavefront own consecutive
-// words, so every store of the indexes is coalesced, and each lane loads its
-// words straight into registers (no LDS staging of the input: occupancy is set
-// by registers alone).  The word order of the scans is row-major.
 template <int BLOCK, int ROWS, int NLV, bool VERIFY>
-__global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
+__global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
 
@@ -559,9 +337,9 @@ __global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
     const int64_t word0 = lt * (int64_t)TILE_WORDS;
 
     // ---- load and classify this lane's word of every row ----
-    int code[ROWS];     // escape summary: 0 all backslashes, else 2 + carry out
-    M4 wm0[ROWS], wm1[ROWS];  // the word's transfer function for escape carry-in 0 / 1
-    int flip[ROWS];     // the byte after the leading backslash run is a quote
+    int code[ROWS];  // escape summary: 0 all backslashes, else 2 + carry out
+    M4 wm0[ROWS];    // the word's transfer function for escape carry-in 0
+    int flip[ROWS];  // the byte after the leading backslash run is a quote
     {
       uint4 raw[ROWS][4];
 #pragma unroll
@@ -596,9 +374,8 @@ __global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
         const WordClass w = classify64(d);
         code[r] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
         flip[r] = code[r] ? (int)((w.Q >> ctz64(~w.B)) & 1) : 0;
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-          const uint64_t E = escape_word(w.B, (uint32_t)c);
+        {
+          const uint64_t E = escape_word(w.B, 0u);
           const uint64_t Qe = w.Q & ~E;
           const uint64_t S0 = prefix_xor(Qe);
           M4 m;
@@ -606,7 +383,7 @@ __global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
           m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
           m.tot = popc64(w.O) - popc64(w.C);
           m.nl = popc64(w.NL);
-          if (c == 0) wm0[r] = m; else wm1[r] = m;
+          wm0[r] = m;
         }
         s_wc[r][0][tid] = w.B;
         s_wc[r][1][tid] = w.Q;
@@ -654,7 +431,16 @@ __global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
     M4 wm[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
-      wm[r] = (ent[r] == 3) ? wm1[r] : wm0[r];
+      wm[r] = wm0[r];
+      if (ent[r] == 3) {
+        // escape carry-in 1 (the word before ends in an odd backslash run): rare
+        const uint64_t B = s_wc[r][0][tid], Q = s_wc[r][1][tid], O = s_wc[r][3][tid], C = s_wc[r][4][tid];
+        const uint64_t E = escape_word(B, 1u);
+        const uint64_t Qe = Q & ~E;
+        const uint64_t S0 = prefix_xor(Qe);
+        wm[r].q = popc64(Qe) & 1;
+        wm[r].d0 = popc64(O & ~S0) - popc64(C & ~S0);
+      }
       if (ent[r] == 0 && code[r] != 0) {
         // the tile's first word that is not all backslashes
         s_w0 = r * BLOCK + tid;
@@ -728,44 +514,9 @@ __global__ __launch_bounds__(BLOCK) void index_rows_kernel(IndexLaunch launch) {
       const int p = ent[r] ? (ent[r] - 2) : X.p;
       const int s = X.s ^ mx.q;
       const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-      const int64_t line = X.lines + mx.nl;
-      WordClass wd;
-      wd.B = s_wc[r][0][tid];
-      wd.Q = s_wc[r][1][tid];
-      wd.NL = s_wc[r][2][tid];
-      wd.O = s_wc[r][3][tid];
-      wd.C = s_wc[r][4][tid];
-      wd.P = s_wc[r][5][tid];
-      wd.K = s_wc[r][6][tid];
-      uint64_t valid = ~0ull;
-      const int64_t rem = J.N - w * 64;
-      if (rem < 64) valid = (1ull << rem) - 1ull;
-      const uint64_t E = escape_word(wd.B, (uint32_t)p);
-      const uint64_t Qe = wd.Q & ~E;
-      const uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
-      J.S[w] = S;
-      if (J.RC) J.RC[w] = wd.K & valid;
-      if (J.RQ) J.RQ[w] = wd.Q & valid;
-      if (VERIFY) {
-        if (J.E) J.E[w] = E & valid;
-        if (J.Q) J.Q[w] = Qe & valid;
-      }
-      const uint64_t O = wd.O & ~S, C = wd.C & ~S, P = wd.P & ~S;
-      const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
-      for (int lo = 0; lo < J.L; lo += NLV) {
-        uint64_t out[NLV];
-        if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
-#pragma unroll
-          for (int i = 0; i < NLV; i++) out[i] = 0;
-        } else {
-          leveled_word<NLV>(O, C, P, d, lo, J.L, out);
-        }
-#pragma unroll
-        for (int i = 0; i < NLV; i++)
-          if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
-      }
-      uint64_t nlm = wd.NL;
-      int64_t idx = line;
+      emit_word<NLV, VERIFY>(J, w, s_wc, r, tid, p, s, d);
+      uint64_t nlm = s_wc[r][2][tid];
+      int64_t idx = X.lines + mx.nl;
       while (nlm) {
         const int b = ctz64(nlm);
         if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
@@ -814,19 +565,8 @@ struct IndexConfig {
   int nlv;
 };
 
-#ifndef GJ_INDEX_ROWS
-#define GJ_INDEX_ROWS 0
-#endif
-
 template <int BLOCK, int WPT, int NLV>
 static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
-  if (GJ_INDEX_ROWS) {
-    if (verify)
-      hipLaunchKernelGGL((index_rows_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
-    else
-      hipLaunchKernelGGL((index_rows_kernel<BLOCK, WPT, NLV, false>), dim3(grid), dim3(BLOCK), 0, stream, L);
-    return hipGetLastError();
-  }
   if (verify)
     hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
   else
