@@ -161,8 +161,6 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     q.ir_len = ir_lens[j];
     q.out = (int64_t*)b->arena_spans.p + b->job_span_off[j];
     q.item0 = 0;
-    q.gfirst = b->file_first_job[b->job_file[j]];
-    q.gcount = b->file_first_job[b->job_file[j] + 1] - q.gfirst;
     off += ir_lens[j];
   }
   uint8_t* misc = (uint8_t*)b->dev_misc.p;
@@ -202,7 +200,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
                                               b->index_grid, s));
   if (e1) GJ_HIP(hipEventRecord(e1, s));
   GJ_HIP(batch_plan((const FileJob*)b->dev_files.p, b->nfiles, L.states, (QueryJob*)b->dev_qjobs.p, b->job_file_dev,
-                    b->njobs, b->total_dev, b->lines_dev, b->overflow_dev, b->chain ? 0 : 1, s));
+                    b->njobs, b->total_dev, b->lines_dev, b->overflow_dev, s));
   QueryLaunch Q{};
   Q.jobs = (const QueryJob*)b->dev_qjobs.p;
   Q.njobs = b->njobs;
@@ -211,7 +209,6 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.total_dev = b->total_dev;
   Q.counter = b->counter;
   Q.lines_per_wave = b->qtune.lines_per_wave;
-  Q.grouped = b->chain ? 0 : 1;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
   if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->qtune.group, b->query_grid, s));

@@ -33,12 +33,6 @@ struct QueryJob {
   int64_t ir_len;
   int64_t* out;       // int64[lines][R][2]
   int64_t item0;      // first global work item (line, or tile in tiled mode) of this job
-  // Grouped launches: the jobs [gfirst, gfirst + gcount) share one index and
-  // one range of work items (same item0); a work item is one line, on which
-  // every job of the group is evaluated in turn, so the line's index words are
-  // read from memory once and from cache by the group's other queries.
-  int gfirst;
-  int gcount;
   // tiled mode: lines starting in index tile t are [tile_lines[t], tile_lines[t+1])
   const int64_t* tile_lines;
   int64_t ntiles;
@@ -62,9 +56,6 @@ struct QueryLaunch {
   // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
   // tile prefix); a block stages a tile's leveled and string words in LDS.
   int tiled;
-  // line mode: work items are lines of job groups (QueryJob::gfirst/gcount);
-  // with lines_dev set, every job forms one group over the common index
-  int grouped;
   // Every job's IR lies in [ir_base, ir_base + ir_bytes).  When ir_bytes fits
   // kIrLdsBytes, each block copies it to LDS once and interprets from there.
   const uint8_t* ir_base;

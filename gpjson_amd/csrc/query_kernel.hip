@@ -403,7 +403,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
   __syncthreads();
   auto ir_of = [&](const QueryJob& J) { return ir_lds ? s_ir + (J.ir - launch.ir_base) : J.ir; };
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
-  const int64_t total = launch.lines_dev ? (launch.grouped ? common : common * launch.njobs)
+  const int64_t total = launch.lines_dev ? common * launch.njobs
                                          : (launch.total_dev ? *launch.total_dev : launch.total_items);
   // Several lines per group amortize a claim over short lines, but with few
   // (long) lines they serialize work one group could share: halve the claim
@@ -425,14 +425,10 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
     if ((int64_t)g0 >= total) continue;
     if (launch.lines_dev) {
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-        const int64_t q0 = launch.grouped ? 0 : it / common;
-        const int64_t i = it - q0 * common;
-        const int64_t q1 = launch.grouped ? launch.njobs : q0 + 1;
-        for (int64_t q = q0; q < q1; q++) {  // every query of the group on this line
-          const QueryJob& J = launch.jobs[q];
-          int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-          eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
-        }
+        const int64_t q = it / common, i = it - q * common;
+        const QueryJob& J = launch.jobs[q];
+        int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
       }
       continue;
     }
@@ -443,15 +439,10 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
     }
     for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
       while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
-      // lo: the last job whose range holds the item; with groups, its group
-      const int q0 = launch.grouped ? launch.jobs[lo].gfirst : lo;
-      const int q1 = launch.grouped ? q0 + launch.jobs[lo].gcount : lo + 1;
-      for (int q = q0; q < q1; q++) {
-        const QueryJob& J = launch.jobs[q];
-        const int64_t i = it - J.item0;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
-                  nslots);
-      }
+      const QueryJob& J = launch.jobs[lo];
+      const int64_t i = it - J.item0;
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+                nslots);
     }
   }
 }
@@ -525,7 +516,7 @@ __global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, 
 // interpreter launch needs no host round trip.  One block.
 __global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs,
                                   const int* job_file, int njobs, int64_t* total_out, int64_t* lines_out,
-                                  int* overflow, int grouped) {
+                                  int* overflow) {
   if (threadIdx.x != 0) return;
   int64_t item = 0;
   for (int f = 0; f < nfiles; f++) {
@@ -541,9 +532,6 @@ __global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileSt
     const FileJob& J = files[job_file[j]];
     int64_t lines = lines_out[job_file[j]];
     if (lines > J.nl_cap) lines = J.nl_cap;
-    // grouped: the jobs of one file share its lines as work items
-    const bool same = grouped && j > 0 && job_file[j - 1] == job_file[j];
-    if (same) item -= qjobs[j - 1].lines;
     qjobs[j].lines = lines;
     qjobs[j].item0 = item;
     item += lines;
@@ -552,10 +540,9 @@ __global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileSt
 }
 
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
-                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, int grouped,
-                      hipStream_t stream) {
+                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream) {
   hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, states, qjobs, job_file, njobs,
-                     total_out, lines_out, overflow, grouped);
+                     total_out, lines_out, overflow);
   return hipGetLastError();
 }
 

@@ -245,8 +245,6 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.ir_len = ir_lens[q];
     j.out = tiled ? spans_dev + (int64_t)q * idx->lines * R * 2 : spans_dev;
     j.item0 = tiled ? (int64_t)q * idx->ntiles : 0;
-    j.gfirst = 0;
-    j.gcount = nq;
     j.tile_lines = idx->tile_lines;
     j.ntiles = idx->ntiles;
     j.tile_bytes = idx->tile_bytes;
@@ -264,7 +262,6 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.counter = (unsigned long long*)dbase;
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   L.tiled = tiled ? 1 : 0;
-  L.grouped = (!tiled && !chain) ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;

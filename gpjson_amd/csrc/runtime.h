@@ -27,7 +27,7 @@ hipError_t chain_launch(const QueryLaunch& L, int block, int group, int grid, hi
 bool query_config_supported(int block, int group);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
-                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, int grouped, hipStream_t stream);
+                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream);
 
 // Error reporting: every C entry point funnels failures through these.
 int fail(int code, const std::string& msg);

@@ -248,26 +248,6 @@ void spill_oldest(gpjson_server* s) {
 int launch_batch(gpjson_server* s, Batch& b) {
   gpjson_ctx* ctx = s->qctx;
   const int n = (int)b.reqs.size();
-  // requests on the same file become one job group: each line is read once
-  // and every query of the group is evaluated on it
-  {
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b.files[x] < b.files[y]; });
-    std::vector<std::shared_ptr<Request>> reqs(n);
-    std::vector<ServedFile*> files(n);
-    for (int i = 0; i < n; i++) {
-      reqs[i] = b.reqs[order[i]];
-      files[i] = b.files[order[i]];
-    }
-    b.reqs.swap(reqs);
-    b.files.swap(files);
-  }
-  // the member-chain evaluator when every request is a chain and the IR fits LDS
-  int64_t ir_all = 0;
-  for (int i = 0; i < n; i++) ir_all += (int64_t)b.reqs[i]->cp.ir.size();
-  bool chain = ctx->qtune.mode == kChainMode && ir_all <= kIrLdsBytes;
-  for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
   std::vector<QueryJob> jobs(n);
   int64_t items = 0, elems = 0, ir_total = 0;
   int maxL = 0;
@@ -311,24 +291,13 @@ int launch_batch(gpjson_server* s, Batch& b) {
     j.R = r.R;
     j.ir = (const uint8_t*)b.jobs.p + irb;
     j.ir_len = (int64_t)r.cp.ir.size();
-    // a group: the run of requests on one file (chain launches: one job each)
-    const bool joins = !chain && i > 0 && b.files[i - 1] == b.files[i];
-    if (joins) {
-      j.gfirst = jobs[i - 1].gfirst;
-      j.item0 = jobs[i - 1].item0;
-    } else {
-      j.gfirst = i;
-      j.item0 = items;
-      items += idx->lines;
-    }
+    j.item0 = items;
     std::memcpy(host + irb, r.cp.ir.data(), r.cp.ir.size());
     irb += (int64_t)r.cp.ir.size();
+    items += idx->lines;
     elems += idx->lines * r.R * 2;
     maxL = std::max(maxL, idx->L);
   }
-  for (int i = 0; i < n; i++) jobs[i].gcount = 0;
-  for (int i = 0; i < n; i++) jobs[jobs[i].gfirst].gcount++;
-  for (int i = 0; i < n; i++) jobs[i].gcount = jobs[jobs[i].gfirst].gcount;
   rc = b.spans.ensure((size_t)std::max<int64_t>(elems, 1) * 8);
   if (rc) return rc;
   for (int i = 0; i < n; i++) jobs[i].out = (int64_t*)b.spans.p + b.offsets[i];
@@ -350,12 +319,13 @@ int launch_batch(gpjson_server* s, Batch& b) {
   L.total_items = items;
   L.counter = (unsigned long long*)b.jobs.p;
   L.lines_per_wave = ctx->qtune.lines_per_wave;
-  L.grouped = chain ? 0 : 1;
   L.ir_base = (const uint8_t*)b.jobs.p + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
   GJ_HIP(zero_u64(L.counter, 1, ctx->stream));
   const int waves_per_block = ctx->qtune.block / 64;
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
+  bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
+  for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
   if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, ctx->stream));
   else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream));
   GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, ctx->stream));
