@@ -93,6 +93,9 @@ SIGNATURES = {
     "gpjson_server_wait": (c_int, [c_void_p, ctypes.c_uint64, c_int64, P(c_int64), P(c_int), P(c_double), c_void_p,
                                    c_int64]),
     "gpjson_server_stats": (c_int, [c_void_p, P(c_int64), c_int]),
+    "gpjson_server_result": (c_int, [c_void_p, ctypes.c_uint64, c_int64, P(P(c_int64)), P(c_int64), P(c_int),
+                                     P(c_double)]),
+    "gpjson_server_release": (c_int, [c_void_p, ctypes.c_uint64]),
     "gpjson_server_destroy": (None, [c_void_p]),
     "gpjson_time_copy": (c_int, [c_void_p, c_int64, c_int, P(c_double)]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),

@@ -73,7 +73,7 @@ struct Request {
   std::vector<int64_t> spans;  // results moved out of a batch buffer
   int slot = -1;               // batch slot whose pinned buffer holds the results
   int64_t off = 0;             // element offset in that buffer
-  bool collecting = false;     // a waiter is copying from the slot
+  bool collecting = false;     // a waiter is copying from the slot, or a client holds a view of it
   int64_t file_bytes = 0;
   bool chain = false;          // a member chain (chain evaluator)
 };
@@ -532,6 +532,54 @@ int gpjson_server_wait(gpjson_server* s, uint64_t ticket, int64_t timeout_ms, in
     std::memcpy(spans, (const int64_t*)b.host_spans + r->off, (size_t)cnt * 8);
     lk.lock();
     r->collecting = false;
+    r->slot = -1;
+    if (--b.outstanding == 0) {
+      b.reqs.clear();
+      s->cv.notify_all();
+    }
+  }
+  return GPJSON_OK;
+}
+
+// Zero-copy access: waits like gpjson_server_wait, then gives a pointer to the
+// request's spans where the server holds them (its batch's pinned buffer, or
+// host memory after a spill).  The pointer stays valid, and that batch buffer
+// stays reserved, until gpjson_server_release.
+int gpjson_server_result(gpjson_server* s, uint64_t ticket, int64_t timeout_ms, const int64_t** spans,
+                         int64_t* lines, int* R, double* latency_us) {
+  if (!s || !spans) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return fail(GPJSON_ERR_INVALID_ARG, "unknown ticket");
+  std::shared_ptr<Request> r = it->second;
+  auto ready = [&] { return r->state >= 2 || !s->fatal.empty(); };
+  if (timeout_ms < 0) s->done_cv.wait(lk, ready);
+  else if (!s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+    return fail(GPJSON_ERR_CAPACITY, "timeout");
+  if (r->state == 3) return fail(GPJSON_ERR_HIP, r->error);
+  if (r->state != 2) return fail(GPJSON_ERR_NO_MEMORY, s->fatal);
+  if (r->collecting) return fail(GPJSON_ERR_INVALID_ARG, "ticket is already held or being collected");
+  r->collecting = true;  // keeps the slot reserved (spills skip it)
+  *spans = r->slot >= 0 ? (const int64_t*)s->slots[r->slot].host_spans + r->off : r->spans.data();
+  if (lines) *lines = r->lines;
+  if (R) *R = r->R;
+  if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->completed - r->submitted).count();
+  return GPJSON_OK;
+}
+
+// Ends a zero-copy access: the ticket is forgotten and its batch buffer freed
+// for reuse once every request of that batch is collected or released.
+int gpjson_server_release(gpjson_server* s, uint64_t ticket) {
+  if (!s) return fail(GPJSON_ERR_INVALID_ARG, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return fail(GPJSON_ERR_INVALID_ARG, "unknown ticket");
+  std::shared_ptr<Request> r = it->second;
+  if (!r->collecting || r->state != 2) return fail(GPJSON_ERR_INVALID_ARG, "ticket is not held");
+  s->requests.erase(it);
+  r->collecting = false;
+  if (r->slot >= 0) {
+    Batch& b = s->slots[r->slot];
     r->slot = -1;
     if (--b.outstanding == 0) {
       b.reqs.clear();

@@ -59,6 +59,18 @@ class Server:
                                             out.ctypes.data_as(ctypes.c_void_p), out.size))
         return out, lat.value
 
+    def result(self, ticket: int, timeout_ms: int = -1) -> "ServedResult":
+        """Zero-copy result: a view of the spans in the server's pinned memory,
+        valid until ServedResult.release() (use it as a context manager)."""
+        ptr = ctypes.POINTER(ctypes.c_int64)()
+        lines, R, lat = ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.gpjson_server_result(self._h, ticket, timeout_ms, ctypes.byref(ptr), ctypes.byref(lines),
+                                              ctypes.byref(R), ctypes.byref(lat)))
+        n = lines.value * R.value * 2
+        view = np.ctypeslib.as_array(ptr, shape=(n,)).reshape(lines.value, R.value, 2) if n else \
+            np.empty((lines.value, R.value, 2), dtype=np.int64)
+        return ServedResult(self, ticket, view, lat.value)
+
     def stats(self) -> dict:
         v = (ctypes.c_int64 * 8)()
         _check(self._lib.gpjson_server_stats(self._h, v, 8))
@@ -84,6 +96,24 @@ class Server:
             pass
 
 
+class ServedResult:
+    """Spans of one served request, read in place (see Server.result)."""
+
+    def __init__(self, server: Server, ticket: int, spans: np.ndarray, latency_us: float):
+        self.server, self.ticket, self.spans, self.latency_us = server, ticket, spans, latency_us
+
+    def release(self) -> None:
+        if self.spans is not None:
+            self.spans = None
+            _check(self.server._lib.gpjson_server_release(self.server._h, self.ticket))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+
 def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: float, seed: int = 0,
               keep_spans: bool = False, wait_timeout_ms: int = 120000):
     """Submits `requests` with exponential inter-arrival times (Poisson arrivals
@@ -98,13 +128,14 @@ def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: f
     errors: List[BaseException] = []
 
     def collector():
+        # results are read in place (zero copy) and released at once
         try:
             for i in range(len(requests)):
                 ready[i].wait()
-                sp, l_us = server.wait(tickets[i], wait_timeout_ms)
-                lat[i] = l_us
-                if keep_spans:
-                    spans[i] = sp
+                with server.result(tickets[i], wait_timeout_ms) as res:
+                    lat[i] = res.latency_us
+                    if keep_spans:
+                        spans[i] = res.spans.copy()
         except BaseException as e:  # surfaced to the caller
             errors.append(e)
 

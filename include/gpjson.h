@@ -218,6 +218,14 @@ int gpjson_server_submit(gpjson_server *server, const char *key, const char *pat
  * NULL) copies int64[lines][R][2] spans, after which the ticket is forgotten. */
 int gpjson_server_wait(gpjson_server *server, uint64_t ticket, int64_t timeout_ms, int64_t *lines, int *R,
                        double *latency_us, int64_t *spans, int64_t cap_elems);
+/* Zero-copy variant of gpjson_server_wait: *spans points at the request's
+ * int64[lines][R][2] spans where the server keeps them (pinned host memory).
+ * The pointer stays valid, and the batch buffer holding it stays reserved,
+ * until gpjson_server_release(ticket); release promptly: while every batch
+ * buffer holds unreleased results the server launches nothing new. */
+int gpjson_server_result(gpjson_server *server, uint64_t ticket, int64_t timeout_ms, const int64_t **spans,
+                         int64_t *lines, int *R, double *latency_us);
+int gpjson_server_release(gpjson_server *server, uint64_t ticket);
 /* Fills min(n, 8) counters: resident bytes, resident files, interpreter
  * launches, requests served by them, largest batch, admissions, evictions,
  * spills (batches whose uncollected results were moved to host memory). */

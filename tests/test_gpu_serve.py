@@ -104,6 +104,28 @@ def test_uncollected_results_spill(files, plain):
             assert np.array_equal(sp, plain[(key, p)]), (key, p)
 
 
+def test_zero_copy_results(files, plain):
+    """Result views read the spans in place; views held across later batches
+    stay valid and the server keeps serving."""
+    from gpjson_amd.serve import Server
+    with Server(hbm_budget_bytes=1 << 34, levels=4) as srv:
+        for key, (_, data) in files.items():
+            srv.add(key, data)
+        reqs = _reqs(files, 10, 33)
+        held = []
+        for n, (key, p) in enumerate(reqs):
+            res = srv.result(srv.submit(key, p), 60000)
+            assert np.array_equal(res.spans, plain[(key, p)]), (key, p)
+            if n < 2:
+                held.append((res, plain[(key, p)]))
+            else:
+                res.release()
+        for res, want in held:  # still intact after eight more batches
+            assert np.array_equal(res.spans, want)
+            res.release()
+        print(f"[serve] zero-copy stats {srv.stats()}", flush=True)
+
+
 def test_open_loop_poisson(files, plain):
     from gpjson_amd.serve import Server, open_loop
     with Server(hbm_budget_bytes=1 << 34, levels=4) as srv:
