@@ -40,6 +40,8 @@ struct gpjson_batch {
   bool chain = false;    // every job is a member chain: chain evaluator
   // device memory
   DevBuf arena_s, arena_rc, arena_rq, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
+  DevBuf arena_long;      // [2 counters, 64 B][long-line work items]
+  int64_t long_cap = 0;
   int64_t* total_dev = nullptr;
   int64_t* lines_dev = nullptr;
   int* overflow_dev = nullptr;
@@ -116,6 +118,9 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
   if (!rc) rc = b->arena_lb.ensure((size_t)std::max<int64_t>(lb_words, 1) * 8);
   if (!rc) rc = b->arena_nl.ensure((size_t)std::max<int64_t>(nl_elems, 1) * 8);
   if (!rc) rc = b->arena_spans.ensure((size_t)std::max<int64_t>(span_elems, 1) * 8);
+  b->long_cap = 0;
+  for (int j = 0; j < b->njobs; j++) b->long_cap += b->bufs[b->job_file[j]]->n / kLongLineBytes + 1;
+  if (!rc) rc = b->arena_long.ensure(64 + (size_t)b->long_cap * 8);
   if (!rc) rc = b->states.ensure((size_t)std::max<int64_t>(tiles, 1) * sizeof(TileState));
   if (!rc) rc = b->dev_files.ensure(sizeof(FileJob) * (size_t)b->nfiles);
   if (!rc) rc = b->dev_qjobs.ensure(sizeof(QueryJob) * (size_t)b->njobs);
@@ -211,8 +216,16 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
+  const bool long_lines = !b->chain && b->qtune.group != 64;
+  if (long_lines) {
+    Q.long_count = (unsigned long long*)b->arena_long.p;
+    Q.long_items = (int64_t*)((uint8_t*)b->arena_long.p + 64);
+    Q.long_cap = b->long_cap;
+    Q.long_min = kLongLineBytes;
+    GJ_HIP(zero_u64(Q.long_count, 2, s));
+  }
   if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->qtune.group, b->query_grid, s));
-  else GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s));
+  else GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s, ctx->num_cus * 2));
   if (e2) GJ_HIP(hipEventRecord(e2, s));
   return GPJSON_OK;
 }
@@ -378,7 +391,7 @@ void gpjson_batch_free(gpjson_batch* b) {
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&b->arena_s, &b->arena_rc, &b->arena_rq, &b->arena_lb, &b->arena_nl, &b->arena_spans,
-                    &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states};
+                    &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states, &b->arena_long};
   for (DevBuf* d : bufs) d->release();
   delete b;
 }

@@ -56,6 +56,15 @@ struct QueryLaunch {
   // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
   // tile prefix); a block stages a tile's leveled and string words in LDS.
   int tiled;
+  // Long lines (line mode): a lane group that claims a line longer than
+  // long_min bytes appends the work item to long_items (up to long_cap items)
+  // instead of walking it alone; a second launch then evaluates each of those
+  // lines with a whole wavefront.  long_items == nullptr: no deferral.
+  // long_count[0]: items appended; long_count[1]: the second launch's claims.
+  int64_t* long_items;
+  unsigned long long* long_count;
+  int64_t long_cap;
+  int64_t long_min;
   // Every job's IR lies in [ir_base, ir_base + ir_bytes).  When ir_bytes fits
   // kIrLdsBytes, each block copies it to LDS once and interprets from there.
   const uint8_t* ir_base;
@@ -63,5 +72,7 @@ struct QueryLaunch {
 };
 
 constexpr int kIrLdsBytes = 4096;
+// Lines at least this long are evaluated by a whole wavefront (line mode).
+constexpr int64_t kLongLineBytes = 8192;
 
 }  // namespace gj

@@ -425,10 +425,23 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
     if ((int64_t)w0 >= total) break;
     const unsigned long long g0 = w0 + (unsigned long long)(lane / G) * lpw;
     if ((int64_t)g0 >= total) continue;
+    // defers a long line to the wavefront-per-line launch (false: list full)
+    auto defer = [&](const QueryJob& J, int64_t lines, int64_t i, int64_t it) -> bool {
+      if (!launch.long_items || G == 64) return false;
+      const int64_t start = J.NL[i], end = (i + 1 < lines) ? J.NL[i + 1] : J.N;
+      if (end - start < launch.long_min) return false;
+      int64_t slot = 0;
+      if (g.r == 0) slot = (int64_t)atomicAdd(&launch.long_count[0], 1ull);
+      slot = g.shfl_i64(slot, 0);
+      if (slot >= launch.long_cap) return false;
+      if (g.r == 0) launch.long_items[slot] = it;
+      return true;
+    };
     if (launch.lines_dev) {
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
         const int64_t q = it / common, i = it - q * common;
         const QueryJob& J = launch.jobs[q];
+        if (defer(J, common, i, it)) continue;
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
         eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
       }
@@ -441,6 +454,52 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
     }
     for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
       while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
+      const QueryJob& J = launch.jobs[lo];
+      const int64_t i = it - J.item0;
+      if (defer(J, J.lines, i, it)) continue;
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+                nslots);
+    }
+  }
+}
+
+// Second launch of line mode: the long lines the first launch deferred, one
+// wavefront per line (64 words per scan step).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, int nslots) {
+  constexpr int G = 64;
+  extern __shared__ int64_t q_lds[];
+  __shared__ uint8_t s_ir[kIrLdsBytes];
+  const Group<G> g;
+  const int grp = threadIdx.x / G;
+  constexpr int NG = BLOCK / G;
+  int64_t* le = q_lds + (int64_t)grp * nslots;
+  int* ci = reinterpret_cast<int*>(q_lds + (int64_t)NG * nslots) + (int64_t)grp * nslots;
+  const bool ir_lds = launch.ir_base && launch.ir_bytes <= kIrLdsBytes;
+  if (ir_lds)
+    for (int k = threadIdx.x; k < launch.ir_bytes; k += BLOCK) s_ir[k] = launch.ir_base[k];
+  __syncthreads();
+  auto ir_of = [&](const QueryJob& J) { return ir_lds ? s_ir + (J.ir - launch.ir_base) : J.ir; };
+  const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
+  int64_t count = (int64_t)launch.long_count[0];
+  if (count > launch.long_cap) count = launch.long_cap;
+  for (;;) {
+    int64_t k = 0;
+    if (g.r == 0) k = (int64_t)atomicAdd(&launch.long_count[1], 1ull);
+    k = g.shfl_i64(k, 0);
+    if (k >= count) break;
+    const int64_t it = launch.long_items[k];
+    if (launch.lines_dev) {
+      const int64_t q = it / common, i = it - q * common;
+      const QueryJob& J = launch.jobs[q];
+      int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
+    } else {
+      int lo = 0, hi = (int)launch.njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (launch.jobs[mid].item0 <= it) lo = mid; else hi = mid - 1;
+      }
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
       eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
@@ -587,15 +646,24 @@ static hipError_t launch_b(const QueryLaunch& L, int group, int nslots, int grid
 }
 
 // levels: the deepest level any job's IR reaches (slots levels+2 per group).
-hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream) {
+// In line mode with a long-line list, a second launch (long_grid blocks of 256
+// threads, a wavefront per line) follows on the same stream.
+hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream,
+                        int long_grid) {
   int nslots = levels + 2;
   if (nslots > 66) nslots = 66;
+  hipError_t e = hipErrorInvalidValue;
   switch (block) {
-    case 64: return launch_b<64>(L, group, nslots, grid, stream);
-    case 128: return launch_b<128>(L, group, nslots, grid, stream);
-    case 256: return launch_b<256>(L, group, nslots, grid, stream);
-    default: return hipErrorInvalidValue;
+    case 64: e = launch_b<64>(L, group, nslots, grid, stream); break;
+    case 128: e = launch_b<128>(L, group, nslots, grid, stream); break;
+    case 256: e = launch_b<256>(L, group, nslots, grid, stream); break;
+    default: break;
   }
+  if (e != hipSuccess || !L.long_items || L.tiled || group == 64) return e;
+  const size_t slots = (size_t)(256 / 64) * nslots * 12 + 8;
+  hipLaunchKernelGGL(query_long_kernel<256>, dim3(long_grid > 0 ? long_grid : 1), dim3(256), slots, stream, L,
+                     nslots);
+  return hipGetLastError();
 }
 
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream) {

@@ -209,12 +209,16 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
                            PreparedQueries* out) {
   int64_t ir_total = 0;
   for (int q = 0; q < nq; q++) ir_total += ir_lens[q];
-  // table layout: [work counter, 64 B][QueryJob x nq][IR bytes]
+  // table layout: [work counter, 64 B][QueryJob x nq][IR bytes][long-line
+  // counters, 64 B][long-line items]
   const size_t jobs_off = 64;
   const size_t jobs_bytes = sizeof(QueryJob) * (size_t)nq;
-  const size_t bytes = jobs_off + jobs_bytes + (size_t)ir_total + 16;
-  // (also waits for the stream, so a table being replaced is no longer read)
-  int rc = ensure_host_scratch(ctx, bytes);
+  const size_t long_off = (jobs_off + jobs_bytes + (size_t)ir_total + 16 + 63) / 64 * 64;
+  const int64_t long_cap = (int64_t)nq * (idx->n / kLongLineBytes + 1);
+  const size_t bytes = long_off + 64 + (size_t)long_cap * 8;
+  // (also waits for the stream, so a table being replaced is no longer read;
+  // only the part before the long-line area is uploaded)
+  int rc = ensure_host_scratch(ctx, long_off);
   if (rc) return rc;
   rc = table->ensure(bytes);
   if (rc) return rc;
@@ -252,7 +256,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     off += ir_lens[q];
     hj[q] = j;
   }
-  GJ_HIP(hipMemcpyAsync(dbase, hbase, bytes, hipMemcpyHostToDevice, ctx->stream));
+  GJ_HIP(hipMemcpyAsync(dbase, hbase, long_off, hipMemcpyHostToDevice, ctx->stream));
   QueryLaunch L{};
   L.jobs = (const QueryJob*)(dbase + jobs_off);
   L.njobs = nq;
@@ -264,6 +268,12 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.tiled = tiled ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
+  if (!tiled && !chain && ctx->qtune.group != 64) {
+    L.long_count = (unsigned long long*)(dbase + long_off);
+    L.long_items = (int64_t*)(dbase + long_off + 64);
+    L.long_cap = long_cap;
+    L.long_min = kLongLineBytes;
+  }
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
   out->chain = chain;
@@ -273,14 +283,16 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   const int waves_per_block = ctx->qtune.block / 64;
   out->grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   if (tiled) out->grid = (int)std::max<int64_t>(1, std::min<int64_t>(L.total_items, (int64_t)ctx->num_cus * 4));
+  out->long_grid = ctx->num_cus * 2;
   return GPJSON_OK;
 }
 
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel, hipEvent_t after_kernel) {
   GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
+  if (pq.launch.long_count) GJ_HIP(zero_u64(pq.launch.long_count, 2, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
   if (pq.chain) GJ_HIP(chain_launch(pq.launch, pq.block, pq.group, pq.grid, ctx->stream));
-  else GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream));
+  else GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream, pq.long_grid));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
   return GPJSON_OK;
 }

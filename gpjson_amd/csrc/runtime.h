@@ -19,7 +19,8 @@ hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, 
 hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
 bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
-hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream);
+hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream,
+                        int long_grid = 0);
 // chain_kernel.hip: member-chain programs, (KEY k DOWN)^D STORE END; the
 // launch's IR must fit kIrLdsBytes.
 int chain_depth(const uint8_t* ir, int64_t len);  // D, or 0 if ir is not a member chain
@@ -126,6 +127,7 @@ namespace gj {
 struct PreparedQueries {
   QueryLaunch launch{};
   bool chain = false;  // run on the member-chain evaluator
+  int long_grid = 0;   // blocks of the long-line launch
   int grid = 0;
   int block = 256;
   int group = 4;

@@ -82,6 +82,7 @@ struct Batch {
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<ServedFile*> files;
   DevBuf spans, jobs;
+  DevBuf longl;  // [2 counters, 64 B][long-line work items]
   void* host_spans = nullptr;  // pinned
   size_t host_cap = 0;
   void* host_jobs = nullptr;   // pinned staging of the job table + IR bytes
@@ -326,8 +327,19 @@ int launch_batch(gpjson_server* s, Batch& b) {
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
+  if (!chain && ctx->qtune.group != 64) {
+    int64_t cap = 0;
+    for (int i = 0; i < n; i++) cap += b.files[i]->n / kLongLineBytes + 1;
+    rc = b.longl.ensure(64 + (size_t)cap * 8);
+    if (rc) return rc;
+    L.long_count = (unsigned long long*)b.longl.p;
+    L.long_items = (int64_t*)((uint8_t*)b.longl.p + 64);
+    L.long_cap = cap;
+    L.long_min = kLongLineBytes;
+    GJ_HIP(zero_u64(L.long_count, 2, ctx->stream));
+  }
   if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, ctx->stream));
-  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream));
+  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream, ctx->num_cus * 2));
   GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, ctx->stream));
   GJ_HIP(hipEventRecord(b.done, ctx->stream));
   return GPJSON_OK;
@@ -621,6 +633,7 @@ void gpjson_server_destroy(gpjson_server* s) {
   for (Batch& b : s->slots) {
     b.spans.release();
     b.jobs.release();
+    b.longl.release();
     if (b.host_spans) (void)hipHostFree(b.host_spans);
     if (b.host_jobs) (void)hipHostFree(b.host_jobs);
     if (b.done) (void)hipEventDestroy(b.done);
