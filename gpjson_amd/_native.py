@@ -29,6 +29,7 @@ GPJSON_INDEX_ESCAPE = 3
 GPJSON_INDEX_QUOTE = 4
 GPJSON_INDEX_COLON = 5
 GPJSON_INDEX_RAW_QUOTE = 6
+GPJSON_INDEX_LEVEL_SUMMARY = 7
 
 GPJSON_BUILD_VERIFY = 1
 GPJSON_QUERY_DISABLE_FALLBACK = 1

@@ -197,7 +197,8 @@ class Context:
 
     def set_config(self, index_block: int, index_wpt: int, query_mode: int, query_block: int, query_group: int,
                    query_waves_per_cu: int, lines_per_group: int) -> None:
-        """query_mode: 0 line (lane group per line), 1 tiled, 2 member-chain evaluator (else line)."""
+        """query_mode: 0 line (lane group per line), 1 tiled, 2 member-chain evaluator (else line),
+        3 line with lines of 8 KB or more handed to a wavefront each."""
         _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_mode, query_block, query_group,
                                            query_waves_per_cu, lines_per_group))
 
@@ -390,6 +391,10 @@ class Index:
     def raw_quote_bitmap(self) -> np.ndarray:
         """Every '"' byte, escaped or not (the interpreter's key-start test)."""
         return self.export(N.GPJSON_INDEX_RAW_QUOTE)
+
+    def level_summary(self) -> np.ndarray:
+        """Per level, a bit per leveled-bitmap word that is not zero (the interpreter's skip table)."""
+        return self.export(N.GPJSON_INDEX_LEVEL_SUMMARY)
 
     def free(self) -> None:
         if self._h and self.ctx._h:

@@ -25,7 +25,7 @@ struct IdxCand {
 const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
 
 struct QCand {
-  int tiled, block, group, wpc, lpw;
+  int mode, block, group, wpc, lpw;
 };
 
 }  // namespace
@@ -55,8 +55,8 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
     return fail(GPJSON_ERR_INVALID_ARG, "query block must be 64, 128 or 256 and group 1, 4, 16 or 64");
   if (query_waves_per_cu < 1 || query_waves_per_cu > 64 || lines_per_wave < 1 || lines_per_wave > 1024)
     return fail(GPJSON_ERR_INVALID_ARG, "query shape out of range");
-  if (query_mode < kLineMode || query_mode > kChainMode)
-    return fail(GPJSON_ERR_INVALID_ARG, "query mode must be 0 (line), 1 (tiled) or 2 (chain)");
+  if (query_mode < kLineMode || query_mode > kLineLongMode)
+    return fail(GPJSON_ERR_INVALID_ARG, "query mode must be 0 (line), 1 (tiled), 2 (chain) or 3 (line, long lines to a wavefront)");
   GJ_HIP(hipSetDevice(ctx->device));
   GJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->itune.block = index_block;
@@ -123,6 +123,9 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   for (int group : {1, 4, 16, 64})
     for (int wpc : {16, 32})
       for (int lpw : {1, 4}) qc.push_back({0, 256, group, wpc, lpw});
+  // long lines to a wavefront each
+  for (int group : {1, 4})
+    for (int wpc : {16, 32}) qc.push_back({3, 256, group, wpc, group == 1 ? 4 : 1});
   // member-chain evaluator (used only when every query is a member chain)
   for (int group : {1, 4, 16})
     for (int wpc : {8, 16, 32}) qc.push_back({2, 256, group, wpc, 1});
@@ -130,7 +133,7 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
   double bestq = 1e30;
   QueryTune best_q = keep_q;
   for (const QCand& c : qc) {
-    ctx->qtune.mode = c.tiled;
+    ctx->qtune.mode = c.mode;
     ctx->qtune.block = c.block;
     ctx->qtune.group = c.group;
     if (c.wpc) ctx->qtune.waves_per_cu = c.wpc;
@@ -151,7 +154,7 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
     if (rc) break;
     double ms = sum / iterations;
     std::snprintf(line, sizeof(line), "query mode=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
-                  c.tiled, c.block, c.group, c.wpc, c.lpw, ms);
+                  c.mode, c.block, c.group, c.wpc, c.lpw, ms);
     rep += line;
     if (ms < bestq) {
       bestq = ms;

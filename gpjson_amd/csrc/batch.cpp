@@ -102,7 +102,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     s_off[f] = s_words;
     s_words += (J.W + 7) / 8 * 8;
     lb_off[f] = lb_words;
-    lb_words += J.Wp * std::max(J.L, 1);
+    lb_words += leveled_alloc_words(J.Wp, J.L);
     nl_off[f] = nl_elems;
     nl_elems += (J.nl_cap + 7) / 8 * 8;
   }
@@ -133,6 +133,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     J.RC = (uint64_t*)b->arena_rc.p + s_off[f];
     J.RQ = (uint64_t*)b->arena_rq.p + s_off[f];
     J.LB = (uint64_t*)b->arena_lb.p + lb_off[f];
+    J.LS = J.LB + J.Wp * std::max(J.L, 1);
+    J.Ws = summary_stride(J.Wp);
     J.NL = (int64_t*)b->arena_nl.p + nl_off[f];
     J.E = nullptr;
     J.Q = nullptr;
@@ -157,6 +159,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     q.lines = 0;
     q.S = J.S;
     q.LB = J.LB;
+    q.LS = J.LS;
+    q.Ws = J.Ws;
     q.RC = J.RC;
     q.RQ = J.RQ;
     q.Wp = J.Wp;
@@ -216,7 +220,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
-  const bool long_lines = !b->chain && b->qtune.group != 64;
+  const bool long_lines = !b->chain && b->qtune.mode == kLineLongMode && b->qtune.group != 64;
   if (long_lines) {
     Q.long_count = (unsigned long long*)b->arena_long.p;
     Q.long_items = (int64_t*)((uint8_t*)b->arena_long.p + 64);

@@ -19,6 +19,17 @@ struct alignas(64) TileState {
   unsigned long long pad[3];
 };
 
+// Level summaries: bit k of summary word s of level l is set iff leveled word
+// 64*s + k of level l is not zero, so a scan over a level can skip 64 words
+// (4 KB of input) with one load where the level has no structure.  They follow
+// the L levels in the same allocation, Ws words per level.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline int64_t summary_stride(int64_t Wp) { return (Wp + 63) / 64; }
+// Words of a leveled-bitmap allocation (levels, then their summaries).
+inline int64_t leveled_alloc_words(int64_t Wp, int L) { return (Wp + summary_stride(Wp)) * (L > 1 ? L : 1); }
+
 // One file of an index launch.  The plain path launches with one file.
 struct FileJob {
   const uint8_t* F;     // input bytes
@@ -27,6 +38,8 @@ struct FileJob {
   int64_t Wp;           // level stride (W rounded up to 8)
   uint64_t* S;          // string index, W words
   uint64_t* LB;         // leveled bitmaps, L * Wp words
+  uint64_t* LS;         // level summaries, L * Ws words (may be null)
+  int64_t Ws;           // summary stride
   int64_t* NL;          // newline index, capacity nl_cap
   int64_t nl_cap;
   uint64_t* E;          // escape index (verify mode, may be null)

@@ -7,6 +7,7 @@
 //   newline index   int64[lines]   ([0] = 0, then every '\n')
 //   string index    uint64[W]      (in-string bits)
 //   leveled bitmaps uint64[L][Wp]  (structural chars per depth level)
+//   level summaries uint64[L][Ws]  (a bit per leveled word that is not zero)
 //   colon / quote   uint64[W]      (raw ':' and '"' bytes, for the interpreter)
 //   escape / quote  uint64[W]      (only in verify mode)
 //
@@ -288,6 +289,18 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
 #pragma unroll
     for (int i = 0; i < NLV; i++)
       if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+    // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
+    // start at a multiple of 64 (lanes past the file end are inactive, and
+    // lane 0 holds the lowest word, so it is active whenever any lane is)
+    if (J.LS) {
+#pragma unroll
+      for (int i = 0; i < NLV; i++) {
+        if (lo + i < J.L) {
+          const unsigned long long m = __ballot(out[i] != 0);
+          if ((tid & 63) == 0) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
+        }
+      }
+    }
   }
 }
 
@@ -295,6 +308,7 @@ template <int BLOCK, int ROWS, int NLV, bool VERIFY>
 __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
+  static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
 
   __shared__ unsigned s_ticket;
   __shared__ int s_job;

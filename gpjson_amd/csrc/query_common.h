@@ -26,6 +26,8 @@ struct QueryJob {
   const uint64_t* LB; // leveled bitmaps, level stride Wp
   const uint64_t* RC; // raw ':' bytes (bit per input byte)
   const uint64_t* RQ; // raw '"' bytes (bit per input byte)
+  const uint64_t* LS; // level summaries (bit per leveled word), stride Ws; may be null
+  int64_t Ws;
   int64_t Wp;
   int L;              // levels present in LB
   int R;              // result slots per line

@@ -84,8 +84,12 @@ struct GlobalWords {
   const uint64_t* LB;
   int64_t Wp;
   const uint64_t* S;
+  const uint64_t* LS;  // level summaries (may be null: no skipping)
+  int64_t Ws;
   __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return LB[(int64_t)lvl * Wp + wi]; }
   __device__ __forceinline__ uint64_t s(int64_t wi) const { return S[wi]; }
+  __device__ __forceinline__ bool has_summary() const { return LS != nullptr; }
+  __device__ __forceinline__ uint64_t ls(int lvl, int64_t si) const { return LS[(int64_t)lvl * Ws + si]; }
 };
 
 struct LdsWords {
@@ -95,7 +99,33 @@ struct LdsWords {
   int L;
   __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return w[lvl * maxw + (int)(wi - wlo)]; }
   __device__ __forceinline__ uint64_t s(int64_t wi) const { return w[L * maxw + (int)(wi - wlo)]; }
+  __device__ __forceinline__ bool has_summary() const { return false; }
+  __device__ __forceinline__ uint64_t ls(int, int64_t) const { return ~0ull; }
 };
+
+// Summary words of level lvl for the blocks of 64 words that hold word nb and
+// the one after, loaded together with a scan step's words (so they cost no
+// extra round trip); blocks past wend read as empty.
+template <class Words>
+__device__ __forceinline__ void load_summary(const Words& W, int lvl, int64_t nb, int64_t wend, uint64_t& sm0,
+                                             uint64_t& sm1) {
+  sm0 = 0;
+  sm1 = 0;
+  if (!W.has_summary()) return;
+  if (nb <= wend) sm0 = W.ls(lvl, nb >> 6);
+  if ((nb >> 6) + 1 <= (wend >> 6)) sm1 = W.ls(lvl, (nb >> 6) + 1);
+}
+
+// The next word at or after nb whose level word may have a bit: words the
+// summaries show empty are skipped (up to the end of the block after nb's).
+template <class Words>
+__device__ __forceinline__ int64_t skip_empty(const Words& W, int64_t nb, uint64_t sm0, uint64_t sm1) {
+  if (!W.has_summary()) return nb;
+  const uint64_t rest = sm0 & mask_from((int)(nb & 63));
+  if (rest) return (nb & ~63ll) + ctz64(rest);
+  const int64_t next = (nb | 63) + 1;
+  return sm1 ? next + ctz64(sm1) : next + 64;
+}
 
 // Word wi of level lvl restricted to positions [from, to).
 template <class Words>
@@ -128,17 +158,20 @@ template <int G, class Words>
 __device__ int64_t group_first_bit(const Group<G>& g, const Words& W, int lvl, int64_t from, int64_t to) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kBitWords) {
+  for (int64_t b0 = wbeg; b0 <= wend;) {
     const int64_t w0 = b0 + (int64_t)g.r * kBitWords;
-    uint64_t v[kBitWords];
+    const int64_t nb = b0 + (int64_t)G * kBitWords;
+    uint64_t v[kBitWords], sm0, sm1;
 #pragma unroll
     for (int u = 0; u < kBitWords; u++) v[u] = window_word(W, lvl, w0 + u, wbeg, wend, from, to);
+    load_summary(W, lvl, nb, wend, sm0, sm1);
     int64_t found = -1;
 #pragma unroll
     for (int u = kBitWords - 1; u >= 0; u--)
       if (v[u]) found = ((w0 + u) << 6) + ctz64(v[u]);
     const unsigned long long m = g.ballot(found >= 0);
     if (m) return g.shfl_i64(found, ctz64(m));
+    b0 = skip_empty(W, nb, sm0, sm1);
   }
   return -1;
 }
@@ -192,9 +225,10 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
                                   int klen) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += (int64_t)G * kKeyWords) {
+  for (int64_t b0 = wbeg; b0 <= wend;) {
     const int64_t w0 = b0 + (int64_t)g.r * kKeyWords;
-    uint64_t v[kKeyWords], sw[kKeyWords], qw[kKeyWords];
+    const int64_t nb = b0 + (int64_t)G * kKeyWords;
+    uint64_t v[kKeyWords], sw[kKeyWords], qw[kKeyWords], sm0, sm1;
 #pragma unroll
     for (int u = 0; u < kKeyWords; u++) {
       const int64_t wi = w0 + u;
@@ -207,6 +241,7 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
         qw[u] = RQ[wi];
       }
     }
+    load_summary(W, lvl, nb, wend, sm0, sm1);
     int64_t found = -1;
 #pragma unroll
     for (int u = 0; u < kKeyWords; u++) {
@@ -219,28 +254,71 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
     }
     const unsigned long long m = g.ballot(found >= 0);
     if (m) return g.shfl_i64(found, ctz64(m));
+    b0 = skip_empty(W, nb, sm0, sm1);
   }
   return -1;
 }
 
+// Bit per byte of the 64-byte word wi of F that equals ',' (byte-parallel
+// compare of 16-byte loads; the file's last partial word byte by byte).
+__device__ __forceinline__ uint64_t comma_word(const uint8_t* F, int64_t n, int64_t wi) {
+  const int64_t off = wi << 6;
+  uint32_t d[16];
+  if (off + 64 <= n) {
+    const uint4* src = reinterpret_cast<const uint4*>(F + off);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 x = src[q];
+      d[4 * q] = x.x;
+      d[4 * q + 1] = x.y;
+      d[4 * q + 2] = x.z;
+      d[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) d[k] = 0;
+    for (int b = 0; off + b < n && b < 64; b++) d[b >> 2] |= (uint32_t)F[off + b] << (8 * (b & 3));
+  }
+  uint64_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t t = d[k] ^ 0x2C2C2C2Cu;
+    // bit 7 of a byte: that byte of t is zero (no carry crosses bytes)
+    const uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
+    // gather the four bit-7 flags into bits 21..24 (the partial products do
+    // not overlap, so the multiply adds no carries)
+    const uint32_t four = (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
+    m |= (uint64_t)four << (4 * k);
+  }
+  return m;
+}
+
 // Walks structural ',' in [from, to), counting from *count; returns the
 // position of the comma at which the count becomes want, or -1 (count advanced
-// by every comma in range).
+// by every comma in range).  A lane tests kBitWords words per step: their
+// level words and text are loaded together, and the text is compared whole.
 template <int G, class Words>
-__device__ int64_t group_find_comma(const Group<G>& g, const Words& W, int lvl, const uint8_t* F, int64_t from,
-                                    int64_t to, int* count, int want) {
+__device__ int64_t group_find_comma(const Group<G>& g, const Words& W, int lvl, const uint8_t* F, int64_t n,
+                                    int64_t from, int64_t to, int* count, int want) {
   if (from >= to) return -1;
   const int64_t wbeg = from >> 6, wend = (to - 1) >> 6;
   int cnt0 = *count;
-  for (int64_t b0 = wbeg; b0 <= wend; b0 += G) {
-    const int64_t wi = b0 + g.r;
-    const uint64_t v = window_word(W, lvl, wi, wbeg, wend, from, to);
-    uint64_t commas = 0;
-    for (uint64_t t = v; t; t &= t - 1) {
-      const int b = ctz64(t);
-      if (F[(wi << 6) + b] == ',') commas |= 1ull << b;
+  for (int64_t b0 = wbeg; b0 <= wend;) {
+    const int64_t w0 = b0 + (int64_t)g.r * kBitWords;
+    const int64_t nb = b0 + (int64_t)G * kBitWords;
+    uint64_t v[kBitWords], fm[kBitWords], sm0, sm1;
+#pragma unroll
+    for (int u = 0; u < kBitWords; u++) {
+      v[u] = window_word(W, lvl, w0 + u, wbeg, wend, from, to);
+      fm[u] = w0 + u <= wend ? comma_word(F, n, w0 + u) : 0;
     }
-    const int c = __popcll(commas);
+    load_summary(W, lvl, nb, wend, sm0, sm1);
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < kBitWords; u++) {
+      v[u] &= fm[u];
+      c += __popcll(v[u]);
+    }
     const int incl = g.scan_add(c);
     const int total = g.shfl(incl, G - 1);
     const int need = want - cnt0;  // 1-based rank of the event comma in this window
@@ -250,14 +328,23 @@ __device__ int64_t group_find_comma(const Group<G>& g, const Words& W, int lvl, 
       const int l = ctz64(m);
       int64_t pos = -1;
       if (g.r == l) {
-        uint64_t cm = commas;
-        for (int k = 1; k < need - (incl - c); k++) cm &= cm - 1;
-        pos = (wi << 6) + ctz64(cm);
+        int k = need - (incl - c);  // 1-based rank within this lane's words
+#pragma unroll
+        for (int u = 0; u < kBitWords; u++) {
+          const int cu = __popcll(v[u]);
+          if (pos < 0 && k <= cu) {
+            uint64_t cm = v[u];
+            for (int t = 1; t < k; t++) cm &= cm - 1;
+            pos = ((w0 + u) << 6) + ctz64(cm);
+          }
+          k -= cu;
+        }
       }
       *count = want;
       return g.shfl_i64(pos, l);
     }
     cnt0 += total;
+    b0 = skip_empty(W, nb, sm0, sm1);
   }
   *count = cnt0;
   return -1;
@@ -377,7 +464,7 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
         p = group_find_key(g, W, J.RC, J.RQ, lvl, F, start, j, to, want, want_len);
       } else if (op == kMoveToIndex) {
         int cnt = ci[lvl];
-        p = group_find_comma(g, W, lvl, F, j, to, &cnt, want_idx);
+        p = group_find_comma(g, W, lvl, F, n, j, to, &cnt, want_idx);
         if (lead) ci[lvl] = cnt;
       }
     }
@@ -443,7 +530,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
         const QueryJob& J = launch.jobs[q];
         if (defer(J, common, i, it)) continue;
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
       }
       continue;
     }
@@ -457,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
       if (defer(J, J.lines, i, it)) continue;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
                 nslots);
     }
   }
@@ -493,7 +580,7 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
       const int64_t q = it / common, i = it - q * common;
       const QueryJob& J = launch.jobs[q];
       int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), common, i, out, le, ci, nslots);
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
     } else {
       int lo = 0, hi = (int)launch.njobs - 1;
       while (lo < hi) {
@@ -502,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
       }
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
                 nslots);
     }
   }
@@ -561,7 +648,7 @@ __global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, 
       }
       __syncthreads();
       const LdsWords LW{words, wlo, maxw, J.L};
-      const GlobalWords GW{J.LB, J.Wp, J.S};
+      const GlobalWords GW{J.LB, J.Wp, J.S, J.LS, J.Ws};
       for (int64_t i = la + grp; i < lb; i += NG) {
         int64_t* out = J.out + i * (int64_t)J.R * 2;
         if (staged) eval_line(g, J, LW, ir, lines, i, out, le, ci, nslots);

@@ -176,6 +176,8 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   j.Wp = idx->Wp;
   j.S = idx->S;
   j.LB = idx->LB;
+  j.LS = idx->LS;
+  j.Ws = idx->Ws;
   j.NL = idx->NL;
   j.nl_cap = idx->nl_cap;
   j.E = idx->E;
@@ -240,6 +242,8 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.lines = tiled ? idx->lines : 0;
     j.S = idx->S;
     j.LB = idx->LB;
+    j.LS = idx->LS;
+    j.Ws = idx->Ws;
     j.RC = idx->RC;
     j.RQ = idx->RQ;
     j.Wp = idx->Wp;
@@ -268,7 +272,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.tiled = tiled ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
-  if (!tiled && !chain && ctx->qtune.group != 64) {
+  if (!tiled && !chain && ctx->qtune.mode == kLineLongMode && ctx->qtune.group != 64) {
     L.long_count = (unsigned long long*)(dbase + long_off);
     L.long_items = (int64_t*)(dbase + long_off + 64);
     L.long_cap = long_cap;
@@ -319,6 +323,7 @@ static void free_index_arrays(gpjson_index* idx) {
   void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->RC, idx->RQ, idx->lines_dev, idx->tile_lines};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  idx->LS = nullptr;
   idx->S = idx->LB = idx->E = idx->Q = idx->RC = idx->RQ = nullptr;
   idx->NL = idx->lines_dev = idx->tile_lines = nullptr;
 }
@@ -526,7 +531,9 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
   alloc((void**)&idx->S, (size_t)idx->W * 8);
   alloc((void**)&idx->RC, (size_t)idx->W * 8);
   alloc((void**)&idx->RQ, (size_t)idx->W * 8);
-  alloc((void**)&idx->LB, (size_t)idx->Wp * std::max(levels, 1) * 8);
+  alloc((void**)&idx->LB, (size_t)leveled_alloc_words(idx->Wp, levels) * 8);
+  idx->Ws = summary_stride(idx->Wp);
+  if (idx->LB) idx->LS = idx->LB + idx->Wp * std::max(levels, 1);
   alloc((void**)&idx->NL, (size_t)idx->nl_cap * 8);
   alloc((void**)&idx->lines_dev, 64);
   // one entry per tile of the smallest supported tile size, plus the total
@@ -568,6 +575,7 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
     case GPJSON_INDEX_QUOTE: size = idx->W * 8; src = idx->Q; break;
     case GPJSON_INDEX_COLON: size = idx->W * 8; src = idx->RC; break;
     case GPJSON_INDEX_RAW_QUOTE: size = idx->W * 8; src = idx->RQ; break;
+    case GPJSON_INDEX_LEVEL_SUMMARY: size = (idx->W + 63) / 64 * 8 * idx->L; src = idx->LS; break;
     default: return fail(GPJSON_ERR_INVALID_ARG, "unknown index array " + std::to_string(which));
   }
   if (size_bytes) *size_bytes = size;
@@ -578,6 +586,9 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
   if (which == GPJSON_INDEX_LEVELED) {
     GJ_HIP(hipMemcpy2D(dst, (size_t)idx->W * 8, src, (size_t)idx->Wp * 8, (size_t)idx->W * 8, (size_t)idx->L,
                        hipMemcpyDeviceToHost));
+  } else if (which == GPJSON_INDEX_LEVEL_SUMMARY) {
+    const size_t row = (size_t)(idx->W + 63) / 64 * 8;
+    GJ_HIP(hipMemcpy2D(dst, row, src, (size_t)idx->Ws * 8, row, (size_t)idx->L, hipMemcpyDeviceToHost));
   } else {
     GJ_HIP(hipMemcpy(dst, src, (size_t)size, hipMemcpyDeviceToHost));
   }

@@ -48,8 +48,9 @@ struct IndexTune {
 // Interpreter modes: one lane group per line (the general interpreter); tiled
 // (a block stages one index tile's words in LDS, levels <= 4); chain (the
 // member-chain evaluator when every program of a launch is a member chain,
-// else line mode).
-enum QueryMode { kLineMode = 0, kTiledMode = 1, kChainMode = 2 };
+// else line mode); line-long (line mode, and lines of kLongLineBytes or more
+// go to a second launch with a wavefront per line).
+enum QueryMode { kLineMode = 0, kTiledMode = 1, kChainMode = 2, kLineLongMode = 3 };
 
 // Defaults: the shape the autotuner measured fastest on 4 GiB of tweets for
 // $.user.lang: line mode, one lane per line, 4 lines per claim (2.86 ms; 3.31 ms
@@ -108,6 +109,8 @@ struct gpjson_index {
   int flags = 0;
   uint64_t* S = nullptr;
   uint64_t* LB = nullptr;
+  uint64_t* LS = nullptr;  // level summaries, inside the LB allocation
+  int64_t Ws = 0;
   int64_t* NL = nullptr;
   int64_t nl_cap = 0;
   uint64_t* E = nullptr;

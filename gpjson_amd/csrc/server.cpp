@@ -82,7 +82,6 @@ struct Batch {
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<ServedFile*> files;
   DevBuf spans, jobs;
-  DevBuf longl;  // [2 counters, 64 B][long-line work items]
   void* host_spans = nullptr;  // pinned
   size_t host_cap = 0;
   void* host_jobs = nullptr;   // pinned staging of the job table + IR bytes
@@ -125,7 +124,7 @@ namespace {
 int64_t index_bytes(int64_t n, int levels) {
   const int64_t W = (n + 63) / 64, Wp = (W + 7) / 8 * 8;
   // string, raw-colon and raw-quote words, leveled words, newline and tile tables
-  return 3 * W * 8 + Wp * std::max(levels, 1) * 8 + (n / 256 + 4096) * 8 + (n / 16384 + 2) * 8 + 64;
+  return 3 * W * 8 + leveled_alloc_words(Wp, levels) * 8 + (n / 256 + 4096) * 8 + (n / 16384 + 2) * 8 + 64;
 }
 
 // Frees LRU resident files (not used by in-flight batches, not being admitted)
@@ -285,6 +284,8 @@ int launch_batch(gpjson_server* s, Batch& b) {
     j.lines = idx->lines;
     j.S = idx->S;
     j.LB = idx->LB;
+    j.LS = idx->LS;
+    j.Ws = idx->Ws;
     j.RC = idx->RC;
     j.RQ = idx->RQ;
     j.Wp = idx->Wp;
@@ -327,19 +328,8 @@ int launch_batch(gpjson_server* s, Batch& b) {
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
-  if (!chain && ctx->qtune.group != 64) {
-    int64_t cap = 0;
-    for (int i = 0; i < n; i++) cap += b.files[i]->n / kLongLineBytes + 1;
-    rc = b.longl.ensure(64 + (size_t)cap * 8);
-    if (rc) return rc;
-    L.long_count = (unsigned long long*)b.longl.p;
-    L.long_items = (int64_t*)((uint8_t*)b.longl.p + 64);
-    L.long_cap = cap;
-    L.long_min = kLongLineBytes;
-    GJ_HIP(zero_u64(L.long_count, 2, ctx->stream));
-  }
   if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, ctx->stream));
-  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream, ctx->num_cus * 2));
+  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream));
   GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, ctx->stream));
   GJ_HIP(hipEventRecord(b.done, ctx->stream));
   return GPJSON_OK;
@@ -633,7 +623,6 @@ void gpjson_server_destroy(gpjson_server* s) {
   for (Batch& b : s->slots) {
     b.spans.release();
     b.jobs.release();
-    b.longl.release();
     if (b.host_spans) (void)hipHostFree(b.host_spans);
     if (b.host_jobs) (void)hipHostFree(b.host_jobs);
     if (b.done) (void)hipEventDestroy(b.done);

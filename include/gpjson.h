@@ -51,7 +51,11 @@ enum {
   GPJSON_INDEX_ESCAPE = 3,  /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
   GPJSON_INDEX_QUOTE = 4,   /* uint64[ceil(n/64)], only with GPJSON_BUILD_VERIFY */
   GPJSON_INDEX_COLON = 5,   /* uint64[ceil(n/64)]: every ':' byte (no reference counterpart) */
-  GPJSON_INDEX_RAW_QUOTE = 6 /* uint64[ceil(n/64)]: every '"' byte, escaped or not (no reference counterpart) */
+  GPJSON_INDEX_RAW_QUOTE = 6, /* uint64[ceil(n/64)]: every '"' byte, escaped or not (no reference counterpart) */
+  /* uint64[L][ceil(n/4096)]: bit k of word s of level l is set iff word 64*s+k
+     of leveled bitmap l is not zero (no reference counterpart; lets the
+     interpreter skip 4 KB without structure at a level with one load) */
+  GPJSON_INDEX_LEVEL_SUMMARY = 7
 };
 
 /* Flags of gpjson_index_build. */
@@ -238,9 +242,11 @@ void gpjson_server_destroy(gpjson_server *server);
  * index_wpt 64-byte words; interpreter mode (query_mode: 0 = one lane group
  * per line; 1 = tiled, a block stages one index tile's leveled and string words
  * in LDS, up to 4 levels; 2 = member-chain evaluator when every query of a
- * launch is a member chain $.a.b..., else mode 0), block size, lanes per line
- * (query_group: 1, 4, 16 or 64; modes 0 and 1), persistent grid in wavefronts
- * per CU and lines a lane group claims at a time (mode 0).  The reference
+ * launch is a member chain $.a.b..., else mode 0; 3 = mode 0, and lines of
+ * 8 KB or more are handed to a second launch that gives each a wavefront),
+ * block size, lanes per line (query_group: 1, 4, 16 or 64; modes 0, 1 and 3),
+ * persistent grid in wavefronts per CU and lines a lane group claims at a time
+ * (modes 0 and 3).  The reference
  * hard-codes 8x1024 (stages/CombinedIndex.java:11-14) and 512x1024 threads with
  * one thread per line (functions/QueryFunction.java:229). */
 int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *query_mode, int *query_block,

@@ -26,7 +26,18 @@ def _files(seed, nfiles):
     return out
 
 
-def test_batch_equals_plain(ctx):
+@pytest.mark.parametrize("mode", [0, 3])
+def test_batch_equals_plain(ctx, mode):
+    cfg = ctx.get_config()
+    try:
+        ctx.set_config(cfg["index_block"], cfg["index_wpt"], mode, cfg["query_block"], cfg["query_group"],
+                       cfg["query_waves_per_cu"], cfg["lines_per_group"])
+        _batch_equals_plain(ctx)
+    finally:
+        ctx.set_config(*cfg.values())
+
+
+def _batch_equals_plain(ctx):
     from gpjson_amd import compile_path
     files = _files(7, 12)
     bufs = [ctx.stage(d) for _, d in files]

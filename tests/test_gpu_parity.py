@@ -33,6 +33,16 @@ def byte_bitmap(data: bytes, c: int) -> np.ndarray:
     return np.packbits(bits, bitorder="little").view("<u8").astype(np.uint64) if W else np.zeros(0, np.uint64)
 
 
+def level_summary(lb: np.ndarray, levels: int, n: int) -> np.ndarray:
+    """Per level, bit k of word s set iff leveled word 64*s+k is not zero (closed form)."""
+    W = (n + 63) // 64
+    Ws = (W + 63) // 64
+    nz = np.zeros((levels, Ws * 64), dtype=np.uint8)
+    if W:
+        nz[:, :W] = lb.reshape(levels, W) != 0
+    return np.packbits(nz, axis=1, bitorder="little").view("<u8").astype(np.uint64).reshape(-1)
+
+
 def check_index(ctx, data: bytes, levels: int, tag: str = ""):
     _p(f"[parity] index n={len(data)} L={levels} {tag}")
     buf = ctx.stage(data)
@@ -53,6 +63,7 @@ def check_index(ctx, data: bytes, levels: int, tag: str = ""):
         g2, l2 = got.reshape(levels, W), lb.reshape(levels, W)
         bad = np.argwhere(g2 != l2)[:5]
         raise AssertionError(f"leveled bitmaps differ at (level, word) {bad.tolist()}")
+    assert np.array_equal(idx.level_summary(), level_summary(lb, levels, len(data))), "level summaries"
     return buf, idx, nl, s, lb
 
 

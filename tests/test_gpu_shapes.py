@@ -1,6 +1,7 @@
 """Every launch shape the autotuner may pick gives the oracle's bit-exact answers:
 all index tile shapes, and the interpreter in tiled and line mode with every
-lane-group size, and the member-chain evaluator."""
+lane-group size, with and without the long-line launch, and the member-chain
+evaluator."""
 import numpy as np
 import pytest
 
@@ -9,9 +10,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 INDEX_SHAPES = [(256, 1), (256, 2), (256, 4), (512, 1), (512, 2), (128, 2), (128, 4)]
-# (mode, block, group); mode 2 with these mixed query sets runs in line mode
+# (mode, block, group); mode 2 with these mixed query sets runs in line mode;
+# mode 3 hands the adversarial input's lines (4-64 KB) to the long-line launch
 QUERY_SHAPES = [(1, 256, 1), (1, 256, 4), (1, 128, 16), (1, 256, 64), (0, 256, 1), (0, 256, 4), (0, 64, 16),
-                (0, 256, 64), (2, 256, 1)]
+                (0, 256, 64), (2, 256, 1), (3, 256, 1), (3, 256, 4), (3, 128, 16)]
 
 
 @pytest.fixture(scope="module")
