@@ -22,7 +22,7 @@ namespace {
 struct IdxCand {
   int block, wpt;
 };
-const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
+const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 3}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
 
 struct QCand {
   int mode, block, group, wpc, lpw;

@@ -601,7 +601,7 @@ int64_t index_tile_bytes(int block, int wpt) { return (int64_t)block * wpt * 64;
 // smallest supported tile: 128 x 2 words = 16 KiB (same as 256 x 1)
 
 bool index_config_supported(int block, int wpt) {
-  return (block == 256 && (wpt == 1 || wpt == 2 || wpt == 4)) || (block == 512 && (wpt == 1 || wpt == 2)) ||
+  return (block == 256 && (wpt >= 1 && wpt <= 4)) || (block == 512 && (wpt == 1 || wpt == 2)) ||
          (block == 128 && (wpt == 2 || wpt == 4));
 }
 
@@ -609,6 +609,7 @@ hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool 
                         hipStream_t stream) {
   if (block == 256 && wpt == 1) return launch_nlv<256, 1>(L, nlv, verify, grid, stream);
   if (block == 256 && wpt == 2) return launch_nlv<256, 2>(L, nlv, verify, grid, stream);
+  if (block == 256 && wpt == 3) return launch_nlv<256, 3>(L, nlv, verify, grid, stream);
   if (block == 256 && wpt == 4) return launch_nlv<256, 4>(L, nlv, verify, grid, stream);
   if (block == 512 && wpt == 1) return launch_nlv<512, 1>(L, nlv, verify, grid, stream);
   if (block == 512 && wpt == 2) return launch_nlv<512, 2>(L, nlv, verify, grid, stream);

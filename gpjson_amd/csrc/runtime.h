@@ -40,9 +40,13 @@ int hip_fail(hipError_t e, const char* what);
     if (gj_e_ != hipSuccess) return hip_fail(gj_e_, #expr); \
   } while (0)
 
+// Default index tile: 256 threads x 3 rows (48 KiB).  On 4 GiB of tweets it
+// measured 4.39 / 4.26 / 4.42 / 4.65 ms for L = 2, 1, 2, 3 against 4.48 / 4.28 /
+// 4.47 / 4.89 ms with 4 rows (2 blocks per CU by LDS and 187 VGPRs; 3 rows fit
+// 3 blocks) and 5.1-5.5 ms with 2 rows.
 struct IndexTune {
   int block = 256;
-  int wpt = 4;
+  int wpt = 3;
 };
 
 // Interpreter modes: one lane group per line (the general interpreter); tiled
