@@ -69,9 +69,32 @@ __device__ __forceinline__ M4 m_left_div(const M4& x, const M4& z) {
 // backslash run, when the byte after that run is a quote.
 __device__ __forceinline__ M4 m_flip(const M4& x) { return M4{x.q ^ 1, x.tot - x.d0, x.tot, x.nl}; }
 
-__device__ __forceinline__ M4 shfl_up_m(const M4& v, int off) {
-  return M4{__shfl_up(v.q, off), __shfl_up(v.d0, off), __shfl_up(v.tot, off), __shfl_up(v.nl, off)};
+// DPP lane moves of an M4 (GFX9 controls: row_shr:n = 0x110 + n, row_bcast:15 =
+// 0x142, row_bcast:31 = 0x143, wave_shr:1 = 0x138).  Lanes with no source lane,
+// or in rows ROWMASK leaves out, get 0, which is the identity m_id().
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ M4 dpp_m(const M4& v) {
+  return M4{__builtin_amdgcn_update_dpp(0, v.q, CTRL, ROWMASK, 0xF, false),
+            __builtin_amdgcn_update_dpp(0, v.d0, CTRL, ROWMASK, 0xF, false),
+            __builtin_amdgcn_update_dpp(0, v.tot, CTRL, ROWMASK, 0xF, false),
+            __builtin_amdgcn_update_dpp(0, v.nl, CTRL, ROWMASK, 0xF, false)};
 }
+
+// Inclusive scan of M4 over the wavefront in lane order: prefixes within rows
+// of 16 lanes, then row 0's total into row 1 and row 2's into row 3, then rows
+// 0-1's total into rows 2-3.  VALU lane moves, no LDS traffic.
+__device__ __forceinline__ M4 wave_scan_m(M4 x) {
+  x = m_cat(dpp_m<0x111, 0xF>(x), x);
+  x = m_cat(dpp_m<0x112, 0xF>(x), x);
+  x = m_cat(dpp_m<0x114, 0xF>(x), x);
+  x = m_cat(dpp_m<0x118, 0xF>(x), x);
+  x = m_cat(dpp_m<0x142, 0xA>(x), x);
+  x = m_cat(dpp_m<0x143, 0xC>(x), x);
+  return x;
+}
+
+// The value of the lane below (lane 0: the identity).
+__device__ __forceinline__ M4 lane_below_m(const M4& x) { return dpp_m<0x138, 0xF>(x); }
 
 // Carry state at a position.
 struct Carry {
@@ -207,8 +230,10 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
     TileAgg a = agg_id();
     Carry c{0, 0, 0, 0};
     if (k >= 0) {
+      // all five words in one round trip (each validates itself)
       TileState* s = st0 + k;
       const unsigned long long i0 = ld_state(&s->inc0), i1 = ld_state(&s->inc1);
+      const unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
       if ((i0 & kValid) && (i1 & kValid)) {
         status = 2;
         c.p = (int)((i0 >> 62) & 1);
@@ -216,7 +241,6 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
         c.d = sext(i0 & 0xFFFFFFFFFFFFull, 48);
         c.lines = (int64_t)(i1 & ~kValid);
       } else {
-        const unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
         if ((a0 & kValid) && (a1 & kValid) && (a2 & kValid)) {
           status = 1;
           a.esc = ((a0 >> 62) & 1) ? 0 : (2 + (int)((a0 >> 61) & 1));
@@ -255,28 +279,39 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
 
 }  // namespace
 
+// Per-word masks parked in LDS between classification and emission: the
+// in-string mask for escape carry 0 outside a string, newlines, open, close,
+// ':' ',', quotes and ':'; verify mode also keeps the backslashes.
+constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotK = 6, kSlotB = 7;
+
 // The index words of word w -- string index, raw colon and quote bitmaps,
 // verify-mode escape/quote words, leveled bitmaps -- from its masks (row r,
 // lane tid of `wc`), the escape carry p, string state s and depth d before its
 // first byte.
-template <int NLV, bool VERIFY, int ROWS, int BLOCK>
-__device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uint64_t (&wc)[ROWS][7][BLOCK], int r,
-                                          int tid, int p, int s, int64_t d) {
+template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK>
+__device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uint64_t (&wc)[ROWS][NSLOT][BLOCK],
+                                          const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
+                                          int64_t d) {
   uint64_t valid = ~0ull;
   const int64_t rem = J.N - w * 64;
   if (rem < 64) valid = (1ull << rem) - 1ull;
-  const uint64_t Bm = wc[r][0][tid], Qm = wc[r][1][tid];
-  const uint64_t E = escape_word(Bm, (uint32_t)p);
-  const uint64_t Qe = Qm & ~E;
-  const uint64_t S = (prefix_xor(Qe) ^ (s ? ~0ull : 0ull)) & valid;
+  // in-string mask from the one for escape carry 0 outside a string: carry 1
+  // escapes the leading backslash run's first byte instead of its second, which
+  // toggles only a quote right after the run, and so every bit from there on
+  const int ld = lead[r][tid];
+  uint64_t S = wc[r][kSlotS0][tid];
+  if (p && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
+  S = (s ? ~S : S) & valid;
   J.S[w] = S;
-  if (J.RC) J.RC[w] = wc[r][6][tid] & valid;
-  if (J.RQ) J.RQ[w] = Qm & valid;
+  // (bytes past the end of the file were loaded as zeros, which match neither)
+  if (J.RC) J.RC[w] = wc[r][kSlotK][tid];
+  if (J.RQ) J.RQ[w] = wc[r][kSlotQ][tid];
   if (VERIFY) {
+    const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
     if (J.E) J.E[w] = E & valid;
-    if (J.Q) J.Q[w] = Qe & valid;
+    if (J.Q) J.Q[w] = wc[r][kSlotQ][tid] & ~E & valid;
   }
-  const uint64_t O = wc[r][3][tid] & ~S, C = wc[r][4][tid] & ~S, P = wc[r][5][tid] & ~S;
+  const uint64_t O = wc[r][kSlotO][tid] & ~S, C = wc[r][kSlotC][tid] & ~S, P = wc[r][kSlotP][tid] & ~S;
   const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
   for (int lo = 0; lo < J.L; lo += NLV) {
     uint64_t out[NLV];
@@ -318,10 +353,13 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   __shared__ int s_w0_flip;
   __shared__ M4 s_w0_m;
   __shared__ Carry s_carry;
-  // the seven masks of every word, parked here (field-major, so the lanes of a
-  // wavefront touch consecutive 8-byte slots) while the scans and the
-  // lookback run, so they occupy no registers there
-  __shared__ uint64_t s_wc[ROWS][7][BLOCK];
+  // the masks every word needs at emission, parked here (field-major, so the
+  // lanes of a wavefront touch consecutive 8-byte slots) while the scans and
+  // the lookback run, so they occupy no registers there; s_lead: length of the
+  // word's leading backslash run (64: all backslashes) | 0x80 if a quote follows it
+  constexpr int NSLOT = VERIFY ? 8 : 7;
+  __shared__ uint64_t s_wc[ROWS][NSLOT][BLOCK];
+  __shared__ uint8_t s_lead[ROWS][BLOCK];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -388,24 +426,24 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
         const WordClass w = classify64(d);
         code[r] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
         flip[r] = code[r] ? (int)((w.Q >> ctz64(~w.B)) & 1) : 0;
-        {
-          const uint64_t E = escape_word(w.B, 0u);
-          const uint64_t Qe = w.Q & ~E;
-          const uint64_t S0 = prefix_xor(Qe);
-          M4 m;
-          m.q = popc64(Qe) & 1;
-          m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
-          m.tot = popc64(w.O) - popc64(w.C);
-          m.nl = popc64(w.NL);
-          wm0[r] = m;
-        }
-        s_wc[r][0][tid] = w.B;
-        s_wc[r][1][tid] = w.Q;
-        s_wc[r][2][tid] = w.NL;
-        s_wc[r][3][tid] = w.O;
-        s_wc[r][4][tid] = w.C;
-        s_wc[r][5][tid] = w.P;
-        s_wc[r][6][tid] = w.K;
+        const uint64_t E = escape_word(w.B, 0u);
+        const uint64_t Qe = w.Q & ~E;
+        const uint64_t S0 = prefix_xor(Qe);
+        M4 m;
+        m.q = popc64(Qe) & 1;
+        m.d0 = popc64(w.O & ~S0) - popc64(w.C & ~S0);
+        m.tot = popc64(w.O) - popc64(w.C);
+        m.nl = popc64(w.NL);
+        wm0[r] = m;
+        s_wc[r][kSlotS0][tid] = S0;
+        s_wc[r][kSlotNL][tid] = w.NL;
+        s_wc[r][kSlotO][tid] = w.O;
+        s_wc[r][kSlotC][tid] = w.C;
+        s_wc[r][kSlotP][tid] = w.P;
+        if (VERIFY) s_wc[r][kSlotB][tid] = w.B;
+        s_wc[r][kSlotQ][tid] = w.Q;
+        s_wc[r][kSlotK][tid] = w.K;
+        s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
       }
     }
 
@@ -448,12 +486,12 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       wm[r] = wm0[r];
       if (ent[r] == 3) {
         // escape carry-in 1 (the word before ends in an odd backslash run): rare
-        const uint64_t B = s_wc[r][0][tid], Q = s_wc[r][1][tid], O = s_wc[r][3][tid], C = s_wc[r][4][tid];
-        const uint64_t E = escape_word(B, 1u);
-        const uint64_t Qe = Q & ~E;
-        const uint64_t S0 = prefix_xor(Qe);
-        wm[r].q = popc64(Qe) & 1;
-        wm[r].d0 = popc64(O & ~S0) - popc64(C & ~S0);
+        // (the quote right after the leading backslash run, if any, toggles)
+        const int ld = s_lead[r][tid];
+        const uint64_t S0 = s_wc[r][kSlotS0][tid], O = s_wc[r][kSlotO][tid], C = s_wc[r][kSlotC][tid];
+        const uint64_t S1 = (ld & 0x80) ? (S0 ^ (~0ull << (ld & 0x7F))) : S0;
+        wm[r].q ^= ld >> 7;
+        wm[r].d0 = popc64(O & ~S1) - popc64(C & ~S1);
       }
       if (ent[r] == 0 && code[r] != 0) {
         // the tile's first word that is not all backslashes
@@ -467,14 +505,8 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
     M4 lex[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
-      M4 inc = wm[r];
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        M4 y = shfl_up_m(inc, off);
-        if (lane >= off) inc = m_cat(y, inc);
-      }
-      lex[r] = shfl_up_m(inc, 1);
-      if (lane == 0) lex[r] = m_id();
+      const M4 inc = wave_scan_m(wm[r]);
+      lex[r] = lane_below_m(inc);
       if (lane == 63) s_m[r][wave] = inc;
     }
     __syncthreads();
@@ -528,8 +560,8 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       const int p = ent[r] ? (ent[r] - 2) : X.p;
       const int s = X.s ^ mx.q;
       const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-      emit_word<NLV, VERIFY>(J, w, s_wc, r, tid, p, s, d);
-      uint64_t nlm = s_wc[r][2][tid];
+      emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d);
+      uint64_t nlm = s_wc[r][kSlotNL][tid];
       int64_t idx = X.lines + mx.nl;
       while (nlm) {
         const int b = ctz64(nlm);
@@ -579,8 +611,25 @@ struct IndexConfig {
   int nlv;
 };
 
+// Blocks of `fn` the device holds at once (occupancy x CUs).
+static int resident_blocks(const void* fn, int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess) return 1;
+  return cus * (per_cu > 0 ? per_cu : 1) > 0 ? cus * (per_cu > 0 ? per_cu : 1) : 1;
+}
+
+// The grid is persistent: at most as many blocks as are resident at once, each
+// claiming tiles until none are left, so no block is launched per tile.
 template <int BLOCK, int WPT, int NLV>
 static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
+  static int resident[2] = {0, 0};  // per kernel variant (same value on every device of a process)
+  const void* fn = verify ? reinterpret_cast<const void*>(&index_kernel<BLOCK, WPT, NLV, true>)
+                          : reinterpret_cast<const void*>(&index_kernel<BLOCK, WPT, NLV, false>);
+  int& cap = resident[verify ? 1 : 0];
+  if (cap == 0) cap = resident_blocks(fn, BLOCK);
+  if (grid > cap) grid = cap;
   if (verify)
     hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
   else
