@@ -8,15 +8,15 @@ namespace gj {
 constexpr int kMaxLevels = 64;
 
 // Per-tile lookback state: self-validating 64-bit words (bit 63 = written), so a
-// reader never needs a fence between a flag and its payload.  One 64-byte line
-// per tile.
+// reader never needs a fence between a flag and its payload.  The tile's
+// aggregate, its inclusive prefix and (for the last tile of each group of 64
+// tiles) the group's aggregate are each two words, written and read as one
+// 16-byte access.  One 64-byte line per tile.
 struct alignas(64) TileState {
-  unsigned long long agg0;  // valid | allB(62) | carry(61) | q0(60) | q1(59) | nl u32 (0..31)
-  unsigned long long agg1;  // valid | d0 for carry-in 0 (31 bits) | total open-close (31 bits)
-  unsigned long long agg2;  // valid | d0 for carry-in 1 (31 bits)
-  unsigned long long inc0;  // valid | p(62) | s(61) | depth (signed 48 bits)
-  unsigned long long inc1;  // valid | lines (63 bits)
-  unsigned long long pad[3];
+  unsigned long long agg0, agg1;  // the tile's aggregate (packed counts, see the index kernel)
+  unsigned long long inc0;        // valid | p(62) | s(61) | depth (signed 48 bits)
+  unsigned long long inc1;        // valid | lines (63 bits)
+  unsigned long long pad[4];      // [0..1]: group aggregate (last tile of a group)
 };
 
 // Level summaries: bit k of summary word s of level l is set iff leveled word

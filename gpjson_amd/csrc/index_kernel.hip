@@ -135,34 +135,63 @@ __device__ __forceinline__ int64_t sext(unsigned long long v, int bits) {
   return ((int64_t)(v << (64 - bits))) >> (64 - bits);
 }
 
-__device__ __forceinline__ unsigned long long ld_state(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Two state words as one 16-byte access that bypasses the non-coherent caches
+// (volatile: sc0 sc1), so every CU sees the other CUs' published words.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u64x2 ld_state2(const unsigned long long* p) {
+  return *reinterpret_cast<const volatile u64x2*>(p);
 }
 
-__device__ __forceinline__ void st_state(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_state2(unsigned long long* p, unsigned long long w0, unsigned long long w1) {
+  u64x2 v;
+  v.x = w0;
+  v.y = w1;
+  *reinterpret_cast<volatile u64x2*>(p) = v;
 }
 
-__device__ void publish_agg(TileState* st, const TileAgg& a, int nl) {
-  unsigned long long w0 = kValid | ((unsigned long long)(a.esc == 0) << 62) |
-                          ((unsigned long long)(a.esc == 3) << 61) |
-                          ((unsigned long long)(a.m[0].q & 1) << 60) |
-                          ((unsigned long long)(a.m[1].q & 1) << 59) | (unsigned)nl;
-  unsigned long long w1 = kValid | ((unsigned long long)(a.m[0].d0 & 0x7FFFFFFF)) |
-                          ((unsigned long long)(a.m[0].tot & 0x7FFFFFFF) << 31);
-  unsigned long long w2 = kValid | ((unsigned long long)(a.m[1].d0 & 0x7FFFFFFF));
-  st_state(&st->agg1, w1);
-  st_state(&st->agg2, w2);
-  st_state(&st->agg0, w0);
+// Aggregate of a tile (at most 64 KiB) or of a group of 64 tiles (at most
+// 4 MiB) in two words: counts in 24-bit fields.
+//   w0: valid | carry passes (62) | carry out 1 (61) | q0 (60) | q1 (59) | tot (24..47) | newlines (0..23)
+//   w1: valid | d0 for carry-in 1 (24..47) | d0 for carry-in 0 (0..23)
+constexpr unsigned long long kField24 = (1ull << 24) - 1;
+
+__device__ __forceinline__ void pack_agg(const TileAgg& a, unsigned long long& w0, unsigned long long& w1) {
+  w0 = kValid | ((unsigned long long)(a.esc == 0) << 62) | ((unsigned long long)(a.esc == 3) << 61) |
+       ((unsigned long long)(a.m[0].q & 1) << 60) | ((unsigned long long)(a.m[1].q & 1) << 59) |
+       (((unsigned long long)a.m[0].tot & kField24) << 24) | ((unsigned long long)a.m[0].nl & kField24);
+  w1 = kValid | ((unsigned long long)a.m[0].d0 & kField24) | (((unsigned long long)a.m[1].d0 & kField24) << 24);
+}
+
+__device__ __forceinline__ TileAgg decode_agg(unsigned long long w0, unsigned long long w1) {
+  TileAgg a;
+  a.esc = ((w0 >> 62) & 1) ? 0 : (2 + (int)((w0 >> 61) & 1));
+  const int nl = (int)(w0 & kField24);
+  const int tot = (int)sext((w0 >> 24) & kField24, 24);
+  a.m[0] = M4{(int)((w0 >> 60) & 1), (int)sext(w1 & kField24, 24), tot, nl};
+  a.m[1] = M4{(int)((w0 >> 59) & 1), (int)sext((w1 >> 24) & kField24, 24), tot, nl};
+  return a;
+}
+
+__device__ void publish_agg(TileState* st, const TileAgg& a) {
+  unsigned long long w0, w1;
+  pack_agg(a, w0, w1);
+  st_state2(&st->agg0, w0, w1);
+}
+
+// The aggregate of the 64 tiles of the group that st's tile ends.
+__device__ void publish_group(TileState* st, const TileAgg& a) {
+  unsigned long long w0, w1;
+  pack_agg(a, w0, w1);
+  st_state2(&st->pad[0], w0, w1);
 }
 
 __device__ void publish_inc(TileState* st, const Carry& c) {
-  unsigned long long w0 = kValid | ((unsigned long long)(c.p & 1) << 62) |
-                          ((unsigned long long)(c.s & 1) << 61) |
-                          ((unsigned long long)c.d & 0xFFFFFFFFFFFFull);
-  unsigned long long w1 = kValid | ((unsigned long long)c.lines & ~kValid);
-  st_state(&st->inc0, w0);
-  st_state(&st->inc1, w1);
+  const unsigned long long w0 = kValid | ((unsigned long long)(c.p & 1) << 62) |
+                                ((unsigned long long)(c.s & 1) << 61) |
+                                ((unsigned long long)c.d & 0xFFFFFFFFFFFFull);
+  const unsigned long long w1 = kValid | ((unsigned long long)c.lines & ~kValid);
+  st_state2(&st->inc0, w0, w1);
 }
 
 __device__ __forceinline__ TileAgg agg_id() {
@@ -218,62 +247,94 @@ __device__ __forceinline__ TileAgg reduce_tiles(TileAgg x, int lane) {
 }
 
 // Exclusive carry of local tile lt of a file whose tiles start at st0,
-// resolved by one wavefront: each lane reads one predecessor's published state
-// per round trip, so up to 64 tiles are folded per step.  Tile 0 of a file
+// resolved by one wavefront.  Tiles form groups of 64 (tiles 64g .. 64g+63);
+// the last tile of a group publishes the group's aggregate (in its state's pad
+// words) as soon as the group's other tiles have published theirs, and a
+// group's inclusive prefix is that of its last tile.  The walk reads, per
+// round trip, one unit per lane nearest first: the tiles of the cursor tile's
+// group at or below it, then whole groups below.  It stops at the first unit
+// holding an inclusive prefix (done) or not ready: a tile not ready is waited
+// for, a group without its aggregate is walked tile by tile.  Tile 0 of a file
 // always publishes its inclusive prefix, so the walk ends there at the latest.
-__device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane) {
-  TileAgg acc = agg_id();  // tiles (k_hi, lt) already folded
-  int64_t k_hi = lt - 1;
+__device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane, const TileAgg& own) {
+  TileAgg acc = agg_id();  // units between the cursor and lt, already folded
+  int64_t c = lt - 1;      // cursor: the nearest tile not folded yet
+  bool group_todo = (lt & 63) == 63;
+  const Carry file_start{0, 0, -1, 1};  // the empty prefix before tile 0
   for (;;) {
-    const int64_t k = k_hi - lane;
+    if (c < 0) return apply_agg(acc, file_start);
+    const int64_t g = c >> 6;
+    const int p = (int)(c - (g << 6)) + 1;  // tiles of the cursor's group at or below it
     int status = 0;  // 0 not ready, 1 aggregate, 2 inclusive
+    bool agg_ok = false;
     TileAgg a = agg_id();
-    Carry c{0, 0, 0, 0};
-    if (k >= 0) {
-      // all five words in one round trip (each validates itself)
-      TileState* s = st0 + k;
-      const unsigned long long i0 = ld_state(&s->inc0), i1 = ld_state(&s->inc1);
-      const unsigned long long a0 = ld_state(&s->agg0), a1 = ld_state(&s->agg1), a2 = ld_state(&s->agg2);
-      if ((i0 & kValid) && (i1 & kValid)) {
+    Carry cc{0, 0, 0, 0};
+    const TileState* s = nullptr;
+    bool is_group = false;
+    if (lane < p) {
+      s = st0 + (c - lane);
+    } else if (g - 1 - (lane - p) >= 0) {
+      s = st0 + ((g - 1 - (lane - p)) * 64 + 63);
+      is_group = true;
+    } else {
+      status = 2;  // before the file: the empty prefix
+      cc = file_start;
+    }
+    if (s) {
+      // both halves in one round trip (each word validates itself)
+      const u64x2 iv = ld_state2(&s->inc0);
+      const u64x2 av = ld_state2(is_group ? &s->pad[0] : &s->agg0);
+      agg_ok = (av.x & kValid) && (av.y & kValid);
+      if (agg_ok) a = decode_agg(av.x, av.y);
+      if ((iv.x & kValid) && (iv.y & kValid)) {
         status = 2;
-        c.p = (int)((i0 >> 62) & 1);
-        c.s = (int)((i0 >> 61) & 1);
-        c.d = sext(i0 & 0xFFFFFFFFFFFFull, 48);
-        c.lines = (int64_t)(i1 & ~kValid);
-      } else {
-        if ((a0 & kValid) && (a1 & kValid) && (a2 & kValid)) {
-          status = 1;
-          a.esc = ((a0 >> 62) & 1) ? 0 : (2 + (int)((a0 >> 61) & 1));
-          const int nl = (int)(a0 & 0xFFFFFFFFu);
-          const int tot = (int)sext((a1 >> 31) & 0x7FFFFFFF, 31);
-          a.m[0] = M4{(int)((a0 >> 60) & 1), (int)sext(a1 & 0x7FFFFFFF, 31), tot, nl};
-          a.m[1] = M4{(int)((a0 >> 59) & 1), (int)sext(a2 & 0x7FFFFFFF, 31), tot, nl};
-        }
+        cc.p = (int)((iv.x >> 62) & 1);
+        cc.s = (int)((iv.x >> 61) & 1);
+        cc.d = sext(iv.x & 0xFFFFFFFFFFFFull, 48);
+        cc.lines = (int64_t)(iv.y & ~kValid);
+      } else if (agg_ok) {
+        status = 1;
       }
+    }
+    if (group_todo) {
+      // the group's last tile waits for the aggregates of its other 63 tiles
+      // (lanes 0..62, nearest first; the cursor stays at lt - 1 meanwhile) and
+      // publishes the group's: they are all running and publish without waiting
+      const unsigned long long m_ok = __ballot(lane < 63 && agg_ok);
+      if (m_ok != (~0ull >> 1)) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      const TileAgg grp = agg_cat(reduce_tiles(lane < 63 ? a : agg_id(), lane), own);
+      if (lane == 0) publish_group(st0 + lt, grp);
+      group_todo = false;
     }
     const unsigned long long m_inc = __ballot(status == 2);
     const unsigned long long m_nr = __ballot(status == 0);
     const int a_inc = m_inc ? __builtin_ctzll(m_inc) : 64;
     const int a_nr = m_nr ? __builtin_ctzll(m_nr) : 64;
     if (a_inc < a_nr) {
-      // lanes [0, a_inc) are aggregates, lane a_inc the nearest inclusive prefix
+      // units [0, a_inc) are aggregates, unit a_inc the nearest inclusive prefix
       if (lane >= a_inc) a = agg_id();
       const TileAgg part = reduce_tiles(a, lane);
       Carry base;
-      base.p = __shfl(c.p, a_inc);
-      base.s = __shfl(c.s, a_inc);
-      base.d = shfl_i64(c.d, a_inc);
-      base.lines = shfl_i64(c.lines, a_inc);
+      base.p = __shfl(cc.p, a_inc);
+      base.s = __shfl(cc.s, a_inc);
+      base.d = shfl_i64(cc.d, a_inc);
+      base.lines = shfl_i64(cc.lines, a_inc);
       return apply_agg(agg_cat(part, acc), base);
     }
-    if (a_nr == 0) {
+    if (a_nr == 0) {  // unit 0 is always a tile: wait for it
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    // fold the ready aggregates [0, a_nr) and continue below them
+    // fold the ready aggregates [0, a_nr) and continue at the unit that stopped
+    // (all 64 units aggregates: continue below them)
     if (lane >= a_nr) a = agg_id();
     acc = agg_cat(reduce_tiles(a, lane), acc);
-    k_hi -= a_nr;
+    if (a_nr < p) c -= a_nr;                       // a tile
+    else if (a_nr < 64) c = (g - 1 - (a_nr - p)) * 64 + 63;  // a group without its aggregate: its tiles
+    else c = (g - (64 - p)) * 64 - 1;               // below the last group read
   }
 }
 
@@ -344,6 +405,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
   static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
+  static_assert(TILE_WORDS * 64 <= 65536, "lookback state fields assume tiles of at most 64 KiB (groups of 4 MiB)");
 
   __shared__ unsigned s_ticket;
   __shared__ int s_job;
@@ -530,12 +592,14 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       a.m[1] = mtile;
       if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
       Carry cin;
+      // every tile publishes its aggregate (tile 0's too: the last tile of
+      // group 0 waits for the aggregates of all the group's tiles)
+      if (lane == 0) publish_agg(st0 + lt, a);
       if (lt == 0) {
         cin = Carry{0, 0, -1, 1};
         if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
       } else {
-        if (lane == 0) publish_agg(st0 + lt, a, mtile.nl);
-        cin = lookback_wave(st0, lt, lane);
+        cin = lookback_wave(st0, lt, lane, a);
       }
       if (lane == 0) {
         publish_inc(st0 + lt, apply_agg(a, cin));
