@@ -715,7 +715,7 @@ int64_t index_tile_bytes(int block, int wpt) { return (int64_t)block * wpt * 64;
 
 bool index_config_supported(int block, int wpt) {
   return (block == 256 && (wpt >= 1 && wpt <= 4)) || (block == 512 && (wpt == 1 || wpt == 2)) ||
-         (block == 128 && (wpt == 2 || wpt == 4));
+         (block == 128 && (wpt >= 2 && wpt <= 4));
 }
 
 hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
@@ -727,6 +727,7 @@ hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool 
   if (block == 512 && wpt == 1) return launch_nlv<512, 1>(L, nlv, verify, grid, stream);
   if (block == 512 && wpt == 2) return launch_nlv<512, 2>(L, nlv, verify, grid, stream);
   if (block == 128 && wpt == 2) return launch_nlv<128, 2>(L, nlv, verify, grid, stream);
+  if (block == 128 && wpt == 3) return launch_nlv<128, 3>(L, nlv, verify, grid, stream);
   if (block == 128 && wpt == 4) return launch_nlv<128, 4>(L, nlv, verify, grid, stream);
   return hipErrorInvalidValue;
 }
