@@ -39,7 +39,7 @@ struct gpjson_batch {
   int64_t ir_total = 0;  // IR bytes of every job, contiguous in dev_ir
   bool chain = false;    // every job is a member chain: chain evaluator
   // device memory
-  DevBuf arena_s, arena_rc, arena_rq, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
+  DevBuf arena_rec, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
   DevBuf arena_long;      // [2 counters, 64 B][long-line work items]
   int64_t long_cap = 0;
   int64_t* total_dev = nullptr;
@@ -112,9 +112,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     span_elems += b->files[b->job_file[j]].nl_cap * b->job_R[j] * 2;
   }
   b->total_tiles = tiles;
-  int rc = b->arena_s.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
-  if (!rc) rc = b->arena_rc.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
-  if (!rc) rc = b->arena_rq.ensure((size_t)std::max<int64_t>(s_words, 1) * 8);
+  int rc = b->arena_rec.ensure((size_t)std::max<int64_t>(s_words, 1) * kRecWords * 8);
   if (!rc) rc = b->arena_lb.ensure((size_t)std::max<int64_t>(lb_words, 1) * 8);
   if (!rc) rc = b->arena_nl.ensure((size_t)std::max<int64_t>(nl_elems, 1) * 8);
   if (!rc) rc = b->arena_spans.ensure((size_t)std::max<int64_t>(span_elems, 1) * 8);
@@ -129,9 +127,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
   for (int f = 0; f < b->nfiles; f++) {
     FileJob& J = b->files[f];
     J.F = b->bufs[f]->d;
-    J.S = (uint64_t*)b->arena_s.p + s_off[f];
-    J.RC = (uint64_t*)b->arena_rc.p + s_off[f];
-    J.RQ = (uint64_t*)b->arena_rq.p + s_off[f];
+    J.rec = (uint64_t*)b->arena_rec.p + s_off[f] * kRecWords;
     J.LB = (uint64_t*)b->arena_lb.p + lb_off[f];
     J.LS = J.LB + J.Wp * std::max(J.L, 1);
     J.Ws = summary_stride(J.Wp);
@@ -157,12 +153,10 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
     q.N = J.N;
     q.NL = J.NL;
     q.lines = 0;
-    q.S = J.S;
+    q.rec = J.rec;
     q.LB = J.LB;
     q.LS = J.LS;
     q.Ws = J.Ws;
-    q.RC = J.RC;
-    q.RQ = J.RQ;
     q.Wp = J.Wp;
     q.L = J.L;
     q.R = b->job_R[j];
@@ -394,7 +388,7 @@ void gpjson_batch_free(gpjson_batch* b) {
   drop_graph(b);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
-  DevBuf* bufs[] = {&b->arena_s, &b->arena_rc, &b->arena_rq, &b->arena_lb, &b->arena_nl, &b->arena_spans,
+  DevBuf* bufs[] = {&b->arena_rec, &b->arena_lb, &b->arena_nl, &b->arena_spans,
                     &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states, &b->arena_long};
   for (DevBuf* d : bufs) d->release();
   delete b;

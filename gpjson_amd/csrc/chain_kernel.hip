@@ -89,10 +89,8 @@ __global__ __launch_bounds__(BLOCK) void chain_kernel(QueryLaunch launch) {
             item0 = Q.item0;
             J.F = Q.F;
             J.NL = Q.NL;
-            J.S = Q.S;
+            J.rec = Q.rec;
             J.LB = Q.LB;
-            J.RC = Q.RC;
-            J.RQ = Q.RQ;
             J.N = Q.N;
             J.Wp = Q.Wp;
             J.L = Q.L;

@@ -51,10 +51,8 @@ enum ChainPhase : int {
 struct ChainJobView {
   const uint8_t* F;
   const int64_t* NL;
-  const uint64_t* S;
+  const uint64_t* rec;  // string, colon and raw-quote words of each input word
   const uint64_t* LB;
-  const uint64_t* RC;
-  const uint64_t* RQ;
   int64_t* out_base;  // int64[lines][R][2]
   int64_t N, lines, Wp;
   int L, R, ir0;      // ir0: offset of the job's IR in the IR bytes
@@ -133,15 +131,15 @@ GJ_HD void chain_load(const ChainLane& s, const ChainJobView& J, int r, ChainLoa
       if (w <= wlast) {
         ld.lw[u] = lv[w];
         if (s.phase == kKey) {
-          ld.cw[u] = J.RC[w];
-          ld.sw[u + 1] = J.S[w];
-          ld.qw[u + 1] = J.RQ[w];
+          ld.cw[u] = rec_colon(J.rec, w);
+          ld.sw[u + 1] = rec_s(J.rec, w);
+          ld.qw[u + 1] = rec_quote(J.rec, w);
         }
       }
     }
     if (s.phase == kKey && w0 > (s.start >> 6) && w0 - 1 <= wlast) {
-      ld.sw[0] = J.S[w0 - 1];
-      ld.qw[0] = J.RQ[w0 - 1];
+      ld.sw[0] = rec_s(J.rec, w0 - 1);
+      ld.qw[0] = rec_quote(J.rec, w0 - 1);
     }
   } else if (s.phase == kVerify) {
     const int b0 = s.vdone + 8 * r;
@@ -202,7 +200,7 @@ GJ_HD ChainPart chain_lane(const ChainLane& s, const ChainJobView& J, int r, con
         if (slow) {
           // the string run starts before the loaded words: read on
           wi = k >> 6;
-          sv = J.S[wi] & chain_mask_upto((int)(k & 63));
+          sv = rec_s(J.rec, wi) & chain_mask_upto((int)(k & 63));
           for (;;) {
             if (wi == wstart) sv &= ~chain_mask_upto((int)(s.start & 63));
             if (sv) {
@@ -211,7 +209,7 @@ GJ_HD ChainPart chain_lane(const ChainLane& s, const ChainJobView& J, int r, con
             }
             if (wi <= wstart) break;
             wi--;
-            sv = J.S[wi];
+            sv = rec_s(J.rec, wi);
           }
         }
         if (e < 0) continue;
@@ -219,7 +217,7 @@ GJ_HD ChainPart chain_lane(const ChainLane& s, const ChainJobView& J, int r, con
         if (sc < s.start) continue;
         const int64_t qi = sc >> 6;
         const int qidx = (int)(qi - ld.w0) + 1;
-        const uint64_t q = (qidx >= 0 && qidx <= kChainWords) ? chain_pick(ld.qw, qidx) : J.RQ[qi];
+        const uint64_t q = (qidx >= 0 && qidx <= kChainWords) ? chain_pick(ld.qw, qidx) : rec_quote(J.rec, qi);
         if (!((q >> (sc & 63)) & 1ull)) continue;
         out.pos = p;
         out.ss = sc;

@@ -30,13 +30,26 @@ inline int64_t summary_stride(int64_t Wp) { return (Wp + 63) / 64; }
 // Words of a leveled-bitmap allocation (levels, then their summaries).
 inline int64_t leveled_alloc_words(int64_t Wp, int L) { return (Wp + summary_stride(Wp)) * (L > 1 ? L : 1); }
 
+// The string, colon and raw-quote words of a 64-byte input word form one
+// 24-byte record, so an interpreter step that needs all three touches one
+// cache line instead of three.
+constexpr int kRecWords = 3;
+#if defined(__HIPCC__)
+#define GJ_REC_HD __host__ __device__ __forceinline__
+#else
+#define GJ_REC_HD static inline
+#endif
+GJ_REC_HD uint64_t rec_s(const uint64_t* R, int64_t w) { return R[w * kRecWords]; }      // in-string bits
+GJ_REC_HD uint64_t rec_colon(const uint64_t* R, int64_t w) { return R[w * kRecWords + 1]; }  // ':' bytes
+GJ_REC_HD uint64_t rec_quote(const uint64_t* R, int64_t w) { return R[w * kRecWords + 2]; }  // '"' bytes
+
 // One file of an index launch.  The plain path launches with one file.
 struct FileJob {
   const uint8_t* F;     // input bytes
   int64_t N;            // bytes
   int64_t W;            // words = ceil(N/64)
   int64_t Wp;           // level stride (W rounded up to 8)
-  uint64_t* S;          // string index, W words
+  uint64_t* rec;        // records (string, colon, raw-quote words), kRecWords * W words
   uint64_t* LB;         // leveled bitmaps, L * Wp words
   uint64_t* LS;         // level summaries, L * Ws words (may be null)
   int64_t Ws;           // summary stride
@@ -44,8 +57,6 @@ struct FileJob {
   int64_t nl_cap;
   uint64_t* E;          // escape index (verify mode, may be null)
   uint64_t* Q;          // quote index (verify mode, may be null)
-  uint64_t* RC;         // raw ':' bytes, W words (key candidates for the interpreter)
-  uint64_t* RQ;         // raw '"' bytes, W words (key-start test of the interpreter)
   int64_t* tile_lines;  // optional: first line id starting in each tile, [ntiles + 1]
   int64_t tile0;        // first global tile id of this file
   int64_t ntiles;

@@ -5,10 +5,9 @@
 // J/stages/LeveledBitmapsIndex.java:40-103; kernels K/create_combined_*.cu,
 // K/create_quote_index.cu, K/create_string_index.cu, K/create_leveled_*.cu):
 //   newline index   int64[lines]   ([0] = 0, then every '\n')
-//   string index    uint64[W]      (in-string bits)
+//   records         uint64[W][3]   (in-string bits, raw ':' and '"' bytes)
 //   leveled bitmaps uint64[L][Wp]  (structural chars per depth level)
 //   level summaries uint64[L][Ws]  (a bit per leveled word that is not zero)
-//   colon / quote   uint64[W]      (raw ':' and '"' bytes, for the interpreter)
 //   escape / quote  uint64[W]      (only in verify mode)
 //
 // Work decomposition.  A tile is ROWS rows of BLOCK consecutive 64-byte words;
@@ -363,10 +362,12 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
   uint64_t S = wc[r][kSlotS0][tid];
   if (p && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
   S = (s ? ~S : S) & valid;
-  J.S[w] = S;
-  // (bytes past the end of the file were loaded as zeros, which match neither)
-  if (J.RC) J.RC[w] = wc[r][kSlotK][tid];
-  if (J.RQ) J.RQ[w] = wc[r][kSlotQ][tid];
+  // the word's record: string, colon and raw-quote words (bytes past the end
+  // of the file were loaded as zeros, which are neither ':' nor '"')
+  uint64_t* rec = J.rec + w * kRecWords;
+  rec[0] = S;
+  rec[1] = wc[r][kSlotK][tid];
+  rec[2] = wc[r][kSlotQ][tid];
   if (VERIFY) {
     const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
     if (J.E) J.E[w] = E & valid;

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "index_common.h"
+
 namespace gj {
 
 // IR opcodes: ordinal of the reference's Opcode enum
@@ -22,10 +24,8 @@ struct QueryJob {
   int64_t N;
   const int64_t* NL;  // newline index
   int64_t lines;
-  const uint64_t* S;  // string index
+  const uint64_t* rec; // records: string, colon (key candidates) and raw-quote words per input word
   const uint64_t* LB; // leveled bitmaps, level stride Wp
-  const uint64_t* RC; // raw ':' bytes (bit per input byte)
-  const uint64_t* RQ; // raw '"' bytes (bit per input byte)
   const uint64_t* LS; // level summaries (bit per leveled word), stride Ws; may be null
   int64_t Ws;
   int64_t Wp;

@@ -83,11 +83,11 @@ struct Group {
 struct GlobalWords {
   const uint64_t* LB;
   int64_t Wp;
-  const uint64_t* S;
-  const uint64_t* LS;  // level summaries (may be null: no skipping)
+  const uint64_t* rec;  // records: string, colon and raw-quote words
+  const uint64_t* LS;   // level summaries (may be null: no skipping)
   int64_t Ws;
   __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return LB[(int64_t)lvl * Wp + wi]; }
-  __device__ __forceinline__ uint64_t s(int64_t wi) const { return S[wi]; }
+  __device__ __forceinline__ uint64_t s(int64_t wi) const { return rec_s(rec, wi); }
   __device__ __forceinline__ bool has_summary() const { return LS != nullptr; }
   __device__ __forceinline__ uint64_t ls(int lvl, int64_t si) const { return LS[(int64_t)lvl * Ws + si]; }
 };
@@ -182,7 +182,7 @@ __device__ int64_t group_first_bit(const Group<G>& g, const Words& W, int lvl, i
 // reads the raw-quote bitmap, so input bytes are read only for candidates
 // whose length already fits.  sw / qw: string and raw-quote words of p's word.
 template <class Words>
-__device__ bool key_matches(const uint8_t* F, const uint64_t* RQ, const Words& W, int64_t start, int64_t p,
+__device__ bool key_matches(const uint8_t* F, const uint64_t* rec, const Words& W, int64_t start, int64_t p,
                             uint64_t sw_p, uint64_t qw_p, const uint8_t* key, int klen) {
   const int64_t k = p - 1;
   if (k <= start) return false;
@@ -201,7 +201,7 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* RQ, const Words& W
   const int64_t ss = se - klen;
   if (ss < start) return false;
   const int64_t qi = ss >> 6;
-  const uint64_t qw = qi == wp ? qw_p : RQ[qi];
+  const uint64_t qw = qi == wp ? qw_p : rec_quote(rec, qi);
   if (!((qw >> (ss & 63)) & 1ull)) return false;
   // compare 8 bytes per round trip (the loads of a chunk are independent)
   for (int i0 = 0; i0 < klen; i0 += 8) {
@@ -220,7 +220,7 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* RQ, const Words& W
 // Candidates are the level's bits that are ':' bytes (colon bitmap), so no
 // input byte is read to find them.
 template <int G, class Words>
-__device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint64_t* RC, const uint64_t* RQ, int lvl,
+__device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint64_t* rec, int lvl,
                                   const uint8_t* F, int64_t start, int64_t from, int64_t to, const uint8_t* key,
                                   int klen) {
   if (from >= to) return -1;
@@ -236,9 +236,9 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
       sw[u] = 0;
       qw[u] = 0;
       if (wi <= wend) {
-        v[u] = window_word(W, lvl, wi, wbeg, wend, from, to) & RC[wi];
-        sw[u] = W.s(wi);
-        qw[u] = RQ[wi];
+        v[u] = window_word(W, lvl, wi, wbeg, wend, from, to) & rec_colon(rec, wi);
+        sw[u] = rec_s(rec, wi);
+        qw[u] = rec_quote(rec, wi);
       }
     }
     load_summary(W, lvl, nb, wend, sm0, sm1);
@@ -248,7 +248,7 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
       uint64_t t = found < 0 ? v[u] : 0;
       while (t) {
         const int64_t p = ((w0 + u) << 6) + ctz64(t);
-        if (key_matches(F, RQ, W, start, p, sw[u], qw[u], key, klen)) { found = p; break; }
+        if (key_matches(F, rec, W, start, p, sw[u], qw[u], key, klen)) { found = p; break; }
         t &= t - 1;
       }
     }
@@ -461,7 +461,7 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
     int64_t p = -1;
     if (lvl < J.L) {
       if (op == kMoveToKey) {
-        p = group_find_key(g, W, J.RC, J.RQ, lvl, F, start, j, to, want, want_len);
+        p = group_find_key(g, W, J.rec, lvl, F, start, j, to, want, want_len);
       } else if (op == kMoveToIndex) {
         int cnt = ci[lvl];
         p = group_find_comma(g, W, lvl, F, n, j, to, &cnt, want_idx);
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
         const QueryJob& J = launch.jobs[q];
         if (defer(J, common, i, it)) continue;
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
       }
       continue;
     }
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
       if (defer(J, J.lines, i, it)) continue;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
                 nslots);
     }
   }
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
       const int64_t q = it / common, i = it - q * common;
       const QueryJob& J = launch.jobs[q];
       int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
     } else {
       int lo = 0, hi = (int)launch.njobs - 1;
       while (lo < hi) {
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
       }
       const QueryJob& J = launch.jobs[lo];
       const int64_t i = it - J.item0;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.S, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
                 nslots);
     }
   }
@@ -641,14 +641,15 @@ __global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, 
       const int64_t nw = hi_b > lo_b ? ((hi_b - 1) >> 6) - wlo + 1 : 1;
       const bool staged = nw <= maxw;
       if (staged) {
-        for (int lv = 0; lv <= J.L; lv++) {
-          const uint64_t* src = lv < J.L ? J.LB + (int64_t)lv * J.Wp : J.S;
+        for (int lv = 0; lv < J.L; lv++) {
+          const uint64_t* src = J.LB + (int64_t)lv * J.Wp;
           for (int64_t k = threadIdx.x; k < nw; k += BLOCK) words[lv * maxw + k] = src[wlo + k];
         }
+        for (int64_t k = threadIdx.x; k < nw; k += BLOCK) words[J.L * maxw + k] = rec_s(J.rec, wlo + k);
       }
       __syncthreads();
       const LdsWords LW{words, wlo, maxw, J.L};
-      const GlobalWords GW{J.LB, J.Wp, J.S, J.LS, J.Ws};
+      const GlobalWords GW{J.LB, J.Wp, J.rec, J.LS, J.Ws};
       for (int64_t i = la + grp; i < lb; i += NG) {
         int64_t* out = J.out + i * (int64_t)J.R * 2;
         if (staged) eval_line(g, J, LW, ir, lines, i, out, le, ci, nslots);

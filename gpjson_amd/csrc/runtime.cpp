@@ -174,7 +174,7 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   j.N = idx->n;
   j.W = idx->W;
   j.Wp = idx->Wp;
-  j.S = idx->S;
+  j.rec = idx->rec;
   j.LB = idx->LB;
   j.LS = idx->LS;
   j.Ws = idx->Ws;
@@ -182,8 +182,6 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   j.nl_cap = idx->nl_cap;
   j.E = idx->E;
   j.Q = idx->Q;
-  j.RC = idx->RC;
-  j.RQ = idx->RQ;
   j.tile_lines = idx->tile_lines;
   j.tile0 = 0;
   j.ntiles = ntiles;
@@ -240,12 +238,10 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.N = idx->n;
     j.NL = idx->NL;
     j.lines = tiled ? idx->lines : 0;
-    j.S = idx->S;
+    j.rec = idx->rec;
     j.LB = idx->LB;
     j.LS = idx->LS;
     j.Ws = idx->Ws;
-    j.RC = idx->RC;
-    j.RQ = idx->RQ;
     j.Wp = idx->Wp;
     j.L = idx->L;
     j.R = R;
@@ -320,11 +316,11 @@ static int read_lines(gpjson_index* idx) {
 }
 
 static void free_index_arrays(gpjson_index* idx) {
-  void* ps[] = {idx->S, idx->LB, idx->NL, idx->E, idx->Q, idx->RC, idx->RQ, idx->lines_dev, idx->tile_lines};
+  void* ps[] = {idx->rec, idx->LB, idx->NL, idx->E, idx->Q, idx->lines_dev, idx->tile_lines};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   idx->LS = nullptr;
-  idx->S = idx->LB = idx->E = idx->Q = idx->RC = idx->RQ = nullptr;
+  idx->rec = idx->LB = idx->E = idx->Q = nullptr;
   idx->NL = idx->lines_dev = idx->tile_lines = nullptr;
 }
 
@@ -528,9 +524,7 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
   auto alloc = [&](void** p, size_t bytes) {
     if (rc == GPJSON_OK) rc = dev_alloc(p, bytes);
   };
-  alloc((void**)&idx->S, (size_t)idx->W * 8);
-  alloc((void**)&idx->RC, (size_t)idx->W * 8);
-  alloc((void**)&idx->RQ, (size_t)idx->W * 8);
+  alloc((void**)&idx->rec, (size_t)idx->W * kRecWords * 8);
   alloc((void**)&idx->LB, (size_t)leveled_alloc_words(idx->Wp, levels) * 8);
   idx->Ws = summary_stride(idx->Wp);
   if (idx->LB) idx->LS = idx->LB + idx->Wp * std::max(levels, 1);
@@ -569,12 +563,12 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
   const void* src = nullptr;
   switch (which) {
     case GPJSON_INDEX_NEWLINE: size = idx->lines * 8; src = idx->NL; break;
-    case GPJSON_INDEX_STRING: size = idx->W * 8; src = idx->S; break;
+    case GPJSON_INDEX_STRING: size = idx->W * 8; src = idx->rec; break;
     case GPJSON_INDEX_LEVELED: size = idx->W * 8 * idx->L; src = idx->LB; break;
     case GPJSON_INDEX_ESCAPE: size = idx->W * 8; src = idx->E; break;
     case GPJSON_INDEX_QUOTE: size = idx->W * 8; src = idx->Q; break;
-    case GPJSON_INDEX_COLON: size = idx->W * 8; src = idx->RC; break;
-    case GPJSON_INDEX_RAW_QUOTE: size = idx->W * 8; src = idx->RQ; break;
+    case GPJSON_INDEX_COLON: size = idx->W * 8; src = idx->rec ? idx->rec + 1 : nullptr; break;
+    case GPJSON_INDEX_RAW_QUOTE: size = idx->W * 8; src = idx->rec ? idx->rec + 2 : nullptr; break;
     case GPJSON_INDEX_LEVEL_SUMMARY: size = (idx->W + 63) / 64 * 8 * idx->L; src = idx->LS; break;
     default: return fail(GPJSON_ERR_INVALID_ARG, "unknown index array " + std::to_string(which));
   }
@@ -586,6 +580,9 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
   if (which == GPJSON_INDEX_LEVELED) {
     GJ_HIP(hipMemcpy2D(dst, (size_t)idx->W * 8, src, (size_t)idx->Wp * 8, (size_t)idx->W * 8, (size_t)idx->L,
                        hipMemcpyDeviceToHost));
+  } else if (which == GPJSON_INDEX_STRING || which == GPJSON_INDEX_COLON || which == GPJSON_INDEX_RAW_QUOTE) {
+    // one word of each record
+    GJ_HIP(hipMemcpy2D(dst, 8, src, (size_t)kRecWords * 8, 8, (size_t)idx->W, hipMemcpyDeviceToHost));
   } else if (which == GPJSON_INDEX_LEVEL_SUMMARY) {
     const size_t row = (size_t)(idx->W + 63) / 64 * 8;
     GJ_HIP(hipMemcpy2D(dst, row, src, (size_t)idx->Ws * 8, row, (size_t)idx->L, hipMemcpyDeviceToHost));

@@ -111,7 +111,7 @@ struct gpjson_index {
   int64_t n = 0, W = 0, Wp = 0;
   int L = 0;
   int flags = 0;
-  uint64_t* S = nullptr;
+  uint64_t* rec = nullptr;  // records: string, colon and raw-quote words (kRecWords per word)
   uint64_t* LB = nullptr;
   uint64_t* LS = nullptr;  // level summaries, inside the LB allocation
   int64_t Ws = 0;
@@ -119,8 +119,6 @@ struct gpjson_index {
   int64_t nl_cap = 0;
   uint64_t* E = nullptr;
   uint64_t* Q = nullptr;
-  uint64_t* RC = nullptr;  // raw ':' bitmap (interpreter key candidates)
-  uint64_t* RQ = nullptr;  // raw '"' bitmap (interpreter key-start test)
   int64_t lines = 0;
   int64_t* lines_dev = nullptr;  // device copy of the line count
   int64_t* tile_lines = nullptr; // first line id of each index tile (+ total)

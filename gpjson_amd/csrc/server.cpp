@@ -123,8 +123,8 @@ namespace {
 
 int64_t index_bytes(int64_t n, int levels) {
   const int64_t W = (n + 63) / 64, Wp = (W + 7) / 8 * 8;
-  // string, raw-colon and raw-quote words, leveled words, newline and tile tables
-  return 3 * W * 8 + leveled_alloc_words(Wp, levels) * 8 + (n / 256 + 4096) * 8 + (n / 16384 + 2) * 8 + 64;
+  // records (string, colon and raw-quote words), leveled words, newline and tile tables
+  return kRecWords * W * 8 + leveled_alloc_words(Wp, levels) * 8 + (n / 256 + 4096) * 8 + (n / 16384 + 2) * 8 + 64;
 }
 
 // Frees LRU resident files (not used by in-flight batches, not being admitted)
@@ -282,12 +282,10 @@ int launch_batch(gpjson_server* s, Batch& b) {
     j.N = idx->n;
     j.NL = idx->NL;
     j.lines = idx->lines;
-    j.S = idx->S;
+    j.rec = idx->rec;
     j.LB = idx->LB;
     j.LS = idx->LS;
     j.Ws = idx->Ws;
-    j.RC = idx->RC;
-    j.RQ = idx->RQ;
     j.Wp = idx->Wp;
     j.L = idx->L;
     j.R = r.R;

@@ -2,6 +2,7 @@
 // (gpjson_amd/csrc/chain_step.h), run line by line with a group of G lanes
 // emulated in order (lane loads, lane parts, the group's combine, the update),
 // so its results can be compared with the oracle's interpreter without a GPU.
+#include <cstddef>
 #include <vector>
 
 #include "chain_step.h"
@@ -12,7 +13,15 @@ extern "C" {
 int64_t ch_run(const uint8_t* F, int64_t N, const int64_t* NL, int64_t lines, const uint64_t* S, const uint64_t* LB,
                int64_t Wp, int L, const uint64_t* RC, const uint64_t* RQ, const uint8_t* ir, int R, int64_t* out,
                int G, int64_t max_steps) {
-  gj::ChainJobView J{F, NL, S, LB, RC, RQ, out, N, lines, Wp, L, R, 0};
+  // the records the index kernel writes: string, colon and raw-quote words
+  const int64_t W = (N + 63) / 64;
+  std::vector<uint64_t> rec((std::size_t)(W + 1) * gj::kRecWords, 0);
+  for (int64_t w = 0; w < W; w++) {
+    rec[(std::size_t)w * gj::kRecWords] = S[w];
+    rec[(std::size_t)w * gj::kRecWords + 1] = RC[w];
+    rec[(std::size_t)w * gj::kRecWords + 2] = RQ[w];
+  }
+  gj::ChainJobView J{F, NL, rec.data(), LB, out, N, lines, Wp, L, R, 0};
   std::vector<gj::ChainLoads> ld(G);
   int64_t total = 0;
   for (int64_t i = 0; i < lines; i++) {
