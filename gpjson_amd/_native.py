@@ -33,6 +33,7 @@ GPJSON_INDEX_LEVEL_SUMMARY = 7
 
 GPJSON_BUILD_VERIFY = 1
 GPJSON_QUERY_DISABLE_FALLBACK = 1
+GPJSON_QUERY_GATHER = 2
 GPJSON_MAX_LEVELS = 64
 GPJSON_MAX_QUERIES = 255
 
@@ -71,6 +72,14 @@ SIGNATURES = {
     "gpjson_result_spans": (P(c_int64), [c_void_p, c_int64]),
     "gpjson_result_value": (c_int, [c_void_p, c_int64, c_int64, c_int, P(ctypes.c_void_p), P(c_int64)]),
     "gpjson_result_free": (None, [c_void_p]),
+    "gpjson_query_gather": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, P(c_void_p)]),
+    "gpjson_gathered_lines": (c_int64, [c_void_p]),
+    "gpjson_gathered_per_line": (c_int, [c_void_p]),
+    "gpjson_gathered_bytes": (c_int64, [c_void_p, P(c_void_p)]),
+    "gpjson_gathered_spans": (P(c_int64), [c_void_p, c_int64]),
+    "gpjson_gathered_file_spans": (P(c_int64), [c_void_p, c_int64]),
+    "gpjson_gathered_timing": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double)]),
+    "gpjson_gathered_free": (None, [c_void_p]),
     "gpjson_plan_create": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, P(c_void_p)]),
     "gpjson_plan_run": (c_int, [c_void_p, c_int]),
     "gpjson_plan_timing": (c_int, [c_void_p, P(c_double), P(c_double)]),

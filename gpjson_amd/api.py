@@ -166,6 +166,17 @@ class Context:
                                           out.ctypes.data_as(ctypes.c_void_p), out.size))
         return out
 
+    def query_gather(self, idx: "Index", irs: Sequence[bytes], results_per_line: int) -> "Gathered":
+        """Queries over one index with the result bytes gathered on the device and copied back once."""
+        nq = len(irs)
+        bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
+        ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
+        lens = (ctypes.c_int64 * nq)(*[len(ir) for ir in irs])
+        h = ctypes.c_void_p()
+        _check(self._lib.gpjson_query_gather(self._h, idx.handle, ptrs, lens, nq, results_per_line,
+                                             ctypes.byref(h)))
+        return Gathered(h, nq)
+
     def plan(self, buf: "Buffer", irs: Sequence[bytes]) -> "Plan":
         """Index + queries prepared once over a resident buffer (runs allocate nothing)."""
         nq = len(irs)
@@ -408,6 +419,55 @@ class Index:
             pass
 
 
+class Gathered:
+    """Result bytes gathered on the device: per query, spans into the file and
+    spans into one contiguous bytes buffer (-1 where there is no result)."""
+
+    def __init__(self, h, nq: int):
+        self._lib = N.load()
+        self._h = h
+        self.number_of_queries = nq
+        self.number_of_lines = int(self._lib.gpjson_gathered_lines(h))
+        self.results_per_line = int(self._lib.gpjson_gathered_per_line(h))
+        p = ctypes.c_void_p()
+        n = int(self._lib.gpjson_gathered_bytes(h, ctypes.byref(p)))
+        self.bytes = ctypes.string_at(p.value, n) if n else b""
+
+    def _spans(self, fn, q: int) -> np.ndarray:
+        p = fn(self._h, q)
+        n = self.number_of_lines * self.results_per_line * 2
+        if n == 0:
+            return np.zeros((0, self.results_per_line, 2), dtype=np.int64)
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.number_of_lines, self.results_per_line, 2).copy()
+
+    def spans(self, q: int) -> np.ndarray:
+        return self._spans(self._lib.gpjson_gathered_spans, q)
+
+    def file_spans(self, q: int) -> np.ndarray:
+        return self._spans(self._lib.gpjson_gathered_file_spans, q)
+
+    def values(self, q: int) -> List[List[Optional[bytes]]]:
+        """[line][k] -> the element's bytes, or None."""
+        sp = self.spans(q)
+        return [[None if a < 0 else self.bytes[a:b] for a, b in row] for row in sp]
+
+    def timing(self) -> dict:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.gpjson_gathered_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"query_ms": a.value, "gather_ms": b.value, "copy_ms": c.value}
+
+    def free(self) -> None:
+        if self._h:
+            self._lib.gpjson_gathered_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class ResultItem(Sequence):
     """Results of one query on one line: R entries, each str or None."""
 
@@ -533,6 +593,9 @@ class GPJSON:
                 raise GPJSONException("Invalid options", N.GPJSON_ERR_INVALID_ARG)
             if options.get("disable_fallback"):
                 flags |= N.GPJSON_QUERY_DISABLE_FALLBACK
+            # extension: result bytes gathered on the device instead of read from a map of the file
+            if options.get("gather"):
+                flags |= N.GPJSON_QUERY_GATHER
         arr = (ctypes.c_char_p * len(paths))(*[p.encode("utf-8") for p in paths])
         h = ctypes.c_void_p()
         _check(self._ctx._lib.gpjson_query_file(self._ctx.handle, str(file).encode(), arr, len(paths), flags,

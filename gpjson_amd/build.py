@@ -27,6 +27,7 @@ SOURCES = [
     "autotune.cpp",
     "diag.hip",
     "chain_kernel.hip",
+    "gather.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

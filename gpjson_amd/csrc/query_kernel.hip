@@ -409,7 +409,9 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
           j = le[lvl];
           if (lead) le[lvl] = -1;
           lvl--;
-          if (lvl < 0) return;
+          // a level whose end was never found (-1) ends the line: the walk
+          // would otherwise continue from before the line's first byte
+          if (lvl < 0 || j < 0) return;
           break;
         case kMoveDown: {
           lvl++;

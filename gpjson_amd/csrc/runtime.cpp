@@ -379,6 +379,7 @@ void gpjson_ctx_destroy(gpjson_ctx* ctx) {
   ctx->ticket.release();
   ctx->jobs.release();
   ctx->irbuf.release();
+  for (auto& b : ctx->gather) b.release();
   for (int b = 0; b < 2; b++) {
     if (ctx->pinned[b]) (void)hipHostFree(ctx->pinned[b]);
     if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
@@ -627,13 +628,85 @@ int gpjson_query_run(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* co
 // query(file, paths, options) and its Result
 // ---------------------------------------------------------------------------
 
+struct gpjson_gathered {
+  int64_t nq = 0, lines = 0;
+  int R = 0;
+  std::vector<int64_t> file_spans;  // [nq][lines][R][2] into the file
+  std::vector<int64_t> spans;       // [nq][lines][R][2] into bytes
+  std::vector<uint8_t> bytes;
+  double query_ms = 0, gather_ms = 0, copy_ms = 0;
+};
+
 struct gpjson_result {
   int64_t nq = 0, lines = 0;
   int R = 0;
   std::vector<std::vector<int64_t>> spans;
   const uint8_t* map = nullptr;
   int64_t map_len = 0;
+  gpjson_gathered* gathered = nullptr;  // values from the gathered bytes (no map)
 };
+
+namespace {
+
+// Interpreter launch + device gather + one copy back (gpjson_query_gather).
+int run_gather(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs, const int64_t* ir_lens, int nq,
+               int R, gpjson_gathered* g) {
+  using namespace gj;
+  const int64_t n = (int64_t)nq * idx->lines * R;  // elements
+  g->nq = nq;
+  g->lines = idx->lines;
+  g->R = R;
+  int rc = ctx->gather[0].ensure((size_t)std::max<int64_t>(n, 1) * 16);
+  if (!rc) rc = ctx->gather[1].ensure((size_t)(gather_blocks(n) + 8) * 8);
+  if (rc) return rc;
+  int64_t* spans_dev = (int64_t*)ctx->gather[0].p;
+  int64_t* sums = (int64_t*)ctx->gather[1].p + 8;
+  int64_t* total_dev = (int64_t*)ctx->gather[1].p;
+  hipEvent_t ev[4];
+  for (auto& e : ev) GJ_HIP(hipEventCreate(&e));
+  auto done = [&](int r) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return r;
+  };
+  if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) return done(hip_fail(hipErrorUnknown, "event"));
+  rc = enqueue_queries_dev(ctx, idx, irs, ir_lens, nq, R, spans_dev);
+  if (rc) return done(rc);
+  hipError_t e = hipEventRecord(ev[1], ctx->stream);
+  if (e == hipSuccess) e = gather_phase1(spans_dev, n, idx->n, sums, total_dev, ctx->stream);
+  int64_t total = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, total_dev, 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return done(hip_fail(e, "gather sizes"));
+  rc = ctx->gather[2].ensure((size_t)std::max<int64_t>(n, 1) * 16);
+  if (!rc) rc = ctx->gather[3].ensure((size_t)std::max<int64_t>(total, 1));
+  if (rc) return done(rc);
+  int64_t* out_spans = (int64_t*)ctx->gather[2].p;
+  uint8_t* out_bytes = (uint8_t*)ctx->gather[3].p;
+  e = gather_phase2(idx->buf->d, idx->n, spans_dev, n, sums, out_spans, out_bytes, ctx->num_cus * 8, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
+  g->file_spans.resize((size_t)n * 2);
+  g->spans.resize((size_t)n * 2);
+  g->bytes.resize((size_t)total);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpyAsync(g->file_spans.data(), spans_dev, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpyAsync(g->spans.data(), out_spans, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && total > 0)
+    e = hipMemcpyAsync(g->bytes.data(), out_bytes, (size_t)total, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev[3], ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return done(hip_fail(e, "gather"));
+  float a = 0, b = 0, c = 0;
+  (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+  (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+  (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+  g->query_ms = a;
+  g->gather_ms = b;  // includes the one 8-byte read of the total
+  g->copy_ms = c;
+  return done(GPJSON_OK);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -683,6 +756,19 @@ int gpjson_query_file(gpjson_ctx* ctx, const char* file, const char* const* path
     lens[i] = (int64_t)cps[i].ir.size();
   }
   const int64_t per = idx->lines * R * 2;
+  if (flags & GPJSON_QUERY_GATHER) {
+    // values come from the bytes gathered on the device: no map of the file
+    std::unique_ptr<gpjson_gathered> g(new gpjson_gathered());
+    rc = run_gather(ctx, idx, irs.data(), lens.data(), npaths, R, g.get());
+    gpjson_index_free(idx);
+    gpjson_buf_free(buf);
+    if (rc) return rc;
+    for (int i = 0; i < npaths; i++)
+      res->spans[i].assign(g->file_spans.begin() + per * i, g->file_spans.begin() + per * (i + 1));
+    res->gathered = g.release();
+    *out = res.release();
+    return GPJSON_OK;
+  }
   std::vector<int64_t> all((size_t)(per * npaths));
   rc = gpjson_query_run(ctx, idx, irs.data(), lens.data(), npaths, R, all.data(), (int64_t)all.size());
   gpjson_index_free(idx);
@@ -719,6 +805,14 @@ int gpjson_result_value(const gpjson_result* res, int64_t q, int64_t line, int k
     fail(GPJSON_ERR_INVALID_ARG, "index out of range");
     return -1;
   }
+  if (res->gathered) {
+    const gpjson_gathered* g = res->gathered;
+    const int64_t* gs = g->spans.data() + ((q * res->lines + line) * res->R + k) * 2;
+    if (gs[0] == -1) return 0;
+    if (ptr) *ptr = (const char*)(g->bytes.data() + gs[0]);
+    if (len) *len = gs[1] - gs[0];
+    return 1;
+  }
   const int64_t* s = res->spans[q].data() + (line * res->R + k) * 2;
   if (s[0] == -1) return 0;
   int64_t a = std::min(s[0], res->map_len);
@@ -731,8 +825,52 @@ int gpjson_result_value(const gpjson_result* res, int64_t q, int64_t line, int k
 void gpjson_result_free(gpjson_result* res) {
   if (!res) return;
   if (res->map) munmap((void*)res->map, (size_t)res->map_len);
+  delete res->gathered;
   delete res;
 }
+
+int gpjson_query_gather(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs, const int64_t* ir_lens,
+                        int nq, int R, gpjson_gathered** out) {
+  if (!ctx || !idx || !irs || !ir_lens || !out) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  if (nq < 1 || nq > GPJSON_MAX_QUERIES) return fail(GPJSON_ERR_INVALID_ARG, "nq must be in [1, 255]");
+  if (R < 1) return fail(GPJSON_ERR_INVALID_ARG, "R must be >= 1");
+  GJ_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<gpjson_gathered> g(new gpjson_gathered());
+  const int rc = run_gather(ctx, idx, irs, ir_lens, nq, R, g.get());
+  if (rc) return rc;
+  *out = g.release();
+  return GPJSON_OK;
+}
+
+int64_t gpjson_gathered_lines(const gpjson_gathered* g) { return g ? g->lines : -1; }
+int gpjson_gathered_per_line(const gpjson_gathered* g) { return g ? g->R : -1; }
+
+int64_t gpjson_gathered_bytes(const gpjson_gathered* g, const uint8_t** bytes) {
+  if (!g) return -1;
+  if (bytes) *bytes = g->bytes.data();
+  return (int64_t)g->bytes.size();
+}
+
+const int64_t* gpjson_gathered_spans(const gpjson_gathered* g, int64_t q) {
+  if (!g || q < 0 || q >= g->nq) return nullptr;
+  return g->spans.data() + q * g->lines * g->R * 2;
+}
+
+const int64_t* gpjson_gathered_file_spans(const gpjson_gathered* g, int64_t q) {
+  if (!g || q < 0 || q >= g->nq) return nullptr;
+  return g->file_spans.data() + q * g->lines * g->R * 2;
+}
+
+int gpjson_gathered_timing(const gpjson_gathered* g, double* query_ms, double* gather_ms, double* copy_ms) {
+  if (!g) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  if (query_ms) *query_ms = g->query_ms;
+  if (gather_ms) *gather_ms = g->gather_ms;
+  if (copy_ms) *copy_ms = g->copy_ms;
+  return GPJSON_OK;
+}
+
+void gpjson_gathered_free(gpjson_gathered* g) { delete g; }
 
 }  // extern "C"
 

@@ -26,6 +26,13 @@ hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, 
 int chain_depth(const uint8_t* ir, int64_t len);  // D, or 0 if ir is not a member chain
 hipError_t chain_launch(const QueryLaunch& L, int block, int group, int grid, hipStream_t stream);
 bool query_config_supported(int block, int group);
+// gather.hip: result bytes gathered into one buffer (phase 1: sizes and the
+// total; phase 2: offsets and bytes).  block_sums holds gather_blocks(n) int64.
+int64_t gather_blocks(int64_t n);
+hipError_t gather_phase1(const int64_t* spans, int64_t n, int64_t N, int64_t* block_sums, int64_t* total_dev,
+                         hipStream_t stream);
+hipError_t gather_phase2(const uint8_t* F, int64_t N, const int64_t* spans, int64_t n, const int64_t* block_sums,
+                         int64_t* out_spans, uint8_t* out_bytes, int copy_grid, hipStream_t stream);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
 hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
                       int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream);
@@ -89,6 +96,7 @@ struct gpjson_ctx {
   gj::DevBuf ticket;      // index ticket + query counter + line counts
   gj::DevBuf jobs;        // device job arrays
   gj::DevBuf irbuf;       // IR bytes of a query launch
+  gj::DevBuf gather[4];   // result gather: file spans, block sums + total, gathered spans, bytes
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_size = 0;
   hipEvent_t pin_ev[2] = {nullptr, nullptr};

@@ -62,7 +62,13 @@ enum {
 enum { GPJSON_BUILD_VERIFY = 1 /* also materialize escape and quote bitmaps */ };
 
 /* Flags of gpjson_query_file (QueryOptions, functions/QueryOptions.java:8-14). */
-enum { GPJSON_QUERY_DISABLE_FALLBACK = 1 };
+enum {
+  GPJSON_QUERY_DISABLE_FALLBACK = 1,
+  /* (no reference counterpart) gather the result bytes on the device and copy
+     them back once, instead of decoding each element from a second map of
+     the file (QueryFunction.java:247-259): see gpjson_query_gather */
+  GPJSON_QUERY_GATHER = 2
+};
 
 #define GPJSON_MAX_LEVELS 64   /* reference: 16, stages/LeveledBitmapsIndex.java:16 */
 #define GPJSON_MAX_QUERIES 255 /* functions/QueryFunction.java:74-77 */
@@ -158,6 +164,31 @@ const int64_t *gpjson_result_spans(const gpjson_result *res, int64_t q);
  * result/ResultItem.java:32-52): returns 1 and sets ptr and len, or 0 for null. */
 int gpjson_result_value(const gpjson_result *res, int64_t q, int64_t line, int k, const char **ptr, int64_t *len);
 void gpjson_result_free(gpjson_result *res);
+
+/* ---- device-side result gather ------------------------------------------- */
+
+typedef struct gpjson_gathered gpjson_gathered;
+
+/* (no reference counterpart: replaces the span copy of QueryFunction.java:240
+ * and the per-element decoding from a second map of the file,
+ * QueryFunction.java:247-259 / result/ResultItem.java:32-52.)  Runs nq queries
+ * over idx, then gathers the bytes of every result element [q][line][k] into
+ * one buffer on the device (in element order) and copies it and the spans back
+ * once.  *out: per query the file spans int64[lines][R][2] (as gpjson_query_run
+ * returns them) and the gathered spans int64[lines][R][2], offsets into the
+ * gathered bytes (-1, -1 where there is no result). */
+int gpjson_query_gather(gpjson_ctx *ctx, const gpjson_index *idx, const uint8_t *const *irs,
+                        const int64_t *ir_lens, int nq, int R, gpjson_gathered **out);
+int64_t gpjson_gathered_lines(const gpjson_gathered *g);
+int gpjson_gathered_per_line(const gpjson_gathered *g);
+/* total gathered bytes; *bytes (optional) points at them */
+int64_t gpjson_gathered_bytes(const gpjson_gathered *g, const uint8_t **bytes);
+const int64_t *gpjson_gathered_spans(const gpjson_gathered *g, int64_t q);
+const int64_t *gpjson_gathered_file_spans(const gpjson_gathered *g, int64_t q);
+/* milliseconds on the device: the interpreter launch, the gather kernels, and
+ * the copy of spans and bytes to the host */
+int gpjson_gathered_timing(const gpjson_gathered *g, double *query_ms, double *gather_ms, double *copy_ms);
+void gpjson_gathered_free(gpjson_gathered *g);
 
 /* ---- prepared plain path -------------------------------------------------- */
 

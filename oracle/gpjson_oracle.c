@@ -22,7 +22,12 @@
  *     buffer once for all queries, J/functions/QueryFunction.java:197);
  *   - MOVE_TO_INDEX on a value whose first non-blank byte is not '[' ends the
  *     line's evaluation (reference: device assert, K/find_value.cu:151-154);
- *   - blank skipping stops at n (reference reads past the end).
+ *   - blank skipping stops at n (reference reads past the end);
+ *   - MOVE_UP out of a level whose end was never found (its end is -1: the
+ *     level-end search of MOVE_DOWN skips the byte after the cursor, so
+ *     `[0:k]` over an empty array misses the ']') ends the line's evaluation
+ *     (reference: the cursor becomes -1 and the walk restarts at byte 0 of the
+ *     file, reading bitmap word -1 and f[-1]; K/find_value.cu:81-88).
  *
  * Parity pins (tests/test_oracle.py): the reference's fixtures and closed forms
  * (T/stages/StringIndexTest.java:109-168, NewlineIndexTest.java:81-108,
@@ -181,7 +186,7 @@ static void find_value_lines(const uint8_t *f, int64_t n, const int64_t *nl, int
                     j = le[lvl];
                     le[lvl] = -1;
                     lvl--;
-                    if (lvl < 0) goto line_done;
+                    if (lvl < 0 || j < 0) goto line_done;
                     break;
                 case OP_DOWN:
                     lvl++;

@@ -150,6 +150,18 @@ def spans_equal(ctx, data: bytes, paths, tag=""):
     return got
 
 
+def test_ranges_over_empty_arrays(ctx):
+    """`[0:k]` over empty arrays (tweets' "urls" is always [], "hashtags" often):
+    the level-end search skips the ']', STORE writes an end of -1, and MOVE_UP out
+    of that level ends the line -- in the oracle and on the GPU alike."""
+    from gpjson_amd import datagen
+    data = (datagen.generate("tweets", 71, 2 << 20).tobytes()
+            + b'\n{"a": []}\n{"a": [1]}\n{"a": [ ]}\n{"a": [], "b": [[], [1, 2]]}')
+    got = spans_equal(ctx, data, ["$.entities.hashtags[0:2]", "$.entities.urls[0:3]", "$.a[0:2]",
+                                  "$.entities.hashtags[0:2].text", "$.b[0:2][0:2]"], "ranges over empty arrays")
+    assert (got[1][..., 1] == -1).any()  # the unfound level ends were exercised
+
+
 KATS = [
     ("$.user.lang", ['"nl"']),
     ("$.user.id", ["8723"]),

@@ -209,6 +209,24 @@ def test_query_not_found_in_current_level():
     assert sum(v is not None for v in vals) == 0
 
 
+def test_range_over_empty_array_hand_trace():
+    """`$.a[0:2]` on `{"a": []}`, traced by hand through K/find_value.cu:21-263:
+    KEY a matches the ':' at 4, the cursor moves to 5; MOVE_DOWN finds the end
+    of a's value at the '}' (8); MOVE_TO_INDEX 0 skips the blank to the '[' at 6
+    and steps to 7; MOVE_DOWN searches level 1 in (8, 8), finds nothing (-1);
+    STORE writes (7, -1).  MOVE_UP then makes the cursor -1, which ends the
+    line here (the reference would restart at byte 0 of the file)."""
+    from gpjson_amd.api import compile_path
+    data = b'{"a": []}'
+    cp = compile_path("$.a[0:2]")
+    nl, s, lb, spans = O.query(data, [cp.ir], cp.max_depth, cp.num_results)
+    assert spans[0].reshape(-1).tolist() == [7, -1, -1, -1]
+    # the same document with one element: both results from the first branch on
+    data = b'{"a": [1]}'
+    nl, s, lb, spans = O.query(data, [cp.ir], cp.max_depth, cp.num_results)
+    assert spans[0].reshape(-1).tolist() == [7, 8, -1, -1]
+
+
 def test_oracle_mutation_is_caught():
     """A planted mistake in the rule (treating every quote as unescaped) changes
     the escape_carry fixture's string bitmap, so the checks above can fail."""
