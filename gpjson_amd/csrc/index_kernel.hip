@@ -416,6 +416,9 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   __shared__ int s_w0_flip;
   __shared__ M4 s_w0_m;
   __shared__ Carry s_carry;
+  __shared__ Carry s_exact;        // carry-in from the lookback (speculative tiles)
+  __shared__ int s_fnl;            // sequence index of the tile's first word holding a '\n'
+  __shared__ int s_mis;            // the speculative carry-in was wrong
   // the masks every word needs at emission, parked here (field-major, so the
   // lanes of a wavefront touch consecutive 8-byte slots) while the scans and
   // the lookback run, so they occupy no registers there; s_lead: length of the
@@ -434,6 +437,7 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       s_ticket = atomicAdd(launch.ticket, 1u);
       s_w0 = -1;
       s_w0_flip = 0;
+      s_fnl = TILE_WORDS;
       unsigned t = s_ticket;
       int lo = 0, hi = (int)launch.njobs - 1;
       if (t < launch.total_tiles && launch.jobs) {
@@ -507,6 +511,10 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
         s_wc[r][kSlotQ][tid] = w.Q;
         s_wc[r][kSlotK][tid] = w.K;
         s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
+        if (!VERIFY) {
+          const unsigned long long nlw = __ballot(w.NL != 0);
+          if (nlw && lane == __builtin_ctzll(nlw)) atomicMin(&s_fnl, r * BLOCK + tid);
+        }
       }
     }
 
@@ -584,28 +592,60 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       }
     }
 
-    // ---- publish the tile aggregate and resolve the carry-in (wave 0) ----
+    // ---- carry-in: speculated at the first newline, or from the lookback ----
+    // In NDJSON every raw '\n' lies outside strings at depth -1 (one value per
+    // line).  A tile holding a newline whose first byte is neither '\\' nor '"'
+    // (so the escape carry into it changes none of its outputs) takes the
+    // carry-in that puts its first newline in that state, emits its indexes
+    // at once, and resolves its exact carry-in with the lookback afterwards:
+    // only the newline offsets need it.  A wrong guess (the input is not
+    // NDJSON there) re-emits the tile with the exact carry-in.
+    const bool spec = !VERIFY && lt > 0 && s_fnl < TILE_WORDS && s_lead[0][0] == 0;
+    TileAgg a;
+    a.esc = tile_esc;
+    a.m[0] = mtile;
+    a.m[1] = mtile;
+    if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
+    TileState* st0 = launch.states + J.tile0;
     if (wave == 0) {
-      TileState* st0 = launch.states + J.tile0;
-      TileAgg a;
-      a.esc = tile_esc;
-      a.m[0] = mtile;
-      a.m[1] = mtile;
-      if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
-      Carry cin;
       // every tile publishes its aggregate (tile 0's too: the last tile of
       // group 0 waits for the aggregates of all the group's tiles)
       if (lane == 0) publish_agg(st0 + lt, a);
-      if (lt == 0) {
-        cin = Carry{0, 0, -1, 1};
-        if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
-      } else {
-        cin = lookback_wave(st0, lt, lane, a);
+      if (!spec) {
+        Carry cin;
+        if (lt == 0) {
+          cin = Carry{0, 0, -1, 1};
+          if (lane == 0 && J.nl_cap > 0) J.NL[0] = 0;
+        } else {
+          cin = lookback_wave(st0, lt, lane, a);
+        }
+        if (lane == 0) {
+          publish_inc(st0 + lt, apply_agg(a, cin));
+          s_carry = cin;
+          if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
+        }
       }
-      if (lane == 0) {
-        publish_inc(st0 + lt, apply_agg(a, cin));
-        s_carry = cin;
-        if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
+    }
+    if (spec) {
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        if (r * BLOCK + tid == s_fnl) {
+          // state at the word's first byte that makes its first '\n' lie
+          // outside a string at depth -1
+          const int ld = s_lead[r][tid];
+          const int pf = ent[r] ? (ent[r] - 2) : 0;
+          uint64_t S = s_wc[r][kSlotS0][tid];
+          if (pf && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
+          const int b = ctz64(s_wc[r][kSlotNL][tid]);
+          const int sf = (int)((S >> b) & 1);
+          if (sf) S = ~S;
+          const uint64_t below = (1ull << b) - 1ull;
+          const int64_t df = -1 - (int64_t)(popc64(s_wc[r][kSlotO][tid] & ~S & below) -
+                                            popc64(s_wc[r][kSlotC][tid] & ~S & below));
+          const M4 mx = texcl[r];
+          const int sin = sf ^ mx.q;
+          s_carry = Carry{0, sin, df - (sin ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0), 0};
+        }
       }
     }
     __syncthreads();
@@ -615,19 +655,44 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
     const M4 w0m = s_w0_m;
 
     // ---- emit the indexes (lane t writes word t of each row: coalesced) ----
+    auto emit_tile = [&](const Carry& cx, bool fx) {
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        const int sidx = r * BLOCK + tid;
+        const int64_t w = word0 + sidx;
+        if (w >= J.W) continue;
+        M4 mx = texcl[r];
+        if (fx && sidx > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
+        const int p = ent[r] ? (ent[r] - 2) : cx.p;
+        const int s = cx.s ^ mx.q;
+        const int64_t d = cx.d + (cx.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
+        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d);
+      }
+    };
+    emit_tile(X, fix);
+    Carry XL = X;  // carry-in whose line count places the newline offsets
+    if (spec) {
+      if (wave == 0) {
+        const Carry cin = lookback_wave(st0, lt, lane, a);
+        if (lane == 0) {
+          publish_inc(st0 + lt, apply_agg(a, cin));
+          s_exact = cin;
+          s_mis = (cin.s != X.s) || (cin.d != X.d);
+          if (J.tile_lines) J.tile_lines[lt] = cin.lines;
+        }
+      }
+      __syncthreads();
+      XL = s_exact;
+      // a tile whose first byte is neither '\\' nor '"' needs no escape fix
+      if (s_mis) emit_tile(XL, false);
+    }
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
       const int sidx = r * BLOCK + tid;
       const int64_t w = word0 + sidx;
       if (w >= J.W) continue;
-      M4 mx = texcl[r];
-      if (fix && sidx > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
-      const int p = ent[r] ? (ent[r] - 2) : X.p;
-      const int s = X.s ^ mx.q;
-      const int64_t d = X.d + (X.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-      emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d);
       uint64_t nlm = s_wc[r][kSlotNL][tid];
-      int64_t idx = X.lines + mx.nl;
+      int64_t idx = XL.lines + texcl[r].nl;
       while (nlm) {
         const int b = ctz64(nlm);
         if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
