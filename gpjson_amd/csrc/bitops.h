@@ -87,10 +87,6 @@ struct WordClass {
   uint64_t K;   // ':'
 };
 
-// Per byte of t: bit 7 set iff the byte's low seven bits equal k (k < 0x80).
-// lo7 = t & 0x7f7f7f7f.  Byte-wise 0x80 - (x ^ k) with no cross-byte borrow.
-GJ_HD uint32_t eq7(uint32_t lo7, uint32_t k4) { return (lo7 ^ ~k4) - 0x7F7F7F7Fu; }
-
 #if defined(__HIP_DEVICE_COMPILE__)
 GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
   return __builtin_amdgcn_perm(s0, s1, sel);
@@ -108,42 +104,78 @@ GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
 }
 #endif
 
-// Classifies 32 bytes (d[0..7], byte b of d[i] = position 4i+b) into 32-bit
-// masks.  The bytes are first transposed so that dword j holds positions j,
-// j+8, j+16, j+24; a class's per-byte marker (bit 7) then lands on its natural
-// bit position with one right shift by 7-j.
+// 8x8 bit-matrix transpose of the 8 bytes held in (lo, hi): bit c of byte r
+// moves to bit r of byte c, so byte c afterwards holds bit c of each of the 8
+// input bytes.  Three delta swaps (2x2, then 4x4, then 8x8 blocks; Warren,
+// Hacker's Delight, section 7-3); the last one exchanges nibbles between the
+// two halves.
+GJ_HD void transpose8x8(uint32_t& lo, uint32_t& hi) {
+  uint32_t t;
+  t = (lo ^ (lo >> 7)) & 0x00AA00AAu;
+  lo ^= t ^ (t << 7);
+  t = (hi ^ (hi >> 7)) & 0x00AA00AAu;
+  hi ^= t ^ (t << 7);
+  t = (lo ^ (lo >> 14)) & 0x0000CCCCu;
+  lo ^= t ^ (t << 14);
+  t = (hi ^ (hi >> 14)) & 0x0000CCCCu;
+  hi ^= t ^ (t << 14);
+  t = ((lo >> 4) ^ hi) & 0x0F0F0F0Fu;
+  hi ^= t;
+  lo ^= t << 4;
+}
+
+// 4x4 byte transpose: out[k] = byte k of a0, a1, a2, a3 (in that order).
+GJ_HD void transpose4x4_bytes(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t out[4]) {
+  const uint32_t u0 = perm_b32(a1, a0, 0x05010400u);  // a0.0 a1.0 a0.1 a1.1
+  const uint32_t u1 = perm_b32(a1, a0, 0x07030602u);  // a0.2 a1.2 a0.3 a1.3
+  const uint32_t u2 = perm_b32(a3, a2, 0x05010400u);
+  const uint32_t u3 = perm_b32(a3, a2, 0x07030602u);
+  out[0] = perm_b32(u2, u0, 0x05040100u);
+  out[1] = perm_b32(u2, u0, 0x07060302u);
+  out[2] = perm_b32(u3, u1, 0x05040100u);
+  out[3] = perm_b32(u3, u1, 0x07060302u);
+}
+
+// Bit planes of 32 bytes (d[0..7], byte b of d[i] = position 4i+b): bit j of
+// pl[k] is bit k of byte j.  Each 8-byte group is bit-transposed, then byte k
+// of the four groups' results is gathered into plane k.
+GJ_HD void bit_planes32(const uint32_t d[8], uint32_t pl[8]) {
+  uint32_t lo[4], hi[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    lo[g] = d[2 * g];
+    hi[g] = d[2 * g + 1];
+    transpose8x8(lo[g], hi[g]);
+  }
+  transpose4x4_bytes(lo[0], lo[1], lo[2], lo[3], pl);
+  transpose4x4_bytes(hi[0], hi[1], hi[2], hi[3], pl + 4);
+}
+
+// Classifies 32 bytes into 32-bit masks from their bit planes: each class is a
+// conjunction over the 8 bits of a byte, split into high-nibble and low-nibble
+// terms shared between classes.  Bytes from 0x80 up (bit 7 set) match nothing.
+//   '\\' 0x5C  '"' 0x22  '\n' 0x0A  '{' 0x7B '[' 0x5B  '}' 0x7D ']' 0x5D  ':' 0x3A  ',' 0x2C
 GJ_HD void classify32(const uint32_t d[8], uint32_t& B, uint32_t& Q, uint32_t& NL, uint32_t& O,
                       uint32_t& C, uint32_t& P, uint32_t& K) {
-  uint32_t t[8];
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    uint32_t a0 = d[h], a1 = d[h + 2], a2 = d[h + 4], a3 = d[h + 6];
-    uint32_t u0 = perm_b32(a1, a0, 0x05010400u);
-    uint32_t u1 = perm_b32(a1, a0, 0x07030602u);
-    uint32_t u2 = perm_b32(a3, a2, 0x05010400u);
-    uint32_t u3 = perm_b32(a3, a2, 0x07030602u);
-    t[4 * h + 0] = perm_b32(u2, u0, 0x05040100u);
-    t[4 * h + 1] = perm_b32(u2, u0, 0x07060302u);
-    t[4 * h + 2] = perm_b32(u3, u1, 0x05040100u);
-    t[4 * h + 3] = perm_b32(u3, u1, 0x07060302u);
-  }
-  uint32_t b = 0, q = 0, nl = 0, o = 0, c = 0, p = 0, k = 0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    uint32_t x = t[j];
-    uint32_t lo7 = x & 0x7F7F7F7Fu;
-    uint32_t lo7c = lo7 | 0x20202020u;
-    uint32_t nv = (~x & 0x80808080u) >> (7 - j);
-    b |= (eq7(lo7, 0x5C5C5C5Cu) >> (7 - j)) & nv;
-    q |= (eq7(lo7, 0x22222222u) >> (7 - j)) & nv;
-    nl |= (eq7(lo7, 0x0A0A0A0Au) >> (7 - j)) & nv;
-    o |= (eq7(lo7c, 0x7B7B7B7Bu) >> (7 - j)) & nv;
-    c |= (eq7(lo7c, 0x7D7D7D7Du) >> (7 - j)) & nv;
-    const uint32_t colon = eq7(lo7, 0x3A3A3A3Au);
-    p |= ((colon | eq7(lo7, 0x2C2C2C2Cu)) >> (7 - j)) & nv;
-    k |= (colon >> (7 - j)) & nv;
-  }
-  B = b; Q = q; NL = nl; O = o; C = c; P = p; K = k;
+  uint32_t p[8];
+  bit_planes32(d, p);
+  const uint32_t a = ~p[7] & ~p[6];
+  const uint32_t h0 = a & ~p[5] & ~p[4];          // 0x0_
+  const uint32_t h2 = a & p[5] & ~p[4];           // 0x2_
+  const uint32_t h3 = a & p[5] & p[4];            // 0x3_
+  const uint32_t h57 = ~p[7] & p[6] & p[4];       // 0x5_ or 0x7_
+  const uint32_t h5 = h57 & ~p[5];                // 0x5_
+  const uint32_t l1 = p[3] & ~p[2] & p[1];        // 0x_A or 0x_B
+  const uint32_t l2 = p[3] & p[2] & ~p[1];        // 0x_C or 0x_D
+  const uint32_t lA = l1 & ~p[0], lB = l1 & p[0], lC = l2 & ~p[0], lD = l2 & p[0];
+  const uint32_t l_2 = ~p[3] & ~p[2] & p[1] & ~p[0];  // 0x_2
+  B = h5 & lC;
+  Q = h2 & l_2;
+  NL = h0 & lA;
+  O = h57 & lB;
+  C = h57 & lD;
+  K = h3 & lA;
+  P = K | (h2 & lC);
 }
 
 // Classifies one 64-byte word given as 16 little-endian dwords.
