@@ -238,4 +238,28 @@ GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo
   return d;
 }
 
+// The same bits, one bracket at a time, with no dependence between brackets
+// (so the brackets of many words can be spread over the lanes of a wavefront).
+// The bracket at bit t sits at level d + (opens at or before t) - (closes
+// before t); the separators after it, up to the next bracket, sit at the depth
+// after it (that level for an open, one less for a close).  Returns those
+// separators.
+GJ_HD uint64_t bracket_step(uint64_t O, uint64_t C, uint64_t P, int64_t d, int t, int64_t* lvl, int64_t* after) {
+  const uint64_t bit = 1ull << t;
+  const uint64_t le = bit | (bit - 1);
+  const int64_t l = d + popc64(O & le) - popc64(C & (bit - 1));
+  *lvl = l;
+  *after = (O & bit) ? l : l - 1;
+  const uint64_t rest = (O | C) & ~le;
+  const uint64_t seg = rest ? (((rest & (0 - rest)) - 1) & ~le) : ~le;
+  return P & seg;
+}
+
+// Separators before the word's first bracket (all of them if it has none):
+// they sit at depth d.
+GJ_HD uint64_t leading_seps(uint64_t O, uint64_t C, uint64_t P) {
+  const uint64_t T = O | C;
+  return T ? (P & ((T & (0 - T)) - 1)) : P;
+}
+
 }  // namespace gj

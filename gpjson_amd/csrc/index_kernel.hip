@@ -340,21 +340,114 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane, const TileA
 }  // namespace
 
 // Per-word masks parked in LDS between classification and emission: the
-// in-string mask for escape carry 0 outside a string, newlines, open, close,
-// ':' ',', quotes and ':'; verify mode also keeps the backslashes.
-constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotK = 6, kSlotB = 7;
+// in-string mask for escape carry 0 outside a string, newlines, open, close
+// and ':' ','; verify mode also keeps the quotes and backslashes.  (The colon
+// and raw-quote words of the records need no carry: classification writes
+// them.)
+constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotB = 6;
+
+// Leveled bitmaps with the brackets spread over the wavefront.  Lane-per-word bracket loops run as long as the word
+// with the most brackets (about 10x the mean on tweets); here every bracket of
+// the wavefront's 64 words is one work item.  Per wavefront: the word's level
+// words (NLV x 64) and the work list (kBracketCap items of word lane << 6 |
+// bit); a row with more brackets than that takes the per-word loop.
+constexpr int kBracketCap = 128;
+
+// Inclusive prefix sum of an int over the wavefront (DPP, as wave_scan_m).
+__device__ __forceinline__ int wave_scan_int(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// LDS accesses of one wavefront made visible to its other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Level words [lo, lo + NLV) of this lane's word (O, C, P outside strings, d
+// the depth before it; act false: no word), every lane of the wavefront
+// taking part.  lv: this wavefront's NLV x 64 level words; items: its list.
+template <int NLV>
+__device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L,
+                                              bool act, int lane, uint64_t (&lv)[NLV][64],
+                                              uint16_t (&items)[kBracketCap], uint64_t out[NLV]) {
+  const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
+  const bool hit = act && !(dmax < lo || dmin >= lo + NLV) && (O | C | P) != 0;
+  const uint64_t T = hit ? (O | C) : 0;
+  const int n = popc64(T);
+  const int incl = wave_scan_int(n);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  if (total > kBracketCap) {  // wave-uniform: the per-word loop
+#pragma unroll
+    for (int i = 0; i < NLV; i++) out[i] = 0;
+    if (hit) leveled_word<NLV>(O, C, P, d, lo, L, out);
+    return;
+  }
+  // own level words: zero, then the separators before the first bracket
+#pragma unroll
+  for (int i = 0; i < NLV; i++) lv[i][lane] = 0;
+  if (hit) {
+    const int64_t k = d - lo;
+    if (k >= 0 && k < NLV && d < L) lv[k][lane] = leading_seps(O, C, P);
+  }
+  // the work list: this word's brackets at [incl - n, incl)
+  {
+    int j = incl - n;
+    uint64_t Tm = T;
+    while (Tm) {
+      items[j++] = (uint16_t)((lane << 6) | ctz64(Tm));
+      Tm &= Tm - 1;
+    }
+  }
+  wave_lds_sync();
+  for (int u0 = 0; u0 < total; u0 += 64) {
+    const int u = u0 + lane;
+    const int e = u < total ? items[u] : 0;
+    const int o = e >> 6, t = e & 63;
+    const uint64_t Oo = shfl_u64(O, o), Co = shfl_u64(C, o), Po = shfl_u64(P, o);
+    const int64_t dd = (int64_t)shfl_u64((uint64_t)d, o);
+    if (u < total) {
+      int64_t lvl, after;
+      const uint64_t seps = bracket_step(Oo, Co, Po, dd, t, &lvl, &after);
+      const int64_t k = lvl - lo;
+      if (k >= 0 && k < NLV && lvl < L) atomicOr(&lv[k][o], 1ull << t);
+      const int64_t k2 = after - lo;
+      if (seps && k2 >= 0 && k2 < NLV && after < L) atomicOr(&lv[k2][o], seps);
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < NLV; i++) out[i] = lv[i][lane];
+  wave_lds_sync();  // the next row overwrites these words
+}
 
 // The index words of word w -- string index, raw colon and quote bitmaps,
 // verify-mode escape/quote words, leveled bitmaps -- from its masks (row r,
 // lane tid of `wc`), the escape carry p, string state s and depth d before its
 // first byte.
-template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK>
+// act false: a lane past the last word (it writes nothing, but takes part in
+// the wavefront's work when the brackets are spread).
+template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK, int NLVS>
 __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uint64_t (&wc)[ROWS][NSLOT][BLOCK],
                                           const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
-                                          int64_t d) {
+                                          int64_t d, bool act, uint64_t (&lv)[NLVS][64],
+                                          uint16_t (&items)[kBracketCap]) {
+  constexpr bool kSpread = true;
   uint64_t valid = ~0ull;
   const int64_t rem = J.N - w * 64;
-  if (rem < 64) valid = (1ull << rem) - 1ull;
+  if (rem < 64) valid = rem > 0 ? (1ull << rem) - 1ull : 0;
   // in-string mask from the one for escape carry 0 outside a string: carry 1
   // escapes the leading backslash run's first byte instead of its second, which
   // toggles only a quote right after the run, and so every bit from there on
@@ -362,13 +455,9 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
   uint64_t S = wc[r][kSlotS0][tid];
   if (p && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
   S = (s ? ~S : S) & valid;
-  // the word's record: string, colon and raw-quote words (bytes past the end
-  // of the file were loaded as zeros, which are neither ':' nor '"')
-  uint64_t* rec = J.rec + w * kRecWords;
-  rec[0] = S;
-  rec[1] = wc[r][kSlotK][tid];
-  rec[2] = wc[r][kSlotQ][tid];
-  if (VERIFY) {
+  // the string word of the word's record
+  if (act) J.rec[w * kRecWords] = S;
+  if (VERIFY && act) {
     const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
     if (J.E) J.E[w] = E & valid;
     if (J.Q) J.Q[w] = wc[r][kSlotQ][tid] & ~E & valid;
@@ -377,15 +466,19 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
   const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
   for (int lo = 0; lo < J.L; lo += NLV) {
     uint64_t out[NLV];
-    if (dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
-#pragma unroll
-      for (int i = 0; i < NLV; i++) out[i] = 0;
+    if constexpr (kSpread) {
+      levels_spread<NLV>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
     } else {
-      leveled_word<NLV>(O, C, P, d, lo, J.L, out);
+      if (!act || dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
+#pragma unroll
+        for (int i = 0; i < NLV; i++) out[i] = 0;
+      } else {
+        leveled_word<NLV>(O, C, P, d, lo, J.L, out);
+      }
     }
 #pragma unroll
     for (int i = 0; i < NLV; i++)
-      if (lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+      if (act && lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
     // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
     // start at a multiple of 64 (lanes past the file end are inactive, and
     // lane 0 holds the lowest word, so it is active whenever any lane is)
@@ -394,7 +487,7 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
       for (int i = 0; i < NLV; i++) {
         if (lo + i < J.L) {
           const unsigned long long m = __ballot(out[i] != 0);
-          if ((tid & 63) == 0) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
+          if ((tid & 63) == 0 && act) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
         }
       }
     }
@@ -402,7 +495,8 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
 }
 
 template <int BLOCK, int ROWS, int NLV, bool VERIFY>
-__global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
+// 3 wavefronts per SIMD: the resident set the LDS of the default tile allows.
+__global__ __launch_bounds__(BLOCK, 3) void index_kernel(IndexLaunch launch) {
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
   static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
@@ -423,8 +517,12 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
   // lanes of a wavefront touch consecutive 8-byte slots) while the scans and
   // the lookback run, so they occupy no registers there; s_lead: length of the
   // word's leading backslash run (64: all backslashes) | 0x80 if a quote follows it
-  constexpr int NSLOT = VERIFY ? 8 : 7;
+  constexpr int NSLOT = VERIFY ? 7 : 5;
   __shared__ uint64_t s_wc[ROWS][NSLOT][BLOCK];
+  // per wavefront: level words and work list of the spread leveled bitmaps
+  constexpr int NLVS = NLV;
+  __shared__ uint64_t s_lv[NW][NLVS][64];
+  __shared__ uint16_t s_items[NW][kBracketCap];
   __shared__ uint8_t s_lead[ROWS][BLOCK];
 
   const int tid = threadIdx.x;
@@ -507,9 +605,17 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
         s_wc[r][kSlotO][tid] = w.O;
         s_wc[r][kSlotC][tid] = w.C;
         s_wc[r][kSlotP][tid] = w.P;
-        if (VERIFY) s_wc[r][kSlotB][tid] = w.B;
-        s_wc[r][kSlotQ][tid] = w.Q;
-        s_wc[r][kSlotK][tid] = w.K;
+        if (VERIFY) {
+          s_wc[r][kSlotB][tid] = w.B;
+          s_wc[r][kSlotQ][tid] = w.Q;
+        }
+        // the colon and raw-quote words of the record (bytes past the end of
+        // the file were loaded as zeros, which are neither ':' nor '"')
+        const int64_t wr = word0 + (int64_t)r * BLOCK + tid;
+        if (wr < J.W) {
+          J.rec[wr * kRecWords + 1] = w.K;
+          J.rec[wr * kRecWords + 2] = w.Q;
+        }
         s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
         if (!VERIFY) {
           const unsigned long long nlw = __ballot(w.NL != 0);
@@ -660,13 +766,12 @@ __global__ __launch_bounds__(BLOCK) void index_kernel(IndexLaunch launch) {
       for (int r = 0; r < ROWS; r++) {
         const int sidx = r * BLOCK + tid;
         const int64_t w = word0 + sidx;
-        if (w >= J.W) continue;
         M4 mx = texcl[r];
         if (fx && sidx > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
         const int p = ent[r] ? (ent[r] - 2) : cx.p;
         const int s = cx.s ^ mx.q;
         const int64_t d = cx.d + (cx.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d);
+        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, s_lv[wave], s_items[wave]);
       }
     };
     emit_tile(X, fix);

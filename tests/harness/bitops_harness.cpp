@@ -17,4 +17,23 @@ void bh_classify64(const uint8_t* bytes, uint64_t* out) {
 int64_t bh_leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L, uint64_t* out) {
   return gj::leveled_word<8>(O, C, P, d, lo, L, out);
 }
+// leveled bits assembled bracket by bracket (bracket_step, leading_seps), in
+// the order the lanes of a wavefront would add them
+void bh_leveled_by_brackets(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L, uint64_t* out) {
+  for (int k = 0; k < 8; k++) out[k] = 0;
+  auto put = [&](int64_t level, uint64_t bits) {
+    const int64_t k = level - lo;
+    if (k >= 0 && k < 8 && level < L) out[k] |= bits;
+  };
+  put(d, gj::leading_seps(O, C, P));
+  uint64_t T = O | C;
+  while (T) {
+    const int t = __builtin_ctzll(T);
+    int64_t lvl, after;
+    const uint64_t seps = gj::bracket_step(O, C, P, d, t, &lvl, &after);
+    put(lvl, 1ull << t);
+    put(after, seps);
+    T &= T - 1;
+  }
+}
 }

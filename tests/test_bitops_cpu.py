@@ -34,6 +34,9 @@ def lib(tmp_path_factory):
     L.bh_classify64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.bh_leveled_word.restype = ctypes.c_int64
     L.bh_leveled_word.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.bh_leveled_by_brackets.restype = None
+    L.bh_leveled_by_brackets.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                                                   ctypes.c_void_p]
     return L
 
 
@@ -159,3 +162,23 @@ def test_leveled_word(lib):
         ref, rd = leveled_ref(O, C, P, d, lo, L)
         assert dd == rd
         assert [int(x) for x in out] == ref
+
+
+def test_leveled_by_brackets(lib):
+    """Bracket-at-a-time assembly (each bracket's level from popcounts, the
+    separators after it up to the next bracket) equals the per-byte rule."""
+    rng = np.random.default_rng(5)
+    out = np.zeros(8, dtype=np.uint64)
+    for it in range(20000):
+        # dense and sparse bracket mixes, with runs that nest several levels
+        probs = [0.5, 0.1, 0.15, 0.15, 0.1] if it % 2 else [0.8, 0.05, 0.05, 0.05, 0.05]
+        kinds = rng.choice(5, 64, p=probs)  # 0,1: none, 2: open, 3: close, 4: sep
+        O = sum(1 << i for i in range(64) if kinds[i] == 2)
+        C = sum(1 << i for i in range(64) if kinds[i] == 3)
+        P = sum(1 << i for i in range(64) if kinds[i] == 4)
+        d = int(rng.integers(-3, 12))
+        lo = int(rng.choice([0, 2, 8]))
+        L = int(rng.integers(1, 17))
+        lib.bh_leveled_by_brackets(O, C, P, d, lo, L, out.ctypes.data)
+        ref, _ = leveled_ref(O, C, P, d, lo, L)
+        assert [int(x) for x in out] == ref, (it, hex(O), hex(C), hex(P), d, lo, L)
