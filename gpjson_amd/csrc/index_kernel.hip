@@ -340,18 +340,20 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane, const TileA
 }  // namespace
 
 // Per-word masks parked in LDS between classification and emission: the
-// in-string mask for escape carry 0 outside a string, newlines, open, close
-// and ':' ','; verify mode also keeps the quotes and backslashes.  (The colon
-// and raw-quote words of the records need no carry: classification writes
-// them.)
-constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotB = 6;
+// in-string mask for escape carry 0 outside a string, newlines, open, close,
+// ':' ',', quotes and ':'; verify mode also keeps the backslashes.  (The colon
+// and quote words need no carry, but writing them apart from the string word
+// of the same 24-byte record sends many record lines to memory twice.)
+constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotK = 6, kSlotB = 7;
 
 // Leveled bitmaps with the brackets spread over the wavefront.  Lane-per-word bracket loops run as long as the word
 // with the most brackets (about 10x the mean on tweets); here every bracket of
 // the wavefront's 64 words is one work item.  Per wavefront: the word's level
-// words (NLV x 64) and the work list (kBracketCap items of word lane << 6 |
-// bit); a row with more brackets than that takes the per-word loop.
+// words (up to kSpreadLevels x 64) and the work list (kBracketCap items of
+// word lane << 6 | bit); a row with more brackets than that takes the
+// per-word loop.
 constexpr int kBracketCap = 128;
+constexpr int kSpreadLevels = 4;  // levels per pass over the brackets
 
 // Inclusive prefix sum of an int over the wavefront (DPP, as wave_scan_m).
 __device__ __forceinline__ int wave_scan_int(int x) {
@@ -444,7 +446,6 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
                                           const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
                                           int64_t d, bool act, uint64_t (&lv)[NLVS][64],
                                           uint16_t (&items)[kBracketCap]) {
-  constexpr bool kSpread = true;
   uint64_t valid = ~0ull;
   const int64_t rem = J.N - w * 64;
   if (rem < 64) valid = rem > 0 ? (1ull << rem) - 1ull : 0;
@@ -455,36 +456,32 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
   uint64_t S = wc[r][kSlotS0][tid];
   if (p && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
   S = (s ? ~S : S) & valid;
-  // the string word of the word's record
-  if (act) J.rec[w * kRecWords] = S;
+  // the word's record: string, colon and raw-quote words (bytes past the end
+  // of the file were loaded as zeros, which are neither ':' nor '"')
+  if (act) {
+    uint64_t* rec = J.rec + w * kRecWords;
+    rec[0] = S;
+    rec[1] = wc[r][kSlotK][tid];
+    rec[2] = wc[r][kSlotQ][tid];
+  }
   if (VERIFY && act) {
     const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
     if (J.E) J.E[w] = E & valid;
     if (J.Q) J.Q[w] = wc[r][kSlotQ][tid] & ~E & valid;
   }
   const uint64_t O = wc[r][kSlotO][tid] & ~S, C = wc[r][kSlotC][tid] & ~S, P = wc[r][kSlotP][tid] & ~S;
-  const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
-  for (int lo = 0; lo < J.L; lo += NLV) {
-    uint64_t out[NLV];
-    if constexpr (kSpread) {
-      levels_spread<NLV>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
-    } else {
-      if (!act || dmax < lo || dmin >= lo + NLV || (O | C | P) == 0) {
+  for (int lo = 0; lo < J.L; lo += NLVS) {
+    uint64_t out[NLVS];
+    levels_spread<NLVS>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
 #pragma unroll
-        for (int i = 0; i < NLV; i++) out[i] = 0;
-      } else {
-        leveled_word<NLV>(O, C, P, d, lo, J.L, out);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NLV; i++)
+    for (int i = 0; i < NLVS; i++)
       if (act && lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
     // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
     // start at a multiple of 64 (lanes past the file end are inactive, and
     // lane 0 holds the lowest word, so it is active whenever any lane is)
     if (J.LS) {
 #pragma unroll
-      for (int i = 0; i < NLV; i++) {
+      for (int i = 0; i < NLVS; i++) {
         if (lo + i < J.L) {
           const unsigned long long m = __ballot(out[i] != 0);
           if ((tid & 63) == 0 && act) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
@@ -495,8 +492,10 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
 }
 
 template <int BLOCK, int ROWS, int NLV, bool VERIFY>
-// 3 wavefronts per SIMD: the resident set the LDS of the default tile allows.
-__global__ __launch_bounds__(BLOCK, 3) void index_kernel(IndexLaunch launch) {
+// The default tile (256 x 3, no verify words) keeps 3 blocks per CU by LDS:
+// registers are held to the 3 wavefronts per SIMD that allows.
+__global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 : 1) void index_kernel(
+    IndexLaunch launch) {
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
   static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
@@ -517,10 +516,10 @@ __global__ __launch_bounds__(BLOCK, 3) void index_kernel(IndexLaunch launch) {
   // lanes of a wavefront touch consecutive 8-byte slots) while the scans and
   // the lookback run, so they occupy no registers there; s_lead: length of the
   // word's leading backslash run (64: all backslashes) | 0x80 if a quote follows it
-  constexpr int NSLOT = VERIFY ? 7 : 5;
+  constexpr int NSLOT = VERIFY ? 8 : 7;
   __shared__ uint64_t s_wc[ROWS][NSLOT][BLOCK];
   // per wavefront: level words and work list of the spread leveled bitmaps
-  constexpr int NLVS = NLV;
+  constexpr int NLVS = NLV < kSpreadLevels ? NLV : kSpreadLevels;
   __shared__ uint64_t s_lv[NW][NLVS][64];
   __shared__ uint16_t s_items[NW][kBracketCap];
   __shared__ uint8_t s_lead[ROWS][BLOCK];
@@ -605,17 +604,9 @@ __global__ __launch_bounds__(BLOCK, 3) void index_kernel(IndexLaunch launch) {
         s_wc[r][kSlotO][tid] = w.O;
         s_wc[r][kSlotC][tid] = w.C;
         s_wc[r][kSlotP][tid] = w.P;
-        if (VERIFY) {
-          s_wc[r][kSlotB][tid] = w.B;
-          s_wc[r][kSlotQ][tid] = w.Q;
-        }
-        // the colon and raw-quote words of the record (bytes past the end of
-        // the file were loaded as zeros, which are neither ':' nor '"')
-        const int64_t wr = word0 + (int64_t)r * BLOCK + tid;
-        if (wr < J.W) {
-          J.rec[wr * kRecWords + 1] = w.K;
-          J.rec[wr * kRecWords + 2] = w.Q;
-        }
+        if (VERIFY) s_wc[r][kSlotB][tid] = w.B;
+        s_wc[r][kSlotQ][tid] = w.Q;
+        s_wc[r][kSlotK][tid] = w.K;
         s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
         if (!VERIFY) {
           const unsigned long long nlw = __ballot(w.NL != 0);
