@@ -59,6 +59,7 @@ SIGNATURES = {
     "gpjson_buf_size": (c_int64, [c_void_p]),
     "gpjson_buf_free": (None, [c_void_p]),
     "gpjson_index_build": (c_int, [c_void_p, c_void_p, c_int, c_int, P(c_void_p)]),
+    "gpjson_index_file": (c_int, [c_void_p, c_char_p, c_int, c_int, P(c_void_p), P(c_void_p)]),
     "gpjson_index_rebuild": (c_int, [c_void_p]),
     "gpjson_index_lines": (c_int64, [c_void_p]),
     "gpjson_index_levels": (c_int, [c_void_p]),

@@ -23,7 +23,7 @@ from __future__ import annotations
 
 import ctypes
 import weakref
-from typing import List, Optional, Sequence, Union
+from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -153,6 +153,16 @@ class Context:
         flags = N.GPJSON_BUILD_VERIFY if verify else 0
         _check(self._lib.gpjson_index_build(self._h, buf.handle, levels, flags, ctypes.byref(h)))
         return self._adopt(Index(self, h, buf))
+
+    def index_file(self, path: str, levels: int, verify: bool = False) -> Tuple["Buffer", "Index"]:
+        """Reads and indexes a file with the copy of each chunk overlapped with
+        indexing the chunks before it; equals stage_file() then index()."""
+        hb, hi = ctypes.c_void_p(), ctypes.c_void_p()
+        flags = N.GPJSON_BUILD_VERIFY if verify else 0
+        _check(self._lib.gpjson_index_file(self._h, path.encode(), levels, flags, ctypes.byref(hb),
+                                           ctypes.byref(hi)))
+        buf = self._adopt(Buffer(self, hb))
+        return buf, self._adopt(Index(self, hi, buf))
 
     def run_queries(self, idx: "Index", irs: Sequence[bytes], results_per_line: int) -> np.ndarray:
         """int64[nq][lines][R][2] spans for compiled IRs over one index."""

@@ -68,8 +68,9 @@ struct IndexLaunch {
   const FileJob* jobs;     // device array, or null: the single job `one`
   FileJob one;
   int64_t njobs;
-  int64_t total_tiles;
-  TileState* states;       // total_tiles entries, zeroed before launch
+  int64_t total_tiles;     // tiles [tile_begin, total_tiles) are this launch's
+  int64_t tile_begin;      // earlier tiles were indexed by earlier launches
+  TileState* states;       // total_tiles entries, zeroed before the first launch
   unsigned int* ticket;    // zeroed before launch
 };
 

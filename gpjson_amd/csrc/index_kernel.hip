@@ -531,7 +531,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
 
   for (;;) {
     if (tid == 0) {
-      s_ticket = atomicAdd(launch.ticket, 1u);
+      s_ticket = (unsigned)launch.tile_begin + atomicAdd(launch.ticket, 1u);
       s_w0 = -1;
       s_w0_flip = 0;
       s_fnl = TILE_WORDS;

@@ -94,9 +94,20 @@ static int ensure_host_scratch(gpjson_ctx* ctx, size_t bytes) {
   return GPJSON_OK;
 }
 
+// Staging chunk: 64 MiB, or GPJSON_STAGE_CHUNK_BYTES (at least 4 KiB) when set
+// (small chunks let tests cover many chunk boundaries with small files).
+static size_t stage_chunk_bytes() {
+  const char* v = std::getenv("GPJSON_STAGE_CHUNK_BYTES");
+  if (v && *v) {
+    const long long b = std::atoll(v);
+    if (b >= 4096) return (size_t)b;
+  }
+  return (size_t)64 << 20;
+}
+
 static int ensure_pinned(gpjson_ctx* ctx) {
   if (ctx->pinned[0]) return GPJSON_OK;
-  const size_t sz = (size_t)64 << 20;
+  const size_t sz = stage_chunk_bytes();
   for (int b = 0; b < 2; b++) {
     hipError_t e = hipHostMalloc(&ctx->pinned[b], sz, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -154,9 +165,9 @@ int upload_irs(gpjson_ctx* ctx, const uint8_t* const* irs, const int64_t* ir_len
   return GPJSON_OK;
 }
 
-int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
+// Tile geometry, lookback state and launch descriptor of one index build.
+static int index_setup(gpjson_index* idx, const IndexTune& t, IndexLaunch* out) {
   gpjson_ctx* ctx = idx->ctx;
-  const IndexTune& t = tune ? *tune : ctx->itune;
   const int64_t tile_bytes = index_tile_bytes(t.block, t.wpt);
   const int64_t ntiles = (idx->n + tile_bytes - 1) / tile_bytes;
   if (ntiles + 1 > idx->tile_lines_cap) return fail(GPJSON_ERR_CAPACITY, "tile table too small");
@@ -193,6 +204,17 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   L.total_tiles = ntiles;
   L.states = (TileState*)ctx->states.p;
   L.ticket = (unsigned int*)ctx->ticket.p;
+  *out = L;
+  return GPJSON_OK;
+}
+
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
+  gpjson_ctx* ctx = idx->ctx;
+  const IndexTune& t = tune ? *tune : ctx->itune;
+  IndexLaunch L{};
+  int rc = index_setup(idx, t, &L);
+  if (rc) return rc;
+  const int64_t ntiles = L.total_tiles;
   GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
   if (ntiles > 0) {
@@ -315,6 +337,12 @@ static int read_lines(gpjson_index* idx) {
   return GPJSON_OK;
 }
 
+// The index object and its device arrays for buf (nothing is launched).
+static int alloc_index(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags,
+                       std::unique_ptr<gpjson_index>* out);
+static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, gpjson_buf** buf_out,
+                      gpjson_index** idx_out);
+
 static void free_index_arrays(gpjson_index* idx) {
   void* ps[] = {idx->rec, idx->LB, idx->NL, idx->E, idx->Q, idx->lines_dev, idx->tile_lines};
   for (void* p : ps)
@@ -385,6 +413,10 @@ void gpjson_ctx_destroy(gpjson_ctx* ctx) {
     if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
   }
   if (ctx->host_scratch) (void)hipHostFree(ctx->host_scratch);
+  if (ctx->copy_stream) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipStreamDestroy(ctx->copy_stream);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -512,7 +544,37 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
                                          std::to_string(levels));
   *out = nullptr;
   GJ_HIP(hipSetDevice(ctx->device));
-  std::unique_ptr<gpjson_index> idx(new gpjson_index());
+  std::unique_ptr<gpjson_index> idx;
+  int rc = alloc_index(ctx, buf, levels, flags, &idx);
+  if (rc == GPJSON_OK) rc = build_index(idx.get());
+  if (rc) {
+    if (idx) free_index_arrays(idx.get());
+    return rc;
+  }
+  *out = idx.release();
+  return GPJSON_OK;
+}
+
+int gpjson_index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, gpjson_buf** buf_out,
+                      gpjson_index** idx_out) {
+  if (!ctx || !path || !buf_out || !idx_out) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  if (levels < 0 || levels > GPJSON_MAX_LEVELS)
+    return fail(GPJSON_ERR_CAPACITY, "levels must be in [0, " + std::to_string(GPJSON_MAX_LEVELS) + "], got " +
+                                         std::to_string(levels));
+  *buf_out = nullptr;
+  *idx_out = nullptr;
+  GJ_HIP(hipSetDevice(ctx->device));
+  return gj::index_file(ctx, path, levels, flags, buf_out, idx_out);
+}
+
+}  // extern "C"
+
+namespace gj {
+
+static int alloc_index(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags,
+                       std::unique_ptr<gpjson_index>* out) {
+  std::unique_ptr<gpjson_index>& idx = *out;
+  idx.reset(new gpjson_index());
   idx->ctx = ctx;
   idx->buf = buf;
   idx->n = buf->n;
@@ -538,14 +600,107 @@ int gpjson_index_build(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags, 
     alloc((void**)&idx->E, (size_t)idx->W * 8);
     alloc((void**)&idx->Q, (size_t)idx->W * 8);
   }
-  if (rc == GPJSON_OK) rc = build_index(idx.get());
+  return rc;
+}
+
+// Staging overlapped with indexing.  The file is read into the two pinned
+// buffers in turn and each chunk is copied to the device on the copy stream;
+// after a chunk's copy, one index launch on the context stream takes the tiles
+// that chunk completes.  The tile states persist across those launches (reset
+// once), so a launch's lookbacks read the prefixes earlier launches published,
+// and the result is the one-launch index.  Reading chunk k+2, copying chunk k+1
+// and indexing chunk k run at once.
+static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, gpjson_buf** buf_out,
+                      gpjson_index** idx_out) {
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fail(GPJSON_ERR_IO, std::string("Failed to open file ") + path);
+  struct stat st;
+  if (fstat(fd, &st) != 0) {
+    close(fd);
+    return fail(GPJSON_ERR_IO, "Failed to get size of file");
+  }
+  const int64_t n = (int64_t)st.st_size;
+  std::unique_ptr<gpjson_buf> b(new gpjson_buf());
+  b->ctx = ctx;
+  b->n = n;
+  b->owned = true;
+  std::unique_ptr<gpjson_index> idx;
+  bool launched = false;
+  auto run = [&]() -> int {
+    int rc = dev_alloc((void**)&b->d, (size_t)n + 64);
+    if (rc) return rc;
+    rc = alloc_index(ctx, b.get(), levels, flags, &idx);
+    if (rc) return rc;
+    rc = ensure_pinned(ctx);
+    if (rc) return rc;
+    if (!ctx->copy_stream) GJ_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    GJ_HIP(hipStreamSynchronize(ctx->stream));
+    GJ_HIP(hipStreamSynchronize(ctx->copy_stream));
+    const IndexTune t = ctx->itune;
+    IndexLaunch L{};
+    rc = index_setup(idx.get(), t, &L);
+    if (rc) return rc;
+    const int64_t ntiles = L.total_tiles, tile_bytes = idx->tile_bytes;
+    const int nlv = nlv_for_levels(levels);
+    const bool verify = (flags & GPJSON_BUILD_VERIFY) != 0;
+    launched = true;
+    GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream));
+    const int64_t chunk = (int64_t)ctx->pinned_size;
+    int64_t done_tiles = 0;
+    int k = 0;
+    for (int64_t off = 0; off < n; off += chunk, k++) {
+      const int pb = k & 1;
+      const int64_t len = std::min<int64_t>(chunk, n - off);
+      GJ_HIP(hipEventSynchronize(ctx->pin_ev[pb]));  // its previous copy is done
+      int64_t got = 0;
+      while (got < len) {
+        const ssize_t r = pread(fd, (uint8_t*)ctx->pinned[pb] + got, (size_t)(len - got), (off_t)(off + got));
+        if (r <= 0) return fail(GPJSON_ERR_IO, "Size of file has changed while reading");
+        got += r;
+      }
+      GJ_HIP(hipMemcpyAsync(b->d + off, ctx->pinned[pb], (size_t)len, hipMemcpyHostToDevice, ctx->copy_stream));
+      GJ_HIP(hipEventRecord(ctx->pin_ev[pb], ctx->copy_stream));
+      const int64_t end_tiles = off + len >= n ? ntiles : (off + len) / tile_bytes;
+      if (end_tiles > done_tiles) {
+        GJ_HIP(hipStreamWaitEvent(ctx->stream, ctx->pin_ev[pb], 0));
+        L.tile_begin = done_tiles;
+        L.total_tiles = end_tiles;
+        GJ_HIP(zero_u64((unsigned long long*)L.ticket, 1, ctx->stream));
+        GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)(end_tiles - done_tiles), ctx->stream));
+        done_tiles = end_tiles;
+      }
+    }
+    L.tile_begin = 0;
+    L.total_tiles = ntiles;
+    GJ_HIP(index_lines(L, idx->lines_dev, ctx->stream));
+    rc = read_lines(idx.get());
+    if (rc) return rc;
+    struct stat st2;
+    if (fstat(fd, &st2) != 0 || st2.st_size != st.st_size)
+      return fail(GPJSON_ERR_IO, "Size of file has changed while reading");
+    if (idx->lines > idx->nl_cap) rc = build_index(idx.get());  // a larger newline index, from the device copy
+    return rc;
+  };
+  const int rc = run();
+  close(fd);
   if (rc) {
-    free_index_arrays(idx.get());
+    if (launched) {
+      (void)hipStreamSynchronize(ctx->copy_stream);
+      (void)hipStreamSynchronize(ctx->stream);
+    }
+    if (idx) free_index_arrays(idx.get());
+    if (b->d) (void)hipFree(b->d);
     return rc;
   }
-  *out = idx.release();
+  *buf_out = b.release();
+  idx->buf = *buf_out;
+  *idx_out = idx.release();
   return GPJSON_OK;
 }
+
+}  // namespace gj
+
+extern "C" {
 
 int gpjson_index_rebuild(gpjson_index* idx) {
   if (!idx) return fail(GPJSON_ERR_INVALID_ARG, "idx is NULL");
@@ -736,14 +891,9 @@ int gpjson_query_file(gpjson_ctx* ctx, const char* file, const char* const* path
   if (R < 1) R = 1;
   GJ_HIP(hipSetDevice(ctx->device));
   gpjson_buf* buf = nullptr;
-  int rc = gpjson_buf_from_file(ctx, file, &buf);
-  if (rc) return rc;
   gpjson_index* idx = nullptr;
-  rc = gpjson_index_build(ctx, buf, L, 0, &idx);
-  if (rc) {
-    gpjson_buf_free(buf);
-    return rc;
-  }
+  int rc = gpjson_index_file(ctx, file, L, 0, &buf, &idx);
+  if (rc) return rc;
   std::unique_ptr<gpjson_result> res(new gpjson_result());
   res->nq = npaths;
   res->lines = idx->lines;

@@ -91,6 +91,7 @@ struct DevBuf {
 struct gpjson_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // host-to-device staging overlapped with indexing
   int num_cus = 0;
   gj::DevBuf states;      // lookback tile states
   gj::DevBuf ticket;      // index ticket + query counter + line counts
