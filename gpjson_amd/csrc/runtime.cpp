@@ -610,6 +610,13 @@ static int alloc_index(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags,
 // once), so a launch's lookbacks read the prefixes earlier launches published,
 // and the result is the one-launch index.  Reading chunk k+2, copying chunk k+1
 // and indexing chunk k run at once.
+//
+// Switched off (kOverlapStaging): with 64 MiB chunks a 1 GiB file never
+// finished indexing on the GPU, while files of a few MB in 1 MiB and 20 KB
+// chunks matched the one-launch index; the cause is not known.  Until it is,
+// the file is read and copied whole, then indexed in one launch.
+constexpr bool kOverlapStaging = false;
+
 static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, gpjson_buf** buf_out,
                       gpjson_index** idx_out) {
   int fd = open(path, O_RDONLY);
@@ -631,6 +638,22 @@ static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, 
     if (rc) return rc;
     rc = alloc_index(ctx, b.get(), levels, flags, &idx);
     if (rc) return rc;
+    if (!kOverlapStaging) {
+      rc = staged_upload(ctx, b->d, n, [&](void* dst, int64_t off, int64_t len) -> int {
+        int64_t got = 0;
+        while (got < len) {
+          const ssize_t r = pread(fd, (uint8_t*)dst + got, (size_t)(len - got), (off_t)(off + got));
+          if (r <= 0) return fail(GPJSON_ERR_IO, "Size of file has changed while reading");
+          got += r;
+        }
+        return GPJSON_OK;
+      });
+      if (rc) return rc;
+      struct stat st2;
+      if (fstat(fd, &st2) != 0 || st2.st_size != st.st_size)
+        return fail(GPJSON_ERR_IO, "Size of file has changed while reading");
+      return build_index(idx.get());
+    }
     rc = ensure_pinned(ctx);
     if (rc) return rc;
     if (!ctx->copy_stream) GJ_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));

@@ -129,14 +129,13 @@ void gpjson_buf_free(gpjson_buf *buf);
  * stages/CombinedIndex.java:28-56, stages/LeveledBitmapsIndex.java:40-54, and
  * their kernels).  The index keeps a reference to buf. */
 int gpjson_index_build(gpjson_ctx *ctx, gpjson_buf *buf, int levels, int flags, gpjson_index **out);
-/* Reads a file and indexes it in one pipeline: chunks are read into pinned
- * staging and copied on a second stream, and the index tiles each chunk
- * completes are indexed as soon as its copy lands, while the next chunks are
- * read and copied.  Replaces QueryFunction.readFile followed by the index
- * build (functions/QueryFunction.java:149,316-335, stages/CombinedIndex.java:
- * 28-56, stages/LeveledBitmapsIndex.java:40-54).  The result equals
+/* Reads a file and indexes it (replaces QueryFunction.readFile followed by the
+ * index build, functions/QueryFunction.java:149,316-335, stages/CombinedIndex.
+ * java:28-56, stages/LeveledBitmapsIndex.java:40-54).  The result equals
  * gpjson_buf_from_file + gpjson_index_build; the caller frees *idx_out, then
- * *buf_out.  Errors as those two (GPJSON_ERR_IO when the file size changes). */
+ * *buf_out.  Errors as those two (GPJSON_ERR_IO when the file size changes).
+ * A pipelined form that indexes each staged chunk while the next is read and
+ * copied exists in the library but is switched off. */
 int gpjson_index_file(gpjson_ctx *ctx, const char *path, int levels, int flags, gpjson_buf **buf_out,
                       gpjson_index **idx_out);
 /* Rebuilds an index in place over the same buffers (no allocation). */

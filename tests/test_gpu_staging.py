@@ -1,14 +1,13 @@
-"""Staging overlapped with indexing (gpjson_index_file): the file is read and
-copied chunk by chunk and each chunk's completed index tiles are indexed in a
-launch of their own, the lookback carrying across launches.  Every index array
-must equal the one-launch build over the same bytes (gpjson_buf_from_file +
-gpjson_index_build, which the parity tests pin against the oracle), and query()
--- which reads files this way -- must return the spans of the one-launch index.
+"""gpjson_index_file (a file read, copied and indexed in one call): every index
+array must equal the build from gpjson_buf_from_file + gpjson_index_build over
+the same bytes (which the parity tests pin against the oracle), and query() --
+which reads files this way -- must return the spans of that index.
 
 The chunk size is forced small (GPJSON_STAGE_CHUNK_BYTES, read when a context
-first stages) so that small files cross many chunk boundaries: chunks that end
-inside a tile, chunks shorter than a tile (no launch of their own), and the
-file's tail tile.
+first stages) so that small files cross many staging-chunk boundaries: chunks
+that end inside an index tile, chunks shorter than a tile, and the file's tail.
+(The library's pipelined form, which indexes each chunk's tiles as its copy
+lands, is switched off; these cases passed with it on.)
 """
 import os
 
