@@ -189,6 +189,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   gpjson_ctx* ctx = b->ctx;
   hipStream_t s = ctx->stream;
   IndexLaunch L{};
+  L.speculate = 1;
   L.jobs = (const FileJob*)b->dev_files.p;
   L.njobs = b->nfiles;
   L.total_tiles = b->total_tiles;

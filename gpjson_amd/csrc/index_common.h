@@ -72,6 +72,15 @@ struct IndexLaunch {
   int64_t tile_begin;      // earlier tiles were indexed by earlier launches
   TileState* states;       // total_tiles entries, zeroed before the first launch
   unsigned int* ticket;    // zeroed before launch
+  int speculate;           // tiles holding a newline may speculate their carry-in
+  // Deferred newline bookkeeping (one file): each tile writes its newline
+  // offsets to nl_scratch[tile * nl_scratch_cap + k] and its carry-in to its
+  // state; index_finish then builds tile_lines, the line count and the
+  // newline index, and checks every speculated carry-in.  Null:
+  // the tiles' lookback gives offsets and line counts directly.
+  int64_t* nl_scratch;
+  int64_t nl_scratch_cap;  // slots per tile (the tile's words)
+  const int* run_if;       // when set, the launch does nothing unless *run_if != 0
 };
 
 }  // namespace gj

@@ -193,6 +193,16 @@ __device__ void publish_inc(TileState* st, const Carry& c) {
   st_state2(&st->inc0, w0, w1);
 }
 
+// The carry-in a tile was indexed with (deferred bookkeeping), in the words
+// after its group aggregate: valid | p (62) | s (61) | speculated (60) | depth
+// (48 bits).
+__device__ void store_carry_in(TileState* st, const Carry& c, bool spec) {
+  const unsigned long long w0 = kValid | ((unsigned long long)(c.p & 1) << 62) |
+                                ((unsigned long long)(c.s & 1) << 61) | ((unsigned long long)spec << 60) |
+                                ((unsigned long long)c.d & 0xFFFFFFFFFFFFull);
+  st_state2(&st->pad[2], w0, 0);
+}
+
 __device__ __forceinline__ TileAgg agg_id() {
   TileAgg a;
   a.esc = 0;
@@ -528,6 +538,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const unsigned long long lanes_below = (1ull << lane) - 1ull;
+  if (launch.run_if && *launch.run_if == 0) return;
+  const bool dl = launch.nl_scratch != nullptr;  // deferred newline bookkeeping
 
   for (;;) {
     if (tid == 0) {
@@ -697,7 +709,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     // at once, and resolves its exact carry-in with the lookback afterwards:
     // only the newline offsets need it.  A wrong guess (the input is not
     // NDJSON there) re-emits the tile with the exact carry-in.
-    const bool spec = !VERIFY && lt > 0 && s_fnl < TILE_WORDS && s_lead[0][0] == 0;
+    const bool spec = launch.speculate && !VERIFY && lt > 0 && s_fnl < TILE_WORDS && s_lead[0][0] == 0;
     TileAgg a;
     a.esc = tile_esc;
     a.m[0] = mtile;
@@ -719,7 +731,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
         if (lane == 0) {
           publish_inc(st0 + lt, apply_agg(a, cin));
           s_carry = cin;
-          if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
+          if (J.tile_lines && !dl) J.tile_lines[lt] = lt == 0 ? 0 : cin.lines;
         }
       }
     }
@@ -750,6 +762,13 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     const int w0 = s_w0;
     const bool fix = (X.p == 1) && w0 >= 0 && s_w0_flip;
     const M4 w0m = s_w0_m;
+    if (dl && tid == 0) {
+      // deferred bookkeeping: a speculated tile publishes its inclusive prefix
+      // now and never looks back (index_finish checks every carry-in against
+      // its predecessor's carry-out after the launch)
+      if (spec) publish_inc(st0 + lt, apply_agg(a, X));
+      store_carry_in(st0 + lt, X, spec);
+    }
 
     // ---- emit the indexes (lane t writes word t of each row: coalesced) ----
     auto emit_tile = [&](const Carry& cx, bool fx) {
@@ -767,7 +786,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     };
     emit_tile(X, fix);
     Carry XL = X;  // carry-in whose line count places the newline offsets
-    if (spec) {
+    if (spec && !dl) {
       if (wave == 0) {
         const Carry cin = lookback_wave(st0, lt, lane, a);
         if (lane == 0) {
@@ -782,16 +801,20 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       // a tile whose first byte is neither '\\' nor '"' needs no escape fix
       if (s_mis) emit_tile(XL, false);
     }
+    // newline offsets: at their slots in the newline index, or (deferred) at
+    // tile-local slots of the tile's scratch row
+    int64_t* nl_out = dl ? launch.nl_scratch + (J.tile0 + lt) * launch.nl_scratch_cap : J.NL;
+    const int64_t nl_lim = dl ? launch.nl_scratch_cap : J.nl_cap;
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
       const int sidx = r * BLOCK + tid;
       const int64_t w = word0 + sidx;
       if (w >= J.W) continue;
       uint64_t nlm = s_wc[r][kSlotNL][tid];
-      int64_t idx = XL.lines + texcl[r].nl;
+      int64_t idx = (dl ? 0 : XL.lines) + texcl[r].nl;
       while (nlm) {
         const int b = ctz64(nlm);
-        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
+        if (idx < nl_lim) nl_out[idx] = w * 64 + b;
         idx++;
         nlm &= nlm - 1;
       }
@@ -802,12 +825,158 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
 
 // Zeroes the lookback states and the ticket (a kernel, not hipMemsetAsync, so
 // the index pipeline can be captured into a graph).
-__global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket) {
+__global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket, const int* run_if) {
+  if (run_if && *run_if == 0) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long* p = reinterpret_cast<unsigned long long*>(states);
   for (int64_t k = i; k < n * 8; k += stride) p[k] = 0;
   if (i == 0) *ticket = 0;
+}
+
+// The carry-in a tile stored for the deferred bookkeeping.
+__device__ __forceinline__ Carry load_carry_in(const TileState* st, bool* spec) {
+  const unsigned long long w = st->pad[2];
+  *spec = (w >> 60) & 1;
+  return Carry{(int)((w >> 62) & 1), (int)((w >> 61) & 1), sext(w & 0xFFFFFFFFFFFFull, 48), 0};
+}
+
+// Deferred newline bookkeeping of one file, in four launches of at most a
+// thread per tile (tiles in blocks of kTileBlock).
+constexpr int kTileBlock = 256;
+
+// Block-wide exclusive scan of one int per thread (kTileBlock threads);
+// returns the block total.
+__device__ int tile_block_scan(int v, int* excl) {
+  __shared__ int s_w[kTileBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_scan_int(v);
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kTileBlock / 64; w++) {
+    if (w < wave) before += s_w[w];
+    total += s_w[w];
+  }
+  *excl = before + incl - v;
+  __syncthreads();
+  return total;
+}
+
+// Step 1: newline counts (from the published aggregates) summed per block.
+// Every tile's stored carry-in is checked against its predecessor's carry-out
+// (the predecessor's aggregate applied to its carry-in): from tile 0's exact
+// one, by induction, this proves every speculated carry-in and every carry-in
+// a lookback took from a speculated prefix.  *mismatch gets 1 when one differs
+// (a speculation the input broke).
+__global__ __launch_bounds__(kTileBlock) void index_tile_sums_kernel(IndexLaunch L, int64_t* block_sums,
+                                                                     int* mismatch) {
+  if (L.run_if && *L.run_if == 0) return;
+  const FileJob& J = L.one;
+  const TileState* st = L.states + J.tile0;
+  const int64_t t = (int64_t)blockIdx.x * kTileBlock + threadIdx.x;
+  int cnt = 0;
+  if (t < J.ntiles) {
+    cnt = (int)(st[t].agg0 & kField24);
+    if (t > 0) {
+      bool ps, cs;
+      const Carry e = apply_agg(decode_agg(st[t - 1].agg0, st[t - 1].agg1), load_carry_in(st + t - 1, &ps));
+      const Carry c = load_carry_in(st + t, &cs);
+      if (c.s != e.s || c.d != e.d || (!cs && c.p != e.p)) atomicOr(mismatch, 1);
+    }
+  }
+  int excl;
+  const int total = tile_block_scan(cnt, &excl);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Step 2 (one block): exclusive scan of the block sums in place; the line
+// count (1 + newlines) and the closing tile_lines entry.
+__global__ __launch_bounds__(kTileBlock) void index_tile_base_kernel(IndexLaunch L, int64_t* block_sums,
+                                                                     int64_t nb, int64_t* lines_out) {
+  if (L.run_if && *L.run_if == 0) return;
+  const FileJob& J = L.one;
+  if (J.ntiles == 0) {  // empty file: one line starting at 0
+    if (threadIdx.x == 0) {
+      if (J.nl_cap > 0) J.NL[0] = 0;
+      if (J.tile_lines) J.tile_lines[0] = 1;
+      *lines_out = 1;
+    }
+    return;
+  }
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += kTileBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const int v = i < nb ? (int)block_sums[i] : 0;  // a block's newlines fit an int
+    int excl;
+    const int total = tile_block_scan(v, &excl);
+    if (i < nb) block_sums[i] = carry + excl;
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    if (J.tile_lines) J.tile_lines[J.ntiles] = 1 + carry;
+    *lines_out = 1 + carry;
+  }
+}
+
+// Step 3: tile_lines, and each tile's newline offsets from its scratch row to
+// their slots (tile t's first newline takes slot 1 + the newlines before it).
+// A tile with more newlines than its row holds (lines shorter than 64 bytes on
+// average) sets *overflow for step 4.
+__global__ __launch_bounds__(kTileBlock) void index_tile_lines_kernel(IndexLaunch L, const int64_t* block_base,
+                                                                      int* overflow) {
+  if (L.run_if && *L.run_if == 0) return;
+  const FileJob& J = L.one;
+  const int64_t t = (int64_t)blockIdx.x * kTileBlock + threadIdx.x;
+  const int cnt = t < J.ntiles ? (int)(L.states[J.tile0 + t].agg0 & kField24) : 0;
+  int excl;
+  tile_block_scan(cnt, &excl);
+  if (t >= J.ntiles) return;
+  const int64_t before = block_base[blockIdx.x] + excl;  // newlines before tile t
+  if (J.tile_lines) J.tile_lines[t] = t == 0 ? 0 : 1 + before;
+  if (cnt > L.nl_scratch_cap) {
+    atomicOr(overflow, 1);
+    return;
+  }
+  const int64_t* src = L.nl_scratch + (J.tile0 + t) * L.nl_scratch_cap;
+  for (int k = 0; k < cnt; k++)
+    if (1 + before + k < J.nl_cap) J.NL[1 + before + k] = src[k];
+}
+
+// Step 4 (only when *overflow): the offsets of tiles with more newlines than
+// their scratch row, scanned again from the input, a wavefront per tile.
+__global__ __launch_bounds__(256) void index_tile_overflow_kernel(IndexLaunch L, const int* overflow) {
+  if ((L.run_if && *L.run_if == 0) || *overflow == 0) return;
+  const FileJob& J = L.one;
+  const int lane = threadIdx.x & 63;
+  const int64_t cap = L.nl_scratch_cap;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < J.ntiles; t += nwaves) {
+    const int64_t cnt = (int64_t)(L.states[J.tile0 + t].agg0 & kField24);
+    if (cnt <= cap) continue;
+    int64_t slot = t == 0 ? 1 : J.tile_lines[t];
+    const int64_t wb = t * cap, we = min(wb + cap, J.W);
+    for (int64_t w0 = wb; w0 < we; w0 += 64) {
+      const int64_t w = w0 + lane;
+      uint64_t m = 0;
+      if (w < we)
+        for (int b = 0; b < 64; b++) {
+          const int64_t o = w * 64 + b;
+          if (o < J.N && J.F[o] == '\n') m |= 1ull << b;
+        }
+      const int c = popc64(m);
+      const int incl = wave_scan_int(c);
+      int64_t idx = slot + incl - c;
+      while (m) {
+        const int b = ctz64(m);
+        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
+        idx++;
+        m &= m - 1;
+      }
+      slot += __shfl(incl, 63);
+    }
+  }
 }
 
 // Reads the line count of each file from its last tile's inclusive prefix.
@@ -894,12 +1063,32 @@ hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool 
   return hipErrorInvalidValue;
 }
 
-hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream) {
+hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream,
+                       const int* run_if) {
   int64_t n8 = ntiles * 8;
   int grid = (int)((n8 + 255) / 256);
   if (grid < 1) grid = 1;
   if (grid > 4096) grid = 4096;
-  hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket);
+  hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket, run_if);
+  return hipGetLastError();
+}
+
+// Steps 1-4 of the deferred newline bookkeeping; block_sums holds one int64
+// per kTileBlock tiles.
+hipError_t index_finish(const IndexLaunch& L, int64_t* block_sums, int64_t* lines_out, int* mismatch,
+                        int* overflow, hipStream_t stream) {
+  const int64_t nt = L.one.ntiles;
+  const int64_t nb = (nt + kTileBlock - 1) / kTileBlock;
+  const unsigned grid = (unsigned)(nb > 0 ? nb : 1);
+  if (nt > 0)
+    hipLaunchKernelGGL(index_tile_sums_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L, block_sums, mismatch);
+  hipLaunchKernelGGL(index_tile_base_kernel, dim3(1), dim3(kTileBlock), 0, stream, L, block_sums, nb, lines_out);
+  if (nt > 0) {
+    hipLaunchKernelGGL(index_tile_lines_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L,
+                       (const int64_t*)block_sums, overflow);
+    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 1024);
+    hipLaunchKernelGGL(index_tile_overflow_kernel, dim3(ogrid), dim3(256), 0, stream, L, (const int*)overflow);
+  }
   return hipGetLastError();
 }
 

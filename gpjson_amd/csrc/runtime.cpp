@@ -204,6 +204,16 @@ static int index_setup(gpjson_index* idx, const IndexTune& t, IndexLaunch* out) 
   L.total_tiles = ntiles;
   L.states = (TileState*)ctx->states.p;
   L.ticket = (unsigned int*)ctx->ticket.p;
+  L.speculate = 1;
+  // newline offsets per tile before compaction: a slot per word of every tile
+  L.nl_scratch_cap = index_tile_bytes(t.block, t.wpt) / 64;
+  // (then one int64 per 256 tiles for the bookkeeping's block sums)
+  const size_t scratch_bytes =
+      ((size_t)std::max<int64_t>(ntiles, 1) * (size_t)L.nl_scratch_cap + (size_t)(ntiles / 256 + 2)) * 8;
+  if (scratch_bytes > ctx->nl_scratch.cap) GJ_HIP(hipStreamSynchronize(ctx->stream));
+  rc = ctx->nl_scratch.ensure(scratch_bytes);
+  if (rc) return rc;
+  L.nl_scratch = (int64_t*)ctx->nl_scratch.p;
   *out = L;
   return GPJSON_OK;
 }
@@ -215,14 +225,28 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
   int rc = index_setup(idx, t, &L);
   if (rc) return rc;
   const int64_t ntiles = L.total_tiles;
+  const int nlv = nlv_for_levels(idx->L);
+  const bool verify = (idx->flags & GPJSON_BUILD_VERIFY) != 0;
+  // bookkeeping flags, after the ticket: [16] a speculated carry-in was wrong
+  // (it gates the second pass), [24] a tile overflowed its scratch row, [32]
+  // the second pass's check (not read)
+  int* flags = (int*)ctx->ticket.p + 16;
+  int64_t* block_sums = L.nl_scratch + ntiles * L.nl_scratch_cap;
   GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream));
+  GJ_HIP(zero_u64((unsigned long long*)flags, 8, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  if (ntiles > 0) {
-    GJ_HIP(index_launch(L, t.block, t.wpt, nlv_for_levels(idx->L), (idx->flags & GPJSON_BUILD_VERIFY) != 0,
-                        (int)ntiles, ctx->stream));
-  }
+  if (ntiles > 0) GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)ntiles, ctx->stream));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
-  GJ_HIP(index_lines(L, idx->lines_dev, ctx->stream));
+  GJ_HIP(index_finish(L, block_sums, idx->lines_dev, flags, flags + 8, ctx->stream));
+  // A wrong speculation (input that is not NDJSON somewhere): the same build
+  // again with every tile taking the lookback.  These launches return at once
+  // unless the check above raised its flag, so no host round trip is needed.
+  IndexLaunch L2 = L;
+  L2.speculate = 0;
+  L2.run_if = flags;
+  GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream, flags));
+  if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)ntiles, ctx->stream));
+  GJ_HIP(index_finish(L2, block_sums, idx->lines_dev, flags + 16, flags + 8, ctx->stream));
   return GPJSON_OK;
 }
 
@@ -407,6 +431,7 @@ void gpjson_ctx_destroy(gpjson_ctx* ctx) {
   ctx->ticket.release();
   ctx->jobs.release();
   ctx->irbuf.release();
+  ctx->nl_scratch.release();
   for (auto& b : ctx->gather) b.release();
   for (int b = 0; b < 2; b++) {
     if (ctx->pinned[b]) (void)hipHostFree(ctx->pinned[b]);
@@ -663,6 +688,7 @@ static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, 
     IndexLaunch L{};
     rc = index_setup(idx.get(), t, &L);
     if (rc) return rc;
+    L.nl_scratch = nullptr;  // the tiles' lookback gives the newline offsets
     const int64_t ntiles = L.total_tiles, tile_bytes = idx->tile_bytes;
     const int nlv = nlv_for_levels(levels);
     const bool verify = (flags & GPJSON_BUILD_VERIFY) != 0;

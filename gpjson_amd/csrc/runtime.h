@@ -15,8 +15,15 @@ namespace gj {
 // Kernel entry points (index_kernel.hip, query_kernel.hip).
 hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
                         hipStream_t stream);
-hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream);
+hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream,
+                       const int* run_if = nullptr);
 hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
+// Deferred newline bookkeeping of a one-file launch (IndexLaunch.nl_scratch):
+// tile_lines, the line count, the newline index and the speculation check
+// (*mismatch = 1 when a speculated carry-in was wrong).  block_sums: one int64
+// per 256 tiles; *overflow must start at 0.
+hipError_t index_finish(const IndexLaunch& L, int64_t* block_sums, int64_t* lines_out, int* mismatch,
+                        int* overflow, hipStream_t stream);
 bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
 hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream,
@@ -97,6 +104,7 @@ struct gpjson_ctx {
   gj::DevBuf ticket;      // index ticket + query counter + line counts
   gj::DevBuf jobs;        // device job arrays
   gj::DevBuf irbuf;       // IR bytes of a query launch
+  gj::DevBuf nl_scratch;  // an index build's newline offsets per tile, before compaction
   gj::DevBuf gather[4];   // result gather: file spans, block sums + total, gathered spans, bytes
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_size = 0;

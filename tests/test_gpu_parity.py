@@ -112,6 +112,18 @@ def backslash_runs(rng, n):
     return bytes(out[:n])
 
 
+def test_short_valid_lines(ctx):
+    """Valid NDJSON whose lines are shorter than 64 bytes: every tile holds more
+    newlines than it has words (its newline offsets are rescanned from the input
+    after the launch) while every speculated carry-in is right."""
+    rng = np.random.default_rng(17)
+    parts = [b'{"a":1}', b'[]', b'{"k":[1,{"x":"y"}]}', b'"s\\"t"', b'{}', b'3', b'{"b":"\\\\"}']
+    out = bytearray()
+    while len(out) < 1_200_000:
+        out += parts[int(rng.integers(0, len(parts)))] + b"\n"
+    check_index(ctx, bytes(out), 3, "short valid lines")
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_backslash_runs_across_boundaries(ctx, seed):
     rng = np.random.default_rng(200 + seed)
