@@ -526,6 +526,17 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       if (g.r == 0) launch.long_items[slot] = it;
       return true;
     };
+    if (launch.lines_dev && launch.njobs == 1) {
+      // one query: its job once, in scalar registers, not reloaded per line
+      const QueryJob J = launch.jobs[0];
+      const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
+      const uint8_t* ir = ir_of(J);
+      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+        if (defer(J, common, it, it)) continue;
+        eval_line(g, J, W, ir, common, it, J.out + it * (int64_t)J.R * 2, le, ci, nslots);
+      }
+      continue;
+    }
     if (launch.lines_dev) {
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
         const int64_t q = it / common, i = it - q * common;
