@@ -537,10 +537,18 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       }
       continue;
     }
+    // several jobs: each lane keeps its current job in registers and reloads
+    // it only when its lines cross into the next job (lines of a claim are
+    // consecutive), so a line's walk does not start with a load of its job
     if (launch.lines_dev) {
+      int64_t qcur = (int64_t)g0 / common;
+      QueryJob J = launch.jobs[qcur];
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
         const int64_t q = it / common, i = it - q * common;
-        const QueryJob& J = launch.jobs[q];
+        if (q != qcur) {
+          qcur = q;
+          J = launch.jobs[q];
+        }
         if (defer(J, common, i, it)) continue;
         int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
         eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
@@ -552,9 +560,14 @@ __global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int ns
       const int mid = (lo + hi + 1) >> 1;
       if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
     }
+    QueryJob J = launch.jobs[lo];
+    int64_t next0 = lo + 1 < launch.njobs ? launch.jobs[lo + 1].item0 : INT64_MAX;  // first item of the next job
     for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-      while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
-      const QueryJob& J = launch.jobs[lo];
+      if (it >= next0) {
+        while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
+        J = launch.jobs[lo];
+        next0 = lo + 1 < launch.njobs ? launch.jobs[lo + 1].item0 : INT64_MAX;
+      }
       const int64_t i = it - J.item0;
       if (defer(J, J.lines, i, it)) continue;
       eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
