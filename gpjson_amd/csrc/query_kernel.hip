@@ -478,8 +478,12 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
 
 }  // namespace
 
+// Registers held to 128 so that 4 wavefronts fit per SIMD: the walk is a chain
+// of dependent loads, and the fourth wavefront hides more of it than the few
+// spilled registers cost (4 GiB of tweets, four one-query paths: 3.64 against
+// 3.96 ms at 3 wavefronts per SIMD).
 template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void query_kernel(QueryLaunch launch, int nslots) {
+__global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int nslots) {
   constexpr int NG = BLOCK / G;
   extern __shared__ int64_t q_lds[];
   __shared__ uint8_t s_ir[kIrLdsBytes];
