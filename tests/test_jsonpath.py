@@ -54,8 +54,9 @@ GOLDEN = [
     ("$.user.values[999]", key("user") + D + key("values") + D + index(999) + D + S + E, 3, 1),
 ]
 
-# hex strings of the survey's golden-vector table for two of the paths, as an
-# independent check of the helpers above
+# the same IR written out byte by byte for two of the paths (LEB128 lengths and
+# indexes as the reference's IRBuilder encodes them), as an independent check
+# of the helpers above
 HEX = {
     "$.user.lang": "04 04 75 73 65 72 03 04 04 6c 61 6e 67 03 01 00",
     "$.user.values[999]": "04 04 75 73 65 72 03 04 06 76 61 6c 75 65 73 03 05 e7 07 03 01 00",
