@@ -404,7 +404,12 @@ def run_serve(args) -> int:
         ps = rng.integers(0, len(SERVE_PATHS_12), size=n)
         return [(f"file{int(f):03d}", SERVE_PATHS_12[int(p)]) for f, p in zip(fs, ps)]
 
-    # capacity: everything submitted at once (closed burst)
+    # capacity: everything submitted at once (closed burst), after one untimed
+    # burst of the same size (the first one also sets up the server's result
+    # slots and launch state, which varied its time by up to 1.7x)
+    for _ in range(max(args.warmup, 1)):
+        for tk in [srv.submit(k, p) for k, p in draw(args.serve_burst)]:
+            srv.result(tk).release()
     burst = draw(args.serve_burst)
     t0 = time.perf_counter()
     tickets = [srv.submit(k, p) for k, p in burst]
