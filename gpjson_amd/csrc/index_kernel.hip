@@ -515,6 +515,9 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   __shared__ int s_job;
   __shared__ int s_esc[ROWS][NW];  // last escape code that is not 0 of each (row, wave)
   __shared__ M4 s_m[ROWS][NW];     // transfer function of each (row, wave)
+  __shared__ M4 s_pre[ROWS * NW];  // their exclusive prefixes in row-major order
+  __shared__ M4 s_mtile;           // the tile's transfer function
+  static_assert(ROWS * NW <= 16, "the (row, wave) prefixes are scanned within one DPP row");
   __shared__ int s_w0;             // sequence index of the tile's first word that is not all '\\'
   __shared__ int s_w0_flip;
   __shared__ M4 s_w0_m;
@@ -690,16 +693,23 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       if (lane == 63) s_m[r][wave] = inc;
     }
     __syncthreads();
-    M4 texcl[ROWS];
-    M4 mtile = m_id();
-#pragma unroll
-    for (int r = 0; r < ROWS; r++) {
-#pragma unroll
-      for (int w = 0; w < NW; w++) {
-        if (w == wave) texcl[r] = m_cat(mtile, lex[r]);
-        mtile = m_cat(mtile, s_m[r][w]);
-      }
+    // the (row, wave) totals' exclusive prefixes, scanned once by the first
+    // lanes of wave 0 (within one 16-lane DPP row)
+    if (wave == 0) {
+      M4 v = lane < ROWS * NW ? s_m[lane / NW][lane % NW] : m_id();
+      v = m_cat(dpp_m<0x111, 0xF>(v), v);
+      v = m_cat(dpp_m<0x112, 0xF>(v), v);
+      v = m_cat(dpp_m<0x114, 0xF>(v), v);
+      v = m_cat(dpp_m<0x118, 0xF>(v), v);
+      const M4 ex = lane_below_m(v);
+      if (lane < ROWS * NW) s_pre[lane] = ex;
+      if (lane == ROWS * NW - 1) s_mtile = v;
     }
+    __syncthreads();
+    M4 texcl[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) texcl[r] = m_cat(s_pre[r * NW + wave], lex[r]);
+    const M4 mtile = s_mtile;
 
     // ---- carry-in: speculated at the first newline, or from the lookback ----
     // In NDJSON every raw '\n' lies outside strings at depth -1 (one value per
