@@ -66,7 +66,7 @@ def cpu_baseline(data, paths, sample_bytes: int, threads: int):
         "cores": threads,
         "kind": "port",
         "sample": f"first {sample.size} bytes of the same input, {len(paths)} index+query passes "
-                  f"({', '.join(paths)}), {threads} threads, {dt:.2f} s",
+                  f"({', '.join(paths)}), {threads} threads, {dt:.2f} s wall = {dt * threads:.0f} thread-seconds",
     }
 
 
@@ -565,7 +565,7 @@ def main() -> int:
     ap.add_argument("--kind", default="tweets", choices=["tweets", "adversarial"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--paths", default=",".join(PLAIN_PATHS))
-    ap.add_argument("--cpu-sample", type=int, default=512 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
