@@ -12,7 +12,16 @@
 //                     atomic ticket (the device work queue of tile ranges); a
 //                     tile finds its file by binary search over the files'
 //                     first tile ids, and the lookback stops at the file's first
-//                     tile, which starts from the initial carry
+//                     tile, which starts from the initial carry.  A tile holding
+//                     a newline speculates its carry-in (every '\n' of NDJSON
+//                     lies outside strings at depth -1) and writes its newline
+//                     offsets to its own scratch row
+//   bookkeeping       four small kernels over all tiles: every carry-in checked
+//                     against its predecessor's carry-out, per-file line counts,
+//                     newline offsets to their slots; a wrong speculation (input
+//                     that is not NDJSON) raises a flag that lets the same
+//                     reset + index + bookkeeping run again without speculation
+//                     (those launches return at once when the flag is clear)
 //   batch plan kernel per-file line counts -> per-(file, query) work ranges
 //   interpreter       one launch over every (file, query, line) item
 // After the first (uncaptured) run has sized the arrays, the sequence is
@@ -41,6 +50,9 @@ struct gpjson_batch {
   // device memory
   DevBuf arena_rec, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
   DevBuf arena_long;      // [2 counters, 64 B][long-line work items]
+  DevBuf nl_scratch;      // per tile: a slot per word for its newline offsets
+  DevBuf bk;              // bookkeeping: [flags, 128 B][block sums][per-file newline base]
+  int64_t nl_scratch_cap = 0;
   int64_t long_cap = 0;
   int64_t* total_dev = nullptr;
   int64_t* lines_dev = nullptr;
@@ -120,6 +132,9 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const ui
   for (int j = 0; j < b->njobs; j++) b->long_cap += b->bufs[b->job_file[j]]->n / kLongLineBytes + 1;
   if (!rc) rc = b->arena_long.ensure(64 + (size_t)b->long_cap * 8);
   if (!rc) rc = b->states.ensure((size_t)std::max<int64_t>(tiles, 1) * sizeof(TileState));
+  b->nl_scratch_cap = tile_bytes / 64;
+  if (!rc) rc = b->nl_scratch.ensure((size_t)std::max<int64_t>(tiles, 1) * (size_t)b->nl_scratch_cap * 8);
+  if (!rc) rc = b->bk.ensure(128 + (size_t)(tiles / 256 + 2) * 8 + (size_t)b->nfiles * 8);
   if (!rc) rc = b->dev_files.ensure(sizeof(FileJob) * (size_t)b->nfiles);
   if (!rc) rc = b->dev_qjobs.ensure(sizeof(QueryJob) * (size_t)b->njobs);
   if (!rc) rc = b->dev_misc.ensure(4096 + (size_t)b->nfiles * 8 + (size_t)b->njobs * 4);
@@ -195,15 +210,33 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   L.total_tiles = b->total_tiles;
   L.states = (TileState*)b->states.p;
   L.ticket = b->ticket;
+  L.nl_scratch = (int64_t*)b->nl_scratch.p;
+  L.nl_scratch_cap = b->nl_scratch_cap;
+  // bookkeeping flags: [0] a speculated carry-in was wrong (it gates the second
+  // pass), [8] a tile overflowed its scratch row, [16] the second pass's check
+  // (not read)
+  int* flags = (int*)b->bk.p;
+  int64_t* block_sums = (int64_t*)((uint8_t*)b->bk.p + 128);
+  int64_t* file_base = block_sums + b->total_tiles / 256 + 2;
   int maxL = 0;
   for (const auto& f : b->files) maxL = std::max(maxL, f.L);
+  const int nlv = nlv_for_levels(maxL);
   GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s));
+  GJ_HIP(zero_u64((unsigned long long*)flags, 16, s));
   GJ_HIP(zero_u64(b->counter, 1, s));
   if (e0) GJ_HIP(hipEventRecord(e0, s));
-  if (b->total_tiles > 0) GJ_HIP(index_launch(L, b->itune.block, b->itune.wpt, nlv_for_levels(maxL), false,
-                                              b->index_grid, s));
+  if (b->total_tiles > 0)
+    GJ_HIP(index_launch(L, b->itune.block, b->itune.wpt, nlv, false, b->index_grid, s));
+  GJ_HIP(index_finish_batch(L, block_sums, file_base, b->lines_dev, flags, flags + 8, s));
+  IndexLaunch L2 = L;
+  L2.speculate = 0;
+  L2.run_if = flags;
+  GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s, flags));
+  if (b->total_tiles > 0)
+    GJ_HIP(index_launch(L2, b->itune.block, b->itune.wpt, nlv, false, b->index_grid, s));
+  GJ_HIP(index_finish_batch(L2, block_sums, file_base, b->lines_dev, flags + 16, flags + 8, s));
   if (e1) GJ_HIP(hipEventRecord(e1, s));
-  GJ_HIP(batch_plan((const FileJob*)b->dev_files.p, b->nfiles, L.states, (QueryJob*)b->dev_qjobs.p, b->job_file_dev,
+  GJ_HIP(batch_plan((const FileJob*)b->dev_files.p, b->nfiles, (QueryJob*)b->dev_qjobs.p, b->job_file_dev,
                     b->njobs, b->total_dev, b->lines_dev, b->overflow_dev, s));
   QueryLaunch Q{};
   Q.jobs = (const QueryJob*)b->dev_qjobs.p;
@@ -292,8 +325,8 @@ int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, co
   bool ov = false;
   if (!rc) rc = read_back_lines(b.get(), &ov);
   if (!rc && ov) {
-    // the lookback states hold every file's true line count even when its
-    // newline array overflowed: lay out again with exact capacities
+    // the bookkeeping counts every file's lines even when its newline array
+    // overflowed: lay out again with exact capacities
     for (int f = 0; f < nfiles; f++) caps[f] = std::max<int64_t>(b->lines[f], caps[f]);
     rc = layout(b.get(), caps, irs, ir_lens);
     if (!rc) rc = enqueue_batch(b.get(), nullptr, nullptr, nullptr);
@@ -390,7 +423,8 @@ void gpjson_batch_free(gpjson_batch* b) {
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&b->arena_rec, &b->arena_lb, &b->arena_nl, &b->arena_spans,
-                    &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states, &b->arena_long};
+                    &b->dev_files, &b->dev_qjobs, &b->dev_misc, &b->dev_ir, &b->states, &b->arena_long,
+                    &b->nl_scratch, &b->bk};
   for (DevBuf* d : bufs) d->release();
   delete b;
 }

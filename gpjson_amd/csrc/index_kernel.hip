@@ -989,6 +989,147 @@ __global__ __launch_bounds__(256) void index_tile_overflow_kernel(IndexLaunch L,
   }
 }
 
+// The same four steps for a launch over many files (L.jobs, in tile order:
+// file f owns tiles tile0 .. tile0 + ntiles - 1 of the launch).  The block
+// sums and their scan run over all tiles; a file's newlines before one of its
+// tiles are the scan there minus the scan at the file's first tile.
+
+// The file holding tile t: the last with tile0 <= t (an empty file shares
+// tile0 with the file after it, so it is never the last such file of a tile).
+__device__ __forceinline__ int file_of_tile(const FileJob* jobs, int64_t njobs, int64_t t) {
+  int lo = 0, hi = (int)njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Step 1: as index_tile_sums_kernel; a file's first tile has no predecessor
+// to check against.
+__global__ __launch_bounds__(kTileBlock) void batch_tile_sums_kernel(IndexLaunch L, int64_t* block_sums,
+                                                                     int* mismatch) {
+  if (L.run_if && *L.run_if == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * kTileBlock + threadIdx.x;
+  int cnt = 0;
+  if (t < L.total_tiles) {
+    const TileState* st = L.states + t;
+    cnt = (int)(st->agg0 & kField24);
+    if (t > L.jobs[file_of_tile(L.jobs, L.njobs, t)].tile0) {
+      bool ps, cs;
+      const Carry e = apply_agg(decode_agg(st[-1].agg0, st[-1].agg1), load_carry_in(st - 1, &ps));
+      const Carry c = load_carry_in(st, &cs);
+      if (c.s != e.s || c.d != e.d || (!cs && c.p != e.p)) atomicOr(mismatch, 1);
+    }
+  }
+  int excl;
+  const int total = tile_block_scan(cnt, &excl);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Step 2 (one block): exclusive scan of the block sums in place, then per
+// file the newlines before its first tile (file_base) and its line count.
+__global__ __launch_bounds__(kTileBlock) void batch_tile_base_kernel(IndexLaunch L, int64_t* block_sums, int64_t nb,
+                                                                     int64_t* file_base, int64_t* lines_out) {
+  if (L.run_if && *L.run_if == 0) return;
+  __shared__ int64_t s_total;
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += kTileBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const int v = i < nb ? (int)block_sums[i] : 0;
+    int excl;
+    const int total = tile_block_scan(v, &excl);
+    if (i < nb) block_sums[i] = carry + excl;
+    carry += total;
+  }
+  if (threadIdx.x == 0) s_total = carry;
+  __syncthreads();
+  const int64_t total = s_total;
+  // newlines in the tiles before tile x (x <= total_tiles)
+  auto newlines_before = [&](int64_t x) -> int64_t {
+    const int64_t blk = x / kTileBlock;
+    if (blk >= nb) return total;
+    int64_t v = block_sums[blk];
+    for (int64_t i = blk * kTileBlock; i < x; i++) v += (int64_t)(L.states[i].agg0 & kField24);
+    return v;
+  };
+  for (int64_t f = threadIdx.x; f < L.njobs; f += kTileBlock) {
+    const FileJob& J = L.jobs[f];
+    if (J.ntiles == 0) {  // empty file: one line starting at 0
+      if (J.nl_cap > 0) J.NL[0] = 0;
+      if (J.tile_lines) J.tile_lines[0] = 1;
+      file_base[f] = 0;
+      lines_out[f] = 1;
+      continue;
+    }
+    const int64_t b0 = newlines_before(J.tile0);
+    const int64_t lines = 1 + newlines_before(J.tile0 + J.ntiles) - b0;
+    file_base[f] = b0;
+    if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
+    lines_out[f] = lines;
+  }
+}
+
+// Step 3: as index_tile_lines_kernel.  A tile that overflows its scratch row
+// leaves the newlines before it in the row's first slot for step 4.
+__global__ __launch_bounds__(kTileBlock) void batch_tile_lines_kernel(IndexLaunch L, const int64_t* block_base,
+                                                                      const int64_t* file_base, int* overflow) {
+  if (L.run_if && *L.run_if == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * kTileBlock + threadIdx.x;
+  const int cnt = t < L.total_tiles ? (int)(L.states[t].agg0 & kField24) : 0;
+  int excl;
+  tile_block_scan(cnt, &excl);
+  if (t >= L.total_tiles) return;
+  const int f = file_of_tile(L.jobs, L.njobs, t);
+  const FileJob& J = L.jobs[f];
+  const int64_t lt = t - J.tile0;
+  const int64_t before = block_base[blockIdx.x] + excl - file_base[f];  // the file's newlines before tile lt
+  if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : 1 + before;
+  int64_t* row = L.nl_scratch + t * L.nl_scratch_cap;
+  if (cnt > L.nl_scratch_cap) {
+    row[0] = before;
+    atomicOr(overflow, 1);
+    return;
+  }
+  for (int k = 0; k < cnt; k++)
+    if (1 + before + k < J.nl_cap) J.NL[1 + before + k] = row[k];
+}
+
+// Step 4 (only when *overflow): as index_tile_overflow_kernel.
+__global__ __launch_bounds__(256) void batch_tile_overflow_kernel(IndexLaunch L, const int* overflow) {
+  if ((L.run_if && *L.run_if == 0) || *overflow == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t cap = L.nl_scratch_cap;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < L.total_tiles; t += nwaves) {
+    const int64_t cnt = (int64_t)(L.states[t].agg0 & kField24);
+    if (cnt <= cap) continue;
+    const FileJob& J = L.jobs[file_of_tile(L.jobs, L.njobs, t)];
+    const int64_t lt = t - J.tile0;
+    int64_t slot = 1 + L.nl_scratch[t * cap];
+    const int64_t wb = lt * cap, we = min(wb + cap, J.W);
+    for (int64_t w0 = wb; w0 < we; w0 += 64) {
+      const int64_t w = w0 + lane;
+      uint64_t m = 0;
+      if (w < we)
+        for (int b = 0; b < 64; b++) {
+          const int64_t o = w * 64 + b;
+          if (o < J.N && J.F[o] == '\n') m |= 1ull << b;
+        }
+      const int c = popc64(m);
+      const int incl = wave_scan_int(c);
+      int64_t idx = slot + incl - c;
+      while (m) {
+        const int b = ctz64(m);
+        if (idx < J.nl_cap) J.NL[idx] = w * 64 + b;
+        idx++;
+        m &= m - 1;
+      }
+      slot += __shfl(incl, 63);
+    }
+  }
+}
+
 // Reads the line count of each file from its last tile's inclusive prefix.
 __global__ void index_lines_kernel(const FileJob* jobs, FileJob one, int64_t njobs, const TileState* states,
                                    int64_t* lines_out) {
@@ -1080,6 +1221,26 @@ hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, 
   if (grid < 1) grid = 1;
   if (grid > 4096) grid = 4096;
   hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket, run_if);
+  return hipGetLastError();
+}
+
+// Steps 1-4 over the files of a batched launch; block_sums holds one int64
+// per kTileBlock tiles of the launch, file_base and lines_out one per file.
+hipError_t index_finish_batch(const IndexLaunch& L, int64_t* block_sums, int64_t* file_base, int64_t* lines_out,
+                              int* mismatch, int* overflow, hipStream_t stream) {
+  const int64_t nt = L.total_tiles;
+  const int64_t nb = (nt + kTileBlock - 1) / kTileBlock;
+  const unsigned grid = (unsigned)(nb > 0 ? nb : 1);
+  if (nt > 0)
+    hipLaunchKernelGGL(batch_tile_sums_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L, block_sums, mismatch);
+  hipLaunchKernelGGL(batch_tile_base_kernel, dim3(1), dim3(kTileBlock), 0, stream, L, block_sums, nb, file_base,
+                     lines_out);
+  if (nt > 0) {
+    hipLaunchKernelGGL(batch_tile_lines_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L,
+                       (const int64_t*)block_sums, (const int64_t*)file_base, overflow);
+    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 1024);
+    hipLaunchKernelGGL(batch_tile_overflow_kernel, dim3(ogrid), dim3(256), 0, stream, L, (const int*)overflow);
+  }
   return hipGetLastError();
 }
 

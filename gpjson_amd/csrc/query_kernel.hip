@@ -691,25 +691,17 @@ __global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, 
 }
 
 // Batched runs: fills each query job's line count and first work item from the
-// index launch's per-file line counts (last tile's inclusive prefix), so the
-// interpreter launch needs no host round trip.  One block.
-__global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs,
-                                  const int* job_file, int njobs, int64_t* total_out, int64_t* lines_out,
-                                  int* overflow) {
+// per-file line counts of the newline bookkeeping, so the interpreter launch
+// needs no host round trip.  One thread.
+__global__ void batch_plan_kernel(const FileJob* files, int nfiles, QueryJob* qjobs, const int* job_file, int njobs,
+                                  int64_t* total_out, const int64_t* lines_in, int* overflow) {
   if (threadIdx.x != 0) return;
+  for (int f = 0; f < nfiles; f++)
+    if (lines_in[f] > files[f].nl_cap) *overflow = 1;
   int64_t item = 0;
-  for (int f = 0; f < nfiles; f++) {
-    const FileJob& J = files[f];
-    int64_t lines = 1;
-    if (J.ntiles > 0) lines = (int64_t)(states[J.tile0 + J.ntiles - 1].inc1 & ~(1ull << 63));
-    else if (J.nl_cap > 0) J.NL[0] = 0;
-    if (lines > J.nl_cap) *overflow = 1;
-    if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
-    lines_out[f] = lines;  // true count; jobs are clamped to what the newline array holds
-  }
   for (int j = 0; j < njobs; j++) {
     const FileJob& J = files[job_file[j]];
-    int64_t lines = lines_out[job_file[j]];
+    int64_t lines = lines_in[job_file[j]];  // true count; jobs are clamped to what the newline array holds
     if (lines > J.nl_cap) lines = J.nl_cap;
     qjobs[j].lines = lines;
     qjobs[j].item0 = item;
@@ -718,10 +710,10 @@ __global__ void batch_plan_kernel(const FileJob* files, int nfiles, const TileSt
   *total_out = item;
 }
 
-hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
-                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream) {
-  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, states, qjobs, job_file, njobs,
-                     total_out, lines_out, overflow);
+hipError_t batch_plan(const FileJob* files, int nfiles, QueryJob* qjobs, const int* job_file, int njobs,
+                      int64_t* total_out, const int64_t* lines, int* overflow, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, qjobs, job_file, njobs,
+                     total_out, lines, overflow);
   return hipGetLastError();
 }
 

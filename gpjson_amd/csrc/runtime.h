@@ -24,6 +24,10 @@ hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t str
 // per 256 tiles; *overflow must start at 0.
 hipError_t index_finish(const IndexLaunch& L, int64_t* block_sums, int64_t* lines_out, int* mismatch,
                         int* overflow, hipStream_t stream);
+// The same for a launch over many files (IndexLaunch.jobs): file_base and
+// lines_out hold one int64 per file.
+hipError_t index_finish_batch(const IndexLaunch& L, int64_t* block_sums, int64_t* file_base, int64_t* lines_out,
+                              int* mismatch, int* overflow, hipStream_t stream);
 bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
 hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream,
@@ -41,8 +45,10 @@ hipError_t gather_phase1(const int64_t* spans, int64_t n, int64_t N, int64_t* bl
 hipError_t gather_phase2(const uint8_t* F, int64_t N, const int64_t* spans, int64_t n, const int64_t* block_sums,
                          int64_t* out_spans, uint8_t* out_bytes, int copy_grid, hipStream_t stream);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
-hipError_t batch_plan(const FileJob* files, int nfiles, const TileState* states, QueryJob* qjobs, const int* job_file,
-                      int njobs, int64_t* total_out, int64_t* lines_out, int* overflow, hipStream_t stream);
+// Per-file line counts (from index_finish_batch) to query jobs; *overflow = 1
+// when a file has more lines than its newline array holds.
+hipError_t batch_plan(const FileJob* files, int nfiles, QueryJob* qjobs, const int* job_file, int njobs,
+                      int64_t* total_out, const int64_t* lines, int* overflow, hipStream_t stream);
 
 // Error reporting: every C entry point funnels failures through these.
 int fail(int code, const std::string& msg);

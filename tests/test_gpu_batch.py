@@ -1,5 +1,11 @@
 """Batched execution equals the plain path bit for bit (every file, every query),
-both for direct runs and for hipGraph replays with eager work in between."""
+both for direct runs and for hipGraph replays with eager work in between.
+
+Besides tweets and adversarial files of uneven size, the batch holds a byte
+soup that is not NDJSON (a speculated carry-in goes wrong, so the batch's
+second, non-speculating index pass runs), a file of 9-byte lines (every tile
+overflows its newline scratch row, and the file overflows the first newline
+array size, so the batch is laid out again), and an empty file."""
 import sys
 
 import numpy as np
@@ -23,6 +29,10 @@ def _files(seed, nfiles):
         if f == 5:
             size = 17  # tiny file: one partial tile
         out.append((kind, datagen.generate(kind, 1000 + f, size).tobytes()))
+    soup = bytes(np.frombuffer(b'ab"\\\n{}[]:, x', dtype=np.uint8)[rng.integers(0, 13, 200_000)])
+    out.insert(3, ("soup", soup))
+    out.insert(7, ("short", b"".join(b'{"id":%d}\n' % (i % 10) for i in range(33_000))))
+    out.insert(9, ("empty", b""))
     return out
 
 
@@ -41,19 +51,21 @@ def _batch_equals_plain(ctx):
     from gpjson_amd import compile_path
     files = _files(7, 12)
     bufs = [ctx.stage(d) for _, d in files]
-    qsets = [[compile_path(p) for p in (TWEET_Q if k == "tweets" else ADV_Q)] for k, _ in files]
+    qsets = [[compile_path(p) for p in (ADV_Q if k == "adversarial" else TWEET_Q)] for k, _ in files]
     print("[batch] building", flush=True)
     batch = ctx.batch(bufs, [[c.ir for c in qs] for qs in qsets])
     # plain references
-    plain = []
+    plain, plain_lines = [], []
     for (k, d), buf, qs in zip(files, bufs, qsets):
         L = max(c.max_depth for c in qs)
         idx = ctx.index(buf, L)
         plain.append([ctx.run_queries(idx, [c.ir], c.num_results)[0] for c in qs])
+        plain_lines.append(idx.lines)
         idx.free()
 
     def check(tag):
         for f in range(len(files)):
+            assert batch.lines(f) == plain_lines[f], (tag, f, files[f][0])
             for q in range(len(qsets[f])):
                 got = batch.spans(f, q)
                 assert np.array_equal(got, plain[f][q]), (tag, f, q)
