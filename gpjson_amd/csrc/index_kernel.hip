@@ -1045,28 +1045,37 @@ __global__ __launch_bounds__(kTileBlock) void batch_tile_base_kernel(IndexLaunch
   if (threadIdx.x == 0) s_total = carry;
   __syncthreads();
   const int64_t total = s_total;
-  // newlines in the tiles before tile x (x <= total_tiles)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // newlines in the tiles before tile x (x <= total_tiles; x the same in every
+  // lane): the scan at x's block plus the counts of the block's tiles before
+  // x, summed over the wavefront (at most 255 counts of at most 65536)
   auto newlines_before = [&](int64_t x) -> int64_t {
     const int64_t blk = x / kTileBlock;
     if (blk >= nb) return total;
-    int64_t v = block_sums[blk];
-    for (int64_t i = blk * kTileBlock; i < x; i++) v += (int64_t)(L.states[i].agg0 & kField24);
-    return v;
+    int v = 0;
+    for (int64_t i = blk * kTileBlock + lane; i < x; i += 64) v += (int)(L.states[i].agg0 & kField24);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return block_sums[blk] + v;
   };
-  for (int64_t f = threadIdx.x; f < L.njobs; f += kTileBlock) {
+  // a wavefront per file
+  for (int64_t f = wave; f < L.njobs; f += kTileBlock / 64) {
     const FileJob& J = L.jobs[f];
     if (J.ntiles == 0) {  // empty file: one line starting at 0
-      if (J.nl_cap > 0) J.NL[0] = 0;
-      if (J.tile_lines) J.tile_lines[0] = 1;
-      file_base[f] = 0;
-      lines_out[f] = 1;
+      if (lane == 0) {
+        if (J.nl_cap > 0) J.NL[0] = 0;
+        if (J.tile_lines) J.tile_lines[0] = 1;
+        file_base[f] = 0;
+        lines_out[f] = 1;
+      }
       continue;
     }
     const int64_t b0 = newlines_before(J.tile0);
     const int64_t lines = 1 + newlines_before(J.tile0 + J.ntiles) - b0;
-    file_base[f] = b0;
-    if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
-    lines_out[f] = lines;
+    if (lane == 0) {
+      file_base[f] = b0;
+      if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
+      lines_out[f] = lines;
+    }
   }
 }
 

@@ -692,27 +692,46 @@ __global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, 
 
 // Batched runs: fills each query job's line count and first work item from the
 // per-file line counts of the newline bookkeeping, so the interpreter launch
-// needs no host round trip.  One thread.
-__global__ void batch_plan_kernel(const FileJob* files, int nfiles, QueryJob* qjobs, const int* job_file, int njobs,
-                                  int64_t* total_out, const int64_t* lines_in, int* overflow) {
-  if (threadIdx.x != 0) return;
-  for (int f = 0; f < nfiles; f++)
+// needs no host round trip.  One block of kPlanBlock threads: the jobs' line
+// counts are scanned kPlanBlock at a time.
+constexpr int kPlanBlock = 256;
+
+__global__ __launch_bounds__(kPlanBlock) void batch_plan_kernel(const FileJob* files, int nfiles, QueryJob* qjobs,
+                                                                const int* job_file, int njobs, int64_t* total_out,
+                                                                const int64_t* lines_in, int* overflow) {
+  __shared__ int64_t s_scan[kPlanBlock];
+  const int tid = threadIdx.x;
+  for (int f = tid; f < nfiles; f += kPlanBlock)
     if (lines_in[f] > files[f].nl_cap) *overflow = 1;
-  int64_t item = 0;
-  for (int j = 0; j < njobs; j++) {
-    const FileJob& J = files[job_file[j]];
-    int64_t lines = lines_in[job_file[j]];  // true count; jobs are clamped to what the newline array holds
-    if (lines > J.nl_cap) lines = J.nl_cap;
-    qjobs[j].lines = lines;
-    qjobs[j].item0 = item;
-    item += lines;
+  int64_t carry = 0;
+  for (int j0 = 0; j0 < njobs; j0 += kPlanBlock) {
+    const int j = j0 + tid;
+    int64_t lines = 0;
+    if (j < njobs) {
+      const int f = job_file[j];
+      lines = lines_in[f] < files[f].nl_cap ? lines_in[f] : files[f].nl_cap;  // jobs stop at what the newline array holds
+    }
+    s_scan[tid] = lines;
+    __syncthreads();
+    for (int off = 1; off < kPlanBlock; off <<= 1) {  // inclusive scan in place
+      const int64_t x = tid >= off ? s_scan[tid - off] : 0;
+      __syncthreads();
+      s_scan[tid] += x;
+      __syncthreads();
+    }
+    if (j < njobs) {
+      qjobs[j].lines = lines;
+      qjobs[j].item0 = carry + s_scan[tid] - lines;
+    }
+    carry += s_scan[kPlanBlock - 1];
+    __syncthreads();  // the next chunk overwrites s_scan
   }
-  *total_out = item;
+  if (tid == 0) *total_out = carry;
 }
 
 hipError_t batch_plan(const FileJob* files, int nfiles, QueryJob* qjobs, const int* job_file, int njobs,
                       int64_t* total_out, const int64_t* lines, int* overflow, hipStream_t stream) {
-  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(64), 0, stream, files, nfiles, qjobs, job_file, njobs,
+  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(kPlanBlock), 0, stream, files, nfiles, qjobs, job_file, njobs,
                      total_out, lines, overflow);
   return hipGetLastError();
 }
