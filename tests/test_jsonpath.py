@@ -2,7 +2,8 @@
 
 Expected bytes re-encode JSONPathParserTest.java:35-42 (IRBuilder sequences;
 opcode = Opcode ordinal, LEB128 varints, strings as varint length + UTF-8),
-plus the BASELINE query shapes; unsupported paths from :56-58.
+plus the benchmark workloads' query shapes (member chains, ranges, deep
+chains); unsupported paths from :56-58.
 """
 import pytest
 
