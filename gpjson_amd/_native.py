@@ -36,6 +36,9 @@ GPJSON_QUERY_DISABLE_FALLBACK = 1
 GPJSON_QUERY_GATHER = 2
 GPJSON_MAX_LEVELS = 64
 GPJSON_MAX_QUERIES = 255
+GPJSON_NOALLOC_SCRATCH_BYTES = 32 << 20
+GPJSON_NOALLOC_MAX_QUERIES = 8
+GPJSON_NOALLOC_MAX_IR_BYTES = 1024
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -112,6 +115,8 @@ SIGNATURES = {
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),
+    "gpjson_query_noalloc": (c_int, [c_void_p, c_void_p, c_int64, P(c_void_p), P(c_int64), c_int, c_void_p, c_int64,
+                                     c_void_p, c_int64, c_void_p]),
     "gpjson_time_plain": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, P(c_double),
                                   P(c_double), P(c_double)]),
 }

@@ -28,6 +28,7 @@ SOURCES = [
     "diag.hip",
     "chain_kernel.hip",
     "gather.hip",
+    "noalloc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

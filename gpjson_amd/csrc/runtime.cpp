@@ -218,36 +218,41 @@ static int index_setup(gpjson_index* idx, const IndexTune& t, IndexLaunch* out) 
   return GPJSON_OK;
 }
 
-int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
-  gpjson_ctx* ctx = idx->ctx;
-  const IndexTune& t = tune ? *tune : ctx->itune;
-  IndexLaunch L{};
-  int rc = index_setup(idx, t, &L);
-  if (rc) return rc;
+int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels, bool verify, int* flags,
+                           int64_t* lines_dev, hipStream_t stream, hipEvent_t before_kernel, hipEvent_t after_kernel) {
   const int64_t ntiles = L.total_tiles;
-  const int nlv = nlv_for_levels(idx->L);
-  const bool verify = (idx->flags & GPJSON_BUILD_VERIFY) != 0;
-  // bookkeeping flags, after the ticket: [16] a speculated carry-in was wrong
-  // (it gates the second pass), [24] a tile overflowed its scratch row, [32]
-  // the second pass's check (not read)
-  int* flags = (int*)ctx->ticket.p + 16;
+  const int nlv = nlv_for_levels(levels);
   int64_t* block_sums = L.nl_scratch + ntiles * L.nl_scratch_cap;
-  GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream));
-  GJ_HIP(zero_u64((unsigned long long*)flags, 8, ctx->stream));
-  if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  if (ntiles > 0) GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)ntiles, ctx->stream));
-  if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
-  GJ_HIP(index_finish(L, block_sums, idx->lines_dev, flags, flags + 8, ctx->stream));
+  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream));
+  GJ_HIP(zero_u64((unsigned long long*)flags, 8, stream));
+  if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, stream));
+  if (ntiles > 0) GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)ntiles, stream));
+  if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, stream));
+  GJ_HIP(index_finish(L, block_sums, lines_dev, flags, flags + 8, stream));
   // A wrong speculation (input that is not NDJSON somewhere): the same build
   // again with every tile taking the lookback.  These launches return at once
   // unless the check above raised its flag, so no host round trip is needed.
   IndexLaunch L2 = L;
   L2.speculate = 0;
   L2.run_if = flags;
-  GJ_HIP(index_reset(L.states, ntiles, L.ticket, ctx->stream, flags));
-  if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)ntiles, ctx->stream));
-  GJ_HIP(index_finish(L2, block_sums, idx->lines_dev, flags + 16, flags + 8, ctx->stream));
+  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream, flags));
+  if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)ntiles, stream));
+  GJ_HIP(index_finish(L2, block_sums, lines_dev, flags + 16, flags + 8, stream));
   return GPJSON_OK;
+}
+
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
+  gpjson_ctx* ctx = idx->ctx;
+  const IndexTune& t = tune ? *tune : ctx->itune;
+  IndexLaunch L{};
+  int rc = index_setup(idx, t, &L);
+  if (rc) return rc;
+  // bookkeeping flags, after the ticket: [16] a speculated carry-in was wrong
+  // (it gates the second pass), [24] a tile overflowed its scratch row, [32]
+  // the second pass's check (not read)
+  int* flags = (int*)ctx->ticket.p + 16;
+  return enqueue_index_launches(L, t, idx->L, (idx->flags & GPJSON_BUILD_VERIFY) != 0, flags, idx->lines_dev,
+                                ctx->stream, before_kernel, after_kernel);
 }
 
 int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
@@ -419,6 +424,11 @@ int gpjson_ctx_create(int device, gpjson_ctx** out) {
   ctx->device = device;
   GJ_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   GJ_HIP(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int rc = noalloc_scratch_setup(device, &ctx->noalloc_scratch);
+  if (rc) {
+    (void)hipStreamDestroy(ctx->stream);
+    return rc;
+  }
   *out = ctx.release();
   return GPJSON_OK;
 }
@@ -1073,8 +1083,9 @@ void gpjson_gathered_free(gpjson_gathered* g) { delete g; }
 
 }  // extern "C"
 
+namespace gj {
 // Levels and result slots a set of IRs needs: DOWN nesting depth and STORE count.
-static void ir_shape(const uint8_t* const* irs, const int64_t* ir_lens, int nq, int* L_out, int* R_out) {
+void ir_shape(const uint8_t* const* irs, const int64_t* ir_lens, int nq, int* L_out, int* R_out) {
   int L = 0, R = 1;
   for (int q = 0; q < nq; q++) {
     // levels / results are recovered from the IR: count DOWN depth and STOREs
@@ -1102,6 +1113,7 @@ static void ir_shape(const uint8_t* const* irs, const int64_t* ir_lens, int nq, 
   *L_out = std::min(L, GPJSON_MAX_LEVELS);
   *R_out = R;
 }
+}  // namespace gj
 
 struct gpjson_plan {
   gpjson_ctx* ctx = nullptr;

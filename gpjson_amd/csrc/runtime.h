@@ -119,6 +119,7 @@ struct gpjson_ctx {
   size_t host_scratch_size = 0;
   gj::IndexTune itune;
   gj::QueryTune qtune;
+  void* noalloc_scratch = nullptr;  // this device's fixed scratch array (gpjson_query_noalloc)
 };
 
 struct gpjson_buf {
@@ -175,6 +176,18 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel = nullptr, hipEven
 // read on the device; spans_dev receives int64[nq][lines][R][2].
 int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,
                         const int64_t* ir_lens, int nq, int R, int64_t* spans_dev);
+// The launches of one index build over prepared arrays (no allocation): reset,
+// index kernel, newline bookkeeping, and the second pass that runs only after
+// a wrong speculation.  flags: 16 ints of device memory (zeroed here); the line
+// count lands in *lines_dev.
+int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels, bool verify, int* flags,
+                           int64_t* lines_dev, hipStream_t stream, hipEvent_t before_kernel = nullptr,
+                           hipEvent_t after_kernel = nullptr);
+// Levels (deepest DOWN nesting) and result slots (STOREs) a set of IRs needs.
+void ir_shape(const uint8_t* const* irs, const int64_t* ir_lens, int nq, int* L_out, int* R_out);
+// noalloc.hip: the device address of this device's fixed scratch array (the
+// current device must be `device`); looked up once per device.
+int noalloc_scratch_setup(int device, void** out);
 int upload_irs(gpjson_ctx* ctx, const uint8_t* const* irs, const int64_t* ir_lens, int nq,
                std::vector<int64_t>* offsets, const uint8_t** dev_base);
 }  // namespace gj

@@ -222,6 +222,37 @@ int gpjson_plan_spans(gpjson_plan *plan, int64_t *spans, int64_t cap_elems);
 const gpjson_index *gpjson_plan_index(const gpjson_plan *plan);
 void gpjson_plan_free(gpjson_plan *plan);
 
+/* ---- plain path without allocation (no counterpart in the reference) ------ */
+
+#define GPJSON_NOALLOC_SCRATCH_BYTES (32LL << 20) /* fixed scratch array per GPU */
+#define GPJSON_NOALLOC_MAX_QUERIES 8
+#define GPJSON_NOALLOC_MAX_IR_BYTES 1024          /* all queries' IR bytes together */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the allocation-free variant of gpjson_plan_create +
+ * gpjson_plan_run, running the same index and interpreter kernels.  Indexes
+ * the n bytes at `input` (device memory of ctx's device) for the queries'
+ * depth and runs the nq (<= GPJSON_NOALLOC_MAX_QUERIES) compiled queries over
+ * every line in one interpreter launch.
+ *  - Scratch (tile states, newline scratch rows, records, leveled bitmaps and
+ *    summaries, the job table and IR bytes) lives in one fixed device array of
+ *    GPJSON_NOALLOC_SCRATCH_BYTES per GPU, held from the first context created
+ *    on that GPU until the process ends.  An input whose index needs more
+ *    (about 42 MiB at 2 levels, 21 MiB at 8) fails with GPJSON_ERR_CAPACITY
+ *    before anything is launched.
+ *  - Outputs are caller-owned device buffers: newlines int64[newline_cap]
+ *    gets the newline index; spans int64[nq][lines][R][2] (R = the largest
+ *    STORE count of the queries; capacity span_cap elements); info int64[2]
+ *    gets the line count and 1 if it exceeds min(newline_cap,
+ *    span_cap / (nq * R * 2)), in which case no query runs.
+ *  - Enqueued on the context's stream without a host wait; calls no
+ *    allocator.  Every call on one GPU uses that GPU's one array: calls that
+ *    overlap on one GPU (two contexts or two threads) race. */
+int gpjson_query_noalloc(gpjson_ctx *ctx, const uint8_t *input, int64_t n, const uint8_t *const *irs,
+                         const int64_t *ir_lens, int nq, int64_t *newlines, int64_t newline_cap, int64_t *spans,
+                         int64_t span_cap, int64_t *info);
+
 /* ---- batched execution (no counterpart in the reference) ------------------ */
 
 /* Prepares a batch of nfiles resident buffers.  File f runs nq_per_file[f]
