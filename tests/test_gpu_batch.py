@@ -1,5 +1,6 @@
-"""Batched execution equals the plain path bit for bit (every file, every query),
-both for direct runs and for hipGraph replays with eager work in between.
+"""Batched execution equals the oracle and the plain path bit for bit (every
+file, every query), both for direct runs and for hipGraph replays with eager
+work in between.
 
 Besides tweets and adversarial files of uneven size, the batch holds a byte
 soup that is not NDJSON (a speculated carry-in goes wrong, so the batch's
@@ -36,22 +37,37 @@ def _files(seed, nfiles):
     return out
 
 
+@pytest.fixture(scope="module")
+def batch_inputs():
+    """The files, their compiled queries and the oracle's spans per (file, query)."""
+    from gpjson_amd import compile_path
+    from oracle import oracle as O
+    files = _files(7, 12)
+    qsets = [[compile_path(p) for p in (ADV_Q if k == "adversarial" else TWEET_Q)] for k, _ in files]
+    ref = []
+    for (k, d), qs in zip(files, qsets):
+        L = max(c.max_depth for c in qs)
+        nl = O.newline_index(d)
+        s = O.string_index(d)
+        lb = O.leveled_index(d, s, L)
+        ref.append([O.find_value(d, nl, s, lb, L, c.ir, c.num_results).reshape(nl.size, c.num_results, 2)
+                    for c in qs])
+    return files, qsets, ref
+
+
 @pytest.mark.parametrize("mode", [0, 3])
-def test_batch_equals_plain(ctx, mode):
+def test_batch_equals_plain(ctx, mode, batch_inputs):
     cfg = ctx.get_config()
     try:
         ctx.set_config(cfg["index_block"], cfg["index_wpt"], mode, cfg["query_block"], cfg["query_group"],
                        cfg["query_waves_per_cu"], cfg["lines_per_group"])
-        _batch_equals_plain(ctx)
+        _batch_equals_plain(ctx, *batch_inputs)
     finally:
         ctx.set_config(*cfg.values())
 
 
-def _batch_equals_plain(ctx):
-    from gpjson_amd import compile_path
-    files = _files(7, 12)
+def _batch_equals_plain(ctx, files, qsets, oracle):
     bufs = [ctx.stage(d) for _, d in files]
-    qsets = [[compile_path(p) for p in (ADV_Q if k == "adversarial" else TWEET_Q)] for k, _ in files]
     print("[batch] building", flush=True)
     batch = ctx.batch(bufs, [[c.ir for c in qs] for qs in qsets])
     # plain references
@@ -63,11 +79,16 @@ def _batch_equals_plain(ctx):
         plain_lines.append(idx.lines)
         idx.free()
 
+    for f in range(len(files)):
+        for q in range(len(qsets[f])):
+            assert np.array_equal(plain[f][q], oracle[f][q]), ("plain vs oracle", f, q, files[f][0])
+
     def check(tag):
         for f in range(len(files)):
             assert batch.lines(f) == plain_lines[f], (tag, f, files[f][0])
             for q in range(len(qsets[f])):
                 got = batch.spans(f, q)
+                assert np.array_equal(got, oracle[f][q]), (tag, "vs oracle", f, q, files[f][0])
                 assert np.array_equal(got, plain[f][q]), (tag, f, q)
 
     check("first run")

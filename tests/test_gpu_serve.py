@@ -1,5 +1,7 @@
-"""Serving: every served result equals the plain path bit for bit, alone and in a
-concurrent mix, with coalesced launches and LRU eviction under a tight budget."""
+"""Serving: every served result equals the oracle and the plain path bit for
+bit, alone and in a concurrent mix, with coalesced launches and LRU eviction
+under a tight budget.  (The `plain` fixture holds the oracle's spans, after
+checking that the plain path returns the same.)"""
 import threading
 
 import numpy as np
@@ -24,14 +26,22 @@ def files():
 
 @pytest.fixture(scope="module")
 def plain(ctx, files):
+    """Spans per (file, path): the oracle's, checked equal to the plain path's."""
     from gpjson_amd import compile_path
+    from oracle import oracle as O
     ref = {}
     for key, (kind, data) in files.items():
         buf = ctx.stage(data)
         idx = ctx.index(buf, 4)
+        nl = O.newline_index(data)
+        s = O.string_index(data)
+        lb = O.leveled_index(data, s, 4)
         for p in (PATHS if kind == "tweets" else ADV_PATHS):
             cp = compile_path(p)
-            ref[(key, p)] = ctx.run_queries(idx, [cp.ir], cp.num_results)[0]
+            got = ctx.run_queries(idx, [cp.ir], cp.num_results)[0]
+            want = O.find_value(data, nl, s, lb, 4, cp.ir, cp.num_results).reshape(nl.size, cp.num_results, 2)
+            assert np.array_equal(got, want), (key, p)
+            ref[(key, p)] = want
         idx.free()
         buf.free()
     return ref

@@ -288,3 +288,46 @@ def test_generated_tweets_answers_match_json():
         obj = json.loads(line)
         if "user" in obj:
             assert vals[i] == json.dumps(obj["user"]["lang"])
+
+
+_MISSING = object()
+
+
+def _walk(obj, keys):
+    """The value a member chain names in a parsed line, or _MISSING when a key
+    is missing or a value on the way is not an object (the interpreter then has
+    no structural character at the next level, so it stores nothing)."""
+    for k in keys:
+        if not isinstance(obj, dict) or k not in obj:
+            return _MISSING
+        obj = obj[k]
+    return obj
+
+
+@pytest.mark.parametrize("path", ["$.id", "$.place.country_code", "$.user.stats.followers"])
+def test_generated_tweets_headline_paths_match_json(path):
+    """The other headline paths on generated lines: every span parses (json.loads,
+    which allows the span's trailing whitespace) to the value json.loads finds in
+    the line; lines whose chain is broken (place is null on most lines) have no
+    result.  Both outcomes must occur for country_code."""
+    from gpjson_amd import _native, datagen
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("compiler library not built")
+    data = datagen.generate("tweets", 5, 256 << 10).tobytes()
+    vals = _query(data, path)
+    keys = path[2:].split(".")
+    seen = {"value": 0, "none": 0}
+    for i, line in enumerate(data.split(b"\n")):
+        if not line.strip():
+            assert vals[i] is None
+            continue
+        want = _walk(json.loads(line), keys)
+        if want is _MISSING:
+            assert vals[i] is None, (i, path, vals[i])
+            seen["none"] += 1
+        else:
+            assert vals[i] is not None and json.loads(vals[i]) == want, (i, path, vals[i], want)
+            seen["value"] += 1
+    assert seen["value"] > 0
+    if path == "$.place.country_code":
+        assert seen["none"] > seen["value"] > 0, seen
