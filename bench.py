@@ -9,17 +9,21 @@ value = input bytes processed per second over the K timed steps (GB = 1e9 B).
 Prints ONE JSON line.  The roofline object uses the index kernel (the dominant
 kernel): achieved = input bytes per launch / mean index-kernel duration from HIP
 events on the library's stream.  cpu_baseline times the multithreaded CPU
-evaluator (test oracle, kind "port") on a bounded prefix of the same input.
+evaluator (test oracle, kind "port") on a bounded prefix of the same input, on
+every CPU the process may use.  Streaming copy and read probes give the
+achievable HBM bandwidth beside the 8 TB/s peak.
 
-Other modes (not the default line):
-  --mode batch  256 files of lognormal size in [1 MiB, 512 MiB] (16 GiB), tweets
-                and adversarial alternating, 8 member-chain queries per file:
-                one batched hipGraph run versus 256 sequential plain runs,
-                checked bit for bit against the plain spans.
-  --mode serve  64 tweets files (100 GiB) resident under an HBM budget; closed
-                burst capacity, then Poisson open-loop arrivals at 3 rates over
-                12 member-chain paths (p50/p99 latency, GB/s), versus the plain
-                one-query-one-launch path.
+The same line carries the other two halves of the metric (leave them out with
+--no-batch / --no-serve; --mode batch / --mode serve print them as lines of
+their own):
+  batched  256 files of lognormal size in [1 MiB, 512 MiB] (16 GiB), tweets
+           and adversarial alternating, 8 member-chain queries per file: one
+           batched hipGraph run versus 256 sequential plain runs, checked bit
+           for bit against the plain spans and, on a sample of files, the oracle.
+  served   64 tweets files (100 GiB) resident under an HBM budget; closed
+           burst capacity, then Poisson open-loop arrivals at 3 rates over 12
+           member-chain paths (p50/p99 latency from the scheduled arrival,
+           GB/s), versus the plain one-query-one-launch path.
 """
 from __future__ import annotations
 
@@ -64,6 +68,8 @@ def cpu_baseline(data, paths, sample_bytes: int, threads: int):
         "value": round(len(paths) * sample.size / dt / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
+        "host_cpus_available": len(os.sched_getaffinity(0)),
+        "host_cpu_count": os.cpu_count(),
         "kind": "port",
         "sample": f"first {sample.size} bytes of the same input, {len(paths)} index+query passes "
                   f"({', '.join(paths)}), {threads} threads, {dt:.2f} s wall = {dt * threads:.0f} thread-seconds",
@@ -212,11 +218,17 @@ def run_plain(args) -> int:
         p.free()
     buf.free()
     copy_ms = ctx.time_copy(1 << 30, 5)
+    read_ms = ctx.time_read(1 << 30, 5)
+    ctx.close()
     result["hbm_copy_GBps"] = round(2 * (1 << 30) / (copy_ms / 1e3) / 1e9, 1)
+    result["hbm_read_GBps"] = round((1 << 30) / (read_ms / 1e3) / 1e9, 1)
     result["roofline"]["frac_of_copy"] = round(achieved / result["hbm_copy_GBps"], 4)
-    log(f"streaming copy of 1 GiB: {copy_ms:.3f} ms = {result['hbm_copy_GBps']} GB/s (read + write)")
+    result["roofline"]["frac_of_read"] = round(achieved / result["hbm_read_GBps"], 4)
+    log(f"streaming copy of 1 GiB: {copy_ms:.3f} ms = {result['hbm_copy_GBps']} GB/s (read + write); "
+        f"streaming read: {read_ms:.3f} ms = {result['hbm_read_GBps']} GB/s")
+    ok = one_shot_ok
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         log(f"cpu baseline on {min(args.cpu_sample, data.size)} bytes with {threads} threads")
         cpu_spans, result["cpu_baseline"] = cpu_baseline(data, paths, args.cpu_sample, threads)
         # the sample ends at a newline, so all but its last (empty) line are whole lines of the input
@@ -225,12 +237,25 @@ def run_plain(args) -> int:
             k = ref.size // (2 * got.shape[1]) - 1
             oracle_ok &= bool(np.array_equal(got[:k].reshape(-1), ref[:k * got.shape[1] * 2]))
         result["bit_exact_vs_oracle_on_sample"] = oracle_ok
+        ok &= oracle_ok
         log(f"GPU spans equal the oracle's on the sample's lines: {oracle_ok}")
     else:
         result["cpu_baseline"] = None
+    del data, plan_spans
+    if not args.no_batch:
+        result["batched"] = batch_section(args)
+        ok &= result["batched"]["bit_exact_vs_plain"] and result["batched"]["bit_exact_vs_oracle_sample"]
+    if not args.no_serve:
+        result["served"] = serve_section(args)
+        ok &= result["served"]["bit_exact_vs_plain"]
     print(json.dumps(result), flush=True)
-    ctx.close()
-    return 0 if one_shot_ok and result.get("bit_exact_vs_oracle_on_sample", True) else 1
+    return 0 if ok else 1
+
+
+def cpu_threads(args) -> int:
+    """Threads for the CPU evaluator: every CPU this process may run on, unless
+    --cpu-threads says otherwise."""
+    return args.cpu_threads or len(os.sched_getaffinity(0))
 
 
 def batch_sizes(n: int, total: int, seed: int, lo: int = 1 << 20, hi: int = 512 << 20):
@@ -245,30 +270,41 @@ def batch_sizes(n: int, total: int, seed: int, lo: int = 1 << 20, hi: int = 512 
     return [int(x) for x in np.clip(raw * scale, lo, hi)]
 
 
-def run_batch(args) -> int:
+def batch_section(args) -> dict:
     """256 files of uneven size (tweets and adversarial mixed), 8 member-chain queries
     per file: one batched run (persistent index launch + one interpreter launch,
-    replayed as a hipGraph) against 256 sequential plain runs (index + query per file)."""
+    replayed as a hipGraph) against 256 sequential plain runs (index + query per
+    file).  Every (file, query) is compared with the plain path bit for bit, and a
+    sample of files with the multithreaded oracle."""
     import numpy as np
 
     import gpjson_amd
     from gpjson_amd import compile_path, datagen
+    from oracle import oracle as O
 
-    sizes = batch_sizes(args.files, args.batch_total, args.seed)
-    kinds = ["tweets" if i % 2 == 0 else "adversarial" for i in range(args.files)]
-    total = sum(sizes)
-    log(f"batch: {args.files} files, {total} bytes (min {min(sizes)}, max {max(sizes)})")
+    nfiles = args.batch_files
+    sizes = batch_sizes(nfiles, args.batch_total, args.seed)
+    kinds = ["tweets" if i % 2 == 0 else "adversarial" for i in range(nfiles)]
+    log(f"batch: {nfiles} files, {sum(sizes)} bytes (min {min(sizes)}, max {max(sizes)})")
     ctx = gpjson_amd.Context(0)
-    bufs, irs, nbytes = [], [], 0
+    bufs, irs, cps_per_file, nbytes = [], [], [], 0
+    # the oracle's sample: the smallest file of each kind plus every 32nd file
+    sample = set(range(0, nfiles, 32)) | {int(np.argmin(sizes[0::2])) * 2}
+    if nfiles > 1:
+        sample.add(int(np.argmin(sizes[1::2])) * 2 + 1)
+    sample_data = {}
     t = time.perf_counter()
     for i, (kind, n) in enumerate(zip(kinds, sizes)):
         d = datagen.generate(kind, args.seed * 1000 + i, n)
         bufs.append(ctx.stage(d))
         nbytes += d.size
+        if i in sample:
+            sample_data[i] = d
         del d
         ps = TWEET_PATHS_8 if kind == "tweets" else ADVERSARIAL_PATHS_8
-        irs.append([compile_path(p).ir for p in ps])
-        if i % 32 == 31:
+        cps_per_file.append([compile_path(p) for p in ps])
+        irs.append([c.ir for c in cps_per_file[-1]])
+        if i % 64 == 63:
             log(f"staged {i + 1} files, {nbytes / 1e9:.1f} GB, {time.perf_counter() - t:.1f} s")
     ctx.synchronize()
     if args.config:
@@ -280,7 +316,8 @@ def run_batch(args) -> int:
         for line in report.strip().splitlines():
             log("autotune: " + line)
         log(f"autotune on file {big} took {time.perf_counter() - t:.1f} s")
-    log(f"launch shapes: {ctx.get_config()}")
+    shapes = ctx.get_config()
+    log(f"launch shapes: {shapes}")
 
     # plain: one index + one interpreter launch (8 queries) per file, file after file
     plans = [ctx.plan(b, q) for b, q in zip(bufs, irs)]
@@ -288,24 +325,23 @@ def run_batch(args) -> int:
         p.run()
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.batch_steps):
         for p in plans:
             p.run()
     ctx.synchronize()
-    plain_dt = (time.perf_counter() - t0) / args.steps
+    plain_dt = (time.perf_counter() - t0) / args.batch_steps
     plain_gbps = nbytes / plain_dt / 1e9
     log(f"plain: {plain_dt * 1e3:.2f} ms per pass over all files, {plain_gbps:.1f} GB/s")
 
     batch = ctx.batch(bufs, irs)
-    for _ in range(max(args.warmup, 1)):
+    for _ in range(2):
         batch.run(graph=True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        batch.run(graph=True, timed=(k == args.steps - 1))
+    for k in range(args.batch_steps):
+        batch.run(graph=True, timed=(k == args.batch_steps - 1))
     ctx.synchronize()
-    dt = time.perf_counter() - t0
-    batch_dt = dt / args.steps
+    batch_dt = (time.perf_counter() - t0) / args.batch_steps
     value = nbytes / batch_dt / 1e9
     _, total_ms = batch.timing()
     log(f"batched: {batch_dt * 1e3:.2f} ms per run, {value:.1f} GB/s (events {total_ms:.2f} ms)")
@@ -318,60 +354,85 @@ def run_batch(args) -> int:
             if not np.array_equal(batch.spans(f, q), ps[q]):
                 mism += 1
     log(f"batched vs plain: {mism} mismatching (file, query) pairs of {sum(len(q) for q in irs)}")
-    if mism:
-        return 1
-
-    result = common_fields(args, value, dt)
-    result.update({
-        "data": f"synthetic seeded tweets + adversarial NDJSON (seed {args.seed}), resident in HBM",
-        "config": {
-            "workload": f"batched path: {args.files} files, lognormal sizes in [1 MiB, 512 MiB], {nbytes} bytes, "
-                        "alternating tweets/adversarial; 8 member-chain queries per file; one step = one "
-                        "hipGraph replay (persistent index launch + one interpreter launch)",
-            "input_bytes": nbytes,
-            "tweet_paths": TWEET_PATHS_8,
-            "adversarial_paths": ADVERSARIAL_PATHS_8,
-        },
-        "plain_GBps": round(plain_gbps, 3),
-        "plain_ms_per_pass": round(plain_dt * 1e3, 3),
-        "batched_over_plain": round(value / plain_gbps, 3),
-        "frac_of_hbm_peak_end_to_end": round(value / HBM_PEAK_GBPS, 4),
-        "bit_exact_vs_plain": True,
-        "launch_shapes": ctx.get_config(),
-    })
-    print(json.dumps(result), flush=True)
+    # and a sample of files against the multithreaded oracle
+    threads = cpu_threads(args)
+    o_mism, o_bytes = 0, 0
+    for f, d in sorted(sample_data.items()):
+        cps = cps_per_file[f]
+        L = max(c.max_depth for c in cps)
+        nl, s, lb = O.index_mt(d, L, threads)
+        for q, c in enumerate(cps):
+            want = O.find_value_mt(d, nl, s, lb, L, c.ir, c.num_results, threads).reshape(nl.size, c.num_results, 2)
+            if not np.array_equal(batch.spans(f, q), want):
+                o_mism += 1
+        o_bytes += d.size
+    log(f"batched vs oracle on {len(sample_data)} files ({o_bytes} bytes): {o_mism} mismatching (file, query) pairs")
     batch.free()
     for p in plans:
         p.free()
     ctx.close()
-    return 0
+    return {
+        "workload": f"{nfiles} files, lognormal sizes in [1 MiB, 512 MiB], {nbytes} bytes, alternating "
+                    "tweets/adversarial; 8 member-chain queries per file; one run = one hipGraph replay "
+                    "(persistent index launch over every file's tiles + one interpreter launch)",
+        "input_bytes": nbytes,
+        "GBps": round(value, 3),
+        "ms_per_run": round(batch_dt * 1e3, 3),
+        "frac_of_hbm_peak": round(value / HBM_PEAK_GBPS, 4),
+        "plain_GBps": round(plain_gbps, 3),
+        "plain_ms_per_pass": round(plain_dt * 1e3, 3),
+        "batched_over_plain": round(value / plain_gbps, 3),
+        "bit_exact_vs_plain": mism == 0,
+        "bit_exact_vs_oracle_sample": o_mism == 0,
+        "oracle_sample": f"{len(sample_data)} files ({o_bytes} bytes), all 8 queries each, oracle on {threads} threads",
+        "steps": args.batch_steps,
+        "launch_shapes": shapes,
+    }
 
 
-def run_serve(args) -> int:
+def run_batch(args) -> int:
+    r = batch_section(args)
+    result = common_fields(args, r["GBps"], r["ms_per_run"] / 1e3 * args.batch_steps)
+    result["steps"] = args.batch_steps
+    result.update({
+        "data": f"synthetic seeded tweets + adversarial NDJSON (seed {args.seed}), resident in HBM",
+        "config": {"workload": "batched path: " + r["workload"], "input_bytes": r["input_bytes"],
+                   "tweet_paths": TWEET_PATHS_8, "adversarial_paths": ADVERSARIAL_PATHS_8},
+    })
+    result.update({k: v for k, v in r.items() if k not in ("workload", "input_bytes", "GBps")})
+    print(json.dumps(result), flush=True)
+    return 0 if r["bit_exact_vs_plain"] and r["bit_exact_vs_oracle_sample"] else 1
+
+
+def serve_section(args) -> dict:
     """Resident files under an HBM budget; Poisson open-loop queries at 3 rates,
     coalesced into one interpreter launch per batch, against the plain
-    one-query-one-launch path (index + query + result readback per request)."""
+    one-query-one-launch path (index + query + result readback per request).
+    Latencies run from each request's scheduled arrival.  Served spans of the
+    requests on the files the plain path ran are compared with its spans."""
     import numpy as np
 
     import gpjson_amd
     from gpjson_amd import compile_path, datagen
     from gpjson_amd.serve import Server, open_loop
 
-    per_file = args.serve_total // args.files
+    nfiles = args.serve_files
+    per_file = args.serve_total // nfiles
     levels = max(compile_path(p).max_depth for p in SERVE_PATHS_12)
     rng = np.random.default_rng(args.seed)
 
     # plain one-query-one-launch on a few files first (their HBM is released before serving)
     ctx = gpjson_amd.Context(0)
-    sample_files = min(4, args.files)
+    sample_files = min(4, nfiles)
     plain_t, plain_b = 0.0, 0
+    plain_spans = {}
     for i in range(sample_files):
         d = datagen.generate("tweets", args.seed * 1000 + i, per_file)
         buf = ctx.stage(d)
         for p in SERVE_PATHS_12:
             plan = ctx.plan(buf, [compile_path(p).ir])
             plan.run()
-            plan.spans()
+            plain_spans[(f"file{i:03d}", p)] = plan.spans()[0]
             t0 = time.perf_counter()
             for _ in range(2):
                 plan.run()
@@ -388,28 +449,27 @@ def run_serve(args) -> int:
 
     srv = Server(args.serve_budget, levels)
     t = time.perf_counter()
-    for i in range(args.files):
+    for i in range(nfiles):
         d = datagen.generate("tweets", args.seed * 1000 + i, per_file)
         srv.add(f"file{i:03d}", d)
         del d
-        if i % 8 == 7:
+        if i % 16 == 15:
             log(f"registered {i + 1} files ({(i + 1) * per_file / 1e9:.1f} GB), {time.perf_counter() - t:.1f} s")
     t = time.perf_counter()
-    for i in range(args.files):
+    for i in range(nfiles):
         srv.admit(f"file{i:03d}")
-    log(f"admitted {args.files} files in {time.perf_counter() - t:.1f} s; stats {srv.stats()}")
+    log(f"admitted {nfiles} files in {time.perf_counter() - t:.1f} s; stats {srv.stats()}")
 
     def draw(n):
-        fs = rng.integers(0, args.files, size=n)
+        fs = rng.integers(0, nfiles, size=n)
         ps = rng.integers(0, len(SERVE_PATHS_12), size=n)
         return [(f"file{int(f):03d}", SERVE_PATHS_12[int(p)]) for f, p in zip(fs, ps)]
 
     # capacity: everything submitted at once (closed burst), after one untimed
     # burst of the same size (the first one also sets up the server's result
     # slots and launch state, which varied its time by up to 1.7x)
-    for _ in range(max(args.warmup, 1)):
-        for tk in [srv.submit(k, p) for k, p in draw(args.serve_burst)]:
-            srv.result(tk).release()
+    for tk in [srv.submit(k, p) for k, p in draw(args.serve_burst)]:
+        srv.result(tk).release()
     burst = draw(args.serve_burst)
     t0 = time.perf_counter()
     tickets = [srv.submit(k, p) for k, p in burst]
@@ -425,8 +485,9 @@ def run_serve(args) -> int:
     per_rate = []
     for r in rates:
         n = max(100, int(r * args.serve_seconds))
+        reqs = draw(n)
         st0 = srv.stats()
-        lat, wall, nb, _ = open_loop(srv, draw(n), r, seed=args.seed + int(r))
+        lat, wall, nb, _ = open_loop(srv, reqs, r, seed=args.seed + int(r))
         st1 = srv.stats()
         launches = st1["launches"] - st0["launches"]
         per_rate.append({
@@ -438,29 +499,54 @@ def run_serve(args) -> int:
             "mean_batch": round(n / max(launches, 1), 2),
         })
         log(f"rate {r}/s: {per_rate[-1]}")
+    # bit-exactness, outside the timed runs: every (file, path) the plain path
+    # ran, submitted amid requests on every other file and coalesced with them
+    mix = draw(args.serve_burst)
+    checks = list(plain_spans)
+    for k, (key, p) in enumerate(checks):
+        mix.insert((k * len(mix)) // len(checks), (key, p))
+    tickets = [srv.submit(key, p) for key, p in mix]
+    checked = mism = 0
+    for tk, (key, p) in zip(tickets, mix):
+        with srv.result(tk) as res:
+            if (key, p) in plain_spans:
+                checked += 1
+                mism += 0 if np.array_equal(res.spans, plain_spans[(key, p)]) else 1
     st = srv.stats()
     srv.close()
-
-    value = cap_gbps
-    result = common_fields(args, value, cap_dt)
-    result.update({
-        "data": f"synthetic seeded tweets NDJSON (seed {args.seed}), registered in host memory, resident in HBM",
-        "config": {
-            "workload": f"serving: {args.files} tweets files of {per_file} bytes ({args.files * per_file} bytes), "
-                        f"index levels {levels}, HBM budget {args.serve_budget} bytes; 12 member-chain paths over "
-                        "uniformly chosen files; value = closed-burst capacity in input GB/s",
-            "paths": SERVE_PATHS_12,
-            "hbm_budget_bytes": args.serve_budget,
-        },
+    log(f"served vs plain: {mism} mismatching of {checked} requests on the plain path's files, "
+        f"in a burst of {len(mix)}")
+    return {
+        "workload": f"{nfiles} tweets files of {per_file} bytes ({nfiles * per_file} bytes), index levels {levels}, "
+                    f"HBM budget {args.serve_budget} bytes; 12 member-chain paths over uniformly chosen files; "
+                    "Poisson open-loop arrivals, latency from each request's scheduled arrival to its spans "
+                    "being host-visible",
+        "input_bytes": nfiles * per_file,
+        "hbm_budget_bytes": args.serve_budget,
         "capacity_requests_per_s": round(capacity, 1),
+        "capacity_GBps": round(cap_gbps, 3),
         "rates": per_rate,
         "plain_GBps": round(plain_gbps, 3),
         "plain_ms_per_request": round(plain_ms, 3),
-        "served_over_plain": round(value / plain_gbps, 3),
+        "served_over_plain": round(cap_gbps / plain_gbps, 3),
+        "bit_exact_vs_plain": mism == 0,
+        "bit_exact_checked_requests": checked,
         "server_stats": st,
+    }
+
+
+def run_serve(args) -> int:
+    r = serve_section(args)
+    value = r["capacity_GBps"]
+    result = common_fields(args, value, args.serve_burst / r["capacity_requests_per_s"])
+    result.update({
+        "data": f"synthetic seeded tweets NDJSON (seed {args.seed}), registered in host memory, resident in HBM",
+        "config": {"workload": "serving: " + r["workload"] + "; value = closed-burst capacity in input GB/s",
+                   "paths": SERVE_PATHS_12, "hbm_budget_bytes": r["hbm_budget_bytes"]},
     })
+    result.update({k: v for k, v in r.items() if k not in ("workload", "hbm_budget_bytes")})
     print(json.dumps(result), flush=True)
-    return 0
+    return 0 if r["bit_exact_vs_plain"] else 1
 
 
 def run_copy(args) -> int:
@@ -570,13 +656,17 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
     ap.add_argument("--config", default="", help="index_block,index_wpt,query_mode,query_block,query_group,wpc,lpg")
-    ap.add_argument("--files", type=int, default=0, help="batch: 256 files; serve: 64 files")
+    ap.add_argument("--batch-files", type=int, default=256)
     ap.add_argument("--batch-total", type=int, default=16 << 30)
+    ap.add_argument("--batch-steps", type=int, default=3, help="batched and 256-file plain runs timed")
+    ap.add_argument("--no-batch", action="store_true", help="default mode: leave out the batched section")
+    ap.add_argument("--serve-files", type=int, default=64)
     ap.add_argument("--serve-total", type=int, default=100 << 30)
     ap.add_argument("--serve-budget", type=int, default=192 << 30)
     ap.add_argument("--serve-burst", type=int, default=512)
     ap.add_argument("--serve-rates", default="", help="requests/s, comma separated (default: 25/50/80%% of capacity)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--no-serve", action="store_true", help="default mode: leave out the served section")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -584,10 +674,8 @@ def main() -> int:
               "paths are single-GPU by design.", file=sys.stderr)
         return 2
     if args.mode == "batch":
-        args.files = args.files or 256
         return run_batch(args)
     if args.mode == "serve":
-        args.files = args.files or 64
         return run_serve(args)
     if args.mode == "copy":
         return run_copy(args)

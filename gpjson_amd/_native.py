@@ -112,6 +112,7 @@ SIGNATURES = {
     "gpjson_server_release": (c_int, [c_void_p, ctypes.c_uint64]),
     "gpjson_server_destroy": (None, [c_void_p]),
     "gpjson_time_copy": (c_int, [c_void_p, c_int64, c_int, P(c_double)]),
+    "gpjson_time_read": (c_int, [c_void_p, c_int64, c_int, P(c_double)]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),

@@ -239,6 +239,12 @@ class Context:
         _check(self._lib.gpjson_time_copy(self._h, nbytes, iterations, ctypes.byref(ms)))
         return ms.value
 
+    def time_read(self, nbytes: int, iterations: int = 5) -> float:
+        """ms per device streaming read of nbytes (nothing written)."""
+        ms = ctypes.c_double()
+        _check(self._lib.gpjson_time_read(self._h, nbytes, iterations, ctypes.byref(ms)))
+        return ms.value
+
     def time_plain(self, buf: "Buffer", irs: Sequence[bytes], iterations: int):
         """(ms per index+query pass, index ms, query ms) measured with HIP events on the library stream."""
         nq = len(irs)

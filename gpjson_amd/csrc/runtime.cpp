@@ -381,6 +381,20 @@ static void free_index_arrays(gpjson_index* idx) {
   idx->NL = idx->lines_dev = idx->tile_lines = nullptr;
 }
 
+int64_t index_device_bytes(const gpjson_index* idx) {
+  int64_t b = (int64_t)idx->W * kRecWords * 8 + leveled_alloc_words(idx->Wp, idx->L) * 8 + idx->nl_cap * 8 + 64 +
+              idx->tile_lines_cap * 8;
+  if (idx->E) b += idx->W * 8;
+  if (idx->Q) b += idx->W * 8;
+  return b;
+}
+
+int64_t index_scratch_bytes(int64_t n, const IndexTune& t) {
+  const int64_t tile_bytes = index_tile_bytes(t.block, t.wpt);
+  const int64_t ntiles = std::max<int64_t>((n + tile_bytes - 1) / tile_bytes, 1);
+  return ntiles * (int64_t)sizeof(TileState) + (ntiles * (tile_bytes / 64) + ntiles / 256 + 2) * 8;
+}
+
 // Builds (or rebuilds) an index; grows the newline array if the first pass
 // finds more lines than it holds.
 static int build_index(gpjson_index* idx) {

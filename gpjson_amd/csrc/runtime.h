@@ -183,6 +183,11 @@ int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t*
 int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels, bool verify, int* flags,
                            int64_t* lines_dev, hipStream_t stream, hipEvent_t before_kernel = nullptr,
                            hipEvent_t after_kernel = nullptr);
+// Device bytes an index holds (every array, after any newline-array growth).
+int64_t index_device_bytes(const gpjson_index* idx);
+// Context scratch an index build of n bytes needs (tile states, newline
+// scratch rows and block sums); a context keeps the largest it has built.
+int64_t index_scratch_bytes(int64_t n, const IndexTune& t);
 // Levels (deepest DOWN nesting) and result slots (STOREs) a set of IRs needs.
 void ir_shape(const uint8_t* const* irs, const int64_t* ir_lens, int nq, int* L_out, int* R_out);
 // noalloc.hip: the device address of this device's fixed scratch array (the

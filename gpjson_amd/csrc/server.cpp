@@ -15,7 +15,9 @@
 //   - the batching thread coalesces every pending request whose file is
 //     resident into ONE interpreter launch (continuous batching: requests that
 //     arrive while batches are on the GPU join the next batch; up to kSlots
-//     batches are in flight, each with its own span buffers);
+//     batches are in flight, each with its own span buffers and its own
+//     stream, so one batch's span copy to the host overlaps the next batch's
+//     interpreter launch on the device);
 //   - a finished request's spans stay in its batch's pinned buffer until the
 //     client collects them (one copy, made by the waiting thread outside the
 //     server lock); when every slot holds uncollected results and requests are
@@ -86,6 +88,7 @@ struct Batch {
   size_t host_cap = 0;
   void* host_jobs = nullptr;   // pinned staging of the job table + IR bytes
   size_t host_jobs_cap = 0;
+  hipStream_t stream = nullptr;  // this slot's launches and copies
   hipEvent_t done = nullptr;
   bool active = false;           // launched, not yet complete
   int outstanding = 0;           // completed requests whose spans are still here
@@ -109,7 +112,12 @@ struct gpjson_server {
   std::deque<std::shared_ptr<Request>> pending;
   std::deque<ServedFile*> admit_queue;
   uint64_t next_ticket = 1;
-  int64_t resident_bytes = 0;
+  int64_t resident_bytes = 0;  // inputs + indexes of resident (and admitting) files
+  // the admission context's index scratch (grow-only, sized by the largest
+  // file it built), counted against the budget too.  The batch slots' span
+  // and job buffers are not: they hold results in flight, not resident data,
+  // and counting them could leave a file that fits the budget unadmittable.
+  int64_t admit_scratch = 0;
   bool stop = false;
   std::thread batcher, admitter;
   Batch slots[kSlots];
@@ -128,9 +136,10 @@ int64_t index_bytes(int64_t n, int levels) {
 }
 
 // Frees LRU resident files (not used by in-flight batches, not being admitted)
-// until `need` more bytes fit.  Caller holds the lock.
+// until `need` more bytes fit beside the resident files and the admission
+// scratch.  Caller holds the lock.
 bool make_room(gpjson_server* s, int64_t need, std::vector<ServedFile*>* to_free) {
-  while (s->resident_bytes + need > s->budget) {
+  while (s->resident_bytes + s->admit_scratch + need > s->budget) {
     ServedFile* victim = nullptr;
     for (auto& kv : s->files) {
       ServedFile* f = kv.second.get();
@@ -164,11 +173,14 @@ void admitter_main(gpjson_server* s) {
       s->admit_cv.wait(lk, [&] { return s->stop || !s->admit_queue.empty(); });
       if (s->stop) return;
       f = s->admit_queue.front();
-      const int64_t need = f->n + index_bytes(f->n, s->levels);
-      ok = make_room(s, need, &to_free);
+      // the admission context keeps scratch for the largest file it built
+      const int64_t grow = std::max<int64_t>(0, index_scratch_bytes(f->n, s->actx->itune) - s->admit_scratch);
+      const int64_t need = f->n + 64 + index_bytes(f->n, s->levels);
+      ok = make_room(s, need + grow, &to_free);
       if (ok) {
         s->admit_queue.pop_front();
         s->resident_bytes += need;
+        s->admit_scratch += grow;
         f->device_bytes = need;
       }
     }
@@ -181,16 +193,32 @@ void admitter_main(gpjson_server* s) {
     }
     int rc = gpjson_buf_from_host(s->actx, f->host.data(), f->n, &f->buf);
     if (rc == GPJSON_OK) rc = gpjson_index_build(s->actx, f->buf, s->levels, 0, &f->idx);
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     f->admitting = false;
     if (rc != GPJSON_OK) {
       s->fatal = std::string("admission of ") + f->key + " failed: " + gpjson_last_error();
       s->resident_bytes -= f->device_bytes;
       release_device(f);
     } else {
+      // charge what the index really holds (its newline array grows for files
+      // of lines shorter than the first estimate)
+      const int64_t actual = f->n + 64 + index_device_bytes(f->idx);
+      s->resident_bytes += actual - f->device_bytes;
+      f->device_bytes = actual;
       f->resident = true;
       f->last_use = Clock::now();
       s->admissions++;
+      // back under the budget if the estimate was short (the new file is the
+      // most recently used, so others go first)
+      std::vector<ServedFile*> over;
+      f->inflight++;  // never the new file itself
+      make_room(s, 0, &over);
+      f->inflight--;
+      if (!over.empty()) {
+        lk.unlock();
+        for (ServedFile* v : over) release_device(v);
+        lk.lock();
+      }
     }
     s->cv.notify_all();
   }
@@ -246,7 +274,8 @@ void spill_oldest(gpjson_server* s) {
 // Launches one interpreter over every request in `reqs` (files resident).  Lock
 // NOT held (only the batching thread touches slots).
 int launch_batch(gpjson_server* s, Batch& b) {
-  gpjson_ctx* ctx = s->qctx;
+  gpjson_ctx* ctx = s->qctx;  // launch shapes and CU count
+  hipStream_t stream = b.stream;
   const int n = (int)b.reqs.size();
   std::vector<QueryJob> jobs(n);
   int64_t items = 0, elems = 0, ir_total = 0;
@@ -312,7 +341,7 @@ int launch_batch(gpjson_server* s, Batch& b) {
     }
   }
   // the pinned staging stays untouched until this slot's batch completes
-  GJ_HIP(hipMemcpyAsync(b.jobs.p, host, table_bytes, hipMemcpyHostToDevice, ctx->stream));
+  GJ_HIP(hipMemcpyAsync(b.jobs.p, host, table_bytes, hipMemcpyHostToDevice, stream));
   QueryLaunch L{};
   L.jobs = (const QueryJob*)((uint8_t*)b.jobs.p + jobs_off);
   L.njobs = n;
@@ -321,15 +350,15 @@ int launch_batch(gpjson_server* s, Batch& b) {
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   L.ir_base = (const uint8_t*)b.jobs.p + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
-  GJ_HIP(zero_u64(L.counter, 1, ctx->stream));
+  GJ_HIP(zero_u64(L.counter, 1, stream));
   const int waves_per_block = ctx->qtune.block / 64;
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
-  if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, ctx->stream));
-  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, ctx->stream));
-  GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, ctx->stream));
-  GJ_HIP(hipEventRecord(b.done, ctx->stream));
+  if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, stream));
+  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, stream));
+  GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, stream));
+  GJ_HIP(hipEventRecord(b.done, stream));
   return GPJSON_OK;
 }
 
@@ -433,7 +462,11 @@ int gpjson_server_create(int device, int64_t hbm_budget_bytes, int levels, gpjso
   }
   s->budget = hbm_budget_bytes;
   s->levels = levels;
-  for (Batch& b : s->slots) GJ_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+  GJ_HIP(hipSetDevice(device));
+  for (Batch& b : s->slots) {
+    GJ_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    GJ_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));  // default priority
+  }
   gpjson_server* raw = s.get();
   s->batcher = std::thread(batcher_main, raw);
   s->admitter = std::thread(admitter_main, raw);
@@ -449,7 +482,7 @@ int gpjson_server_add(gpjson_server* s, const char* key, const void* data, int64
   f->host.assign((const uint8_t*)data, (const uint8_t*)data + n);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->files.count(key)) return fail(GPJSON_ERR_INVALID_ARG, std::string("file already registered: ") + key);
-  if (f->n + index_bytes(f->n, s->levels) > s->budget)
+  if (f->n + 64 + index_bytes(f->n, s->levels) + index_scratch_bytes(f->n, s->actx->itune) > s->budget)
     return fail(GPJSON_ERR_NO_MEMORY, std::string("file larger than the HBM budget: ") + key);
   s->files[key] = std::move(f);
   return GPJSON_OK;
@@ -618,12 +651,15 @@ void gpjson_server_destroy(gpjson_server* s) {
   (void)hipSetDevice(s->qctx->device);
   (void)hipStreamSynchronize(s->qctx->stream);
   (void)hipStreamSynchronize(s->actx->stream);
+  for (Batch& b : s->slots)
+    if (b.stream) (void)hipStreamSynchronize(b.stream);
   for (Batch& b : s->slots) {
     b.spans.release();
     b.jobs.release();
     if (b.host_spans) (void)hipHostFree(b.host_spans);
     if (b.host_jobs) (void)hipHostFree(b.host_jobs);
     if (b.done) (void)hipEventDestroy(b.done);
+    if (b.stream) (void)hipStreamDestroy(b.stream);
   }
   for (auto& kv : s->files) release_device(kv.second.get());
   gpjson_ctx_destroy(s->actx);

@@ -115,15 +115,22 @@ class ServedResult:
 
 
 def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: float, seed: int = 0,
-              keep_spans: bool = False, wait_timeout_ms: int = 120000):
+              keep_spans=False, wait_timeout_ms: int = 120000):
     """Submits `requests` with exponential inter-arrival times (Poisson arrivals
     at rate_per_s, seeded) from one thread while another collects results.
-    Returns (latencies_us, wall_s, bytes_queried, spans or None)."""
+    Returns (latencies_us, wall_s, bytes_queried, spans or None); keep_spans
+    is True (copy every request's spans) or a predicate on the request index.  A latency
+    runs from the request's scheduled arrival, not from its submit call: when
+    the submitting thread falls behind the schedule, that delay counts too
+    (no coordinated omission).  The server measures submit to completion; the
+    lag of the submit call behind the schedule is added to it."""
+    keep = keep_spans if callable(keep_spans) else (lambda i: bool(keep_spans))
     rng = random.Random(seed)
     gaps = [rng.expovariate(rate_per_s) for _ in requests]
     tickets: List[int] = [0] * len(requests)
     ready = [threading.Event() for _ in requests]
     lat = [0.0] * len(requests)
+    lag_us = [0.0] * len(requests)
     spans = [None] * len(requests)
     errors: List[BaseException] = []
 
@@ -133,8 +140,8 @@ def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: f
             for i in range(len(requests)):
                 ready[i].wait()
                 with server.result(tickets[i], wait_timeout_ms) as res:
-                    lat[i] = res.latency_us
-                    if keep_spans:
+                    lat[i] = lag_us[i] + res.latency_us
+                    if keep(i):
                         spans[i] = res.spans.copy()
         except BaseException as e:  # surfaced to the caller
             errors.append(e)
@@ -148,6 +155,7 @@ def open_loop(server: Server, requests: Sequence[Tuple[str, str]], rate_per_s: f
         d = due - time.perf_counter()
         if d > 0:
             time.sleep(d)
+        lag_us[i] = max(0.0, time.perf_counter() - due) * 1e6
         tickets[i] = server.submit(key, path)
         ready[i].set()
     th.join()

@@ -343,6 +343,10 @@ int gpjson_time_plain(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *ir
  * reference for calibrating FETCH_SIZE / WRITE_SIZE counters.  No counterpart
  * in the reference. */
 int gpjson_time_copy(gpjson_ctx *ctx, int64_t n, int iterations, double *ms_per_copy);
+/* Device streaming read of n bytes (16-byte vector loads, nothing written),
+ * timed with HIP events over `iterations` launches: the achievable HBM read
+ * bandwidth is n / ms_per_read.  No counterpart in the reference. */
+int gpjson_time_read(gpjson_ctx *ctx, int64_t n, int iterations, double *ms_per_read);
 
 #ifdef __cplusplus
 }
