@@ -146,7 +146,9 @@ def run_plain(args) -> int:
         for line in report.strip().splitlines():
             log("autotune: " + line)
         log(f"autotune took {time.perf_counter() - t:.1f} s")
-    log(f"launch shapes: {ctx.get_config()}")
+    if args.fusion:
+        ctx.set_fusion(True)
+    log(f"launch shapes: {ctx.get_config()}; in-tile evaluation {'on' if args.fusion else 'off'}")
     plans = [ctx.plan(buf, [cp.ir]) for cp in cps]
     lines = plans[0].lines
     log(f"plans ready: {lines} lines, levels {[cp.max_depth for cp in cps]}")
@@ -167,6 +169,8 @@ def run_plain(args) -> int:
     value = args.steps * step_bytes / dt / 1e9
 
     timings = [p.timing() for p in plans]
+    fusion = [p.fused() for p in plans]
+    log(f"in-tile evaluation per path (fused, lines left to the interpreter): {fusion}")
     idx_ms = [a for a, _ in timings]
     q_ms = [b for _, b in timings]
     mean_idx_ms = sum(idx_ms) / len(idx_ms)
@@ -212,6 +216,7 @@ def run_plain(args) -> int:
         "query_ms": [round(x, 4) for x in q_ms],
         "staging_GBps": round(data.size / stage_s / 1e9, 2),
         "launch_shapes": ctx.get_config(),
+        "in_tile_evaluation": [{"fused": f, "interpreted_lines": n} for f, n in fusion],
         "bit_exact_vs_one_shot": one_shot_ok,
     })
     for p in plans:
@@ -655,6 +660,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search launch shapes on this input first")
+    ap.add_argument("--fusion", action="store_true",
+                    help="plain: evaluate the lines inside each index tile in the index kernel")
     ap.add_argument("--config", default="", help="index_block,index_wpt,query_mode,query_block,query_group,wpc,lpg")
     ap.add_argument("--batch-files", type=int, default=256)
     ap.add_argument("--batch-total", type=int, default=16 << 30)

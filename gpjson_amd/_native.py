@@ -87,6 +87,7 @@ SIGNATURES = {
     "gpjson_plan_create": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, P(c_void_p)]),
     "gpjson_plan_run": (c_int, [c_void_p, c_int]),
     "gpjson_plan_timing": (c_int, [c_void_p, P(c_double), P(c_double)]),
+    "gpjson_plan_fused": (c_int, [c_void_p, P(c_int), P(c_int64)]),
     "gpjson_plan_lines": (c_int64, [c_void_p]),
     "gpjson_plan_results_per_line": (c_int, [c_void_p]),
     "gpjson_plan_spans": (c_int, [c_void_p, c_void_p, c_int64]),
@@ -115,6 +116,8 @@ SIGNATURES = {
     "gpjson_time_read": (c_int, [c_void_p, c_int64, c_int, P(c_double)]),
     "gpjson_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gpjson_set_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
+    "gpjson_set_fusion": (c_int, [c_void_p, c_int]),
+    "gpjson_get_fusion": (c_int, [c_void_p, P(c_int)]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),
     "gpjson_query_noalloc": (c_int, [c_void_p, c_void_p, c_int64, P(c_void_p), P(c_int64), c_int, c_void_p, c_int64,
                                      c_void_p, c_int64, c_void_p]),

@@ -223,6 +223,15 @@ class Context:
         _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_mode, query_block, query_group,
                                            query_waves_per_cu, lines_per_group))
 
+    def set_fusion(self, enable: bool) -> None:
+        """In-tile evaluation of single member-chain plans (off by default)."""
+        _check(self._lib.gpjson_set_fusion(self._h, 1 if enable else 0))
+
+    def get_fusion(self) -> bool:
+        v = ctypes.c_int()
+        _check(self._lib.gpjson_get_fusion(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
     def autotune(self, buf: "Buffer", irs: Sequence[bytes], iterations: int = 3) -> str:
         """Times every launch shape on buf and installs the fastest; returns the report."""
         nq = len(irs)
@@ -273,6 +282,12 @@ class Plan:
 
     def run(self, timed: bool = False) -> None:
         _check(N.load().gpjson_plan_run(self._h, 1 if timed else 0))
+
+    def fused(self):
+        """(evaluates lines inside index tiles, lines the last run left to the interpreter or -1)."""
+        f, n = ctypes.c_int(), ctypes.c_int64()
+        _check(N.load().gpjson_plan_fused(self._h, ctypes.byref(f), ctypes.byref(n)))
+        return bool(f.value), n.value
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()

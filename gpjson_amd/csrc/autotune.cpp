@@ -69,6 +69,18 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
   return GPJSON_OK;
 }
 
+int gpjson_set_fusion(gpjson_ctx* ctx, int enable) {
+  if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
+  ctx->qtune.fuse = enable ? 1 : 0;
+  return GPJSON_OK;
+}
+
+int gpjson_get_fusion(gpjson_ctx* ctx, int* enabled) {
+  if (!ctx || !enabled) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  *enabled = ctx->qtune.fuse;
+  return GPJSON_OK;
+}
+
 // Times every candidate shape on buf with the given queries (index kernel and
 // interpreter separately, `iterations` timed runs each after one warm-up) and
 // installs the fastest.  `report` (optional) receives one line per candidate.

@@ -64,6 +64,11 @@ struct FileJob {
   int pad_;
 };
 
+// In-tile evaluation limits: chains up to 3 keys (the tile's LDS holds three
+// level words per input word), keys up to 120 bytes together.
+constexpr int kFuseMaxDepth = 3;
+constexpr int kFuseKeyBytes = 120;
+
 struct IndexLaunch {
   const FileJob* jobs;     // device array, or null: the single job `one`
   FileJob one;
@@ -81,6 +86,23 @@ struct IndexLaunch {
   int64_t* nl_scratch;
   int64_t nl_scratch_cap;  // slots per tile (the tile's words)
   const int* run_if;       // when set, the launch does nothing unless *run_if != 0
+  // In-tile evaluation of one member chain (deferred bookkeeping, one file;
+  // fuse_depth = 0: off).  A tile holding at most fused_line_cap(tile words)
+  // newlines evaluates the chain (keys k_0 .. k_{D-1}, concatenated in
+  // fuse_keys, lengths fuse_klen) for every line that starts and ends inside
+  // it and leaves the spans in its newline scratch row; it writes its records
+  // and leveled words only where a line crosses into a neighbouring tile (the
+  // words up to its first newline and from its last one).  The bookkeeping
+  // then places those spans in fuse_spans (int64[lines][2]) and lists every
+  // other line in fuse_list (capacity fuse_list_cap, count *fuse_count) for
+  // the interpreter.
+  int fuse_depth;
+  int fuse_klen[kFuseMaxDepth];
+  uint8_t fuse_keys[kFuseKeyBytes];
+  int64_t* fuse_spans;
+  int64_t* fuse_list;
+  int64_t fuse_list_cap;
+  unsigned long long* fuse_count;  // zeroed with the ticket
 };
 
 }  // namespace gj

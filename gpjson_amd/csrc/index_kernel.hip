@@ -32,6 +32,7 @@
 // aggregate for both carry values.
 #include <hip/hip_runtime.h>
 #include "bitops.h"
+#include "fused_eval.h"
 #include "index_common.h"
 
 namespace gj {
@@ -451,11 +452,46 @@ __device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P
 // first byte.
 // act false: a lane past the last word (it writes nothing, but takes part in
 // the wavefront's work when the brackets are spread).
+// Fused tiles (in-tile chain evaluation, fz.on): the word's index words go to
+// memory only when wr (the word lies at or before the tile's first newline or
+// at or after its last), and the final string word and levels 0..2 are left in
+// the word's LDS slots (S, O, C, P) with the levels' summaries in fz.sl.
+struct FuseEmit {
+  bool on;
+  bool wr;
+  uint64_t* sl;  // [kFuseMaxDepth][TILE_WORDS / 64]
+  int nsum;
+};
+
+// LDS slot of leveled word l < 3 of a fused tile (the bracket masks are dead by then).
+__device__ __forceinline__ constexpr int fuse_lb_slot(int l) { return l == 0 ? kSlotO : (l == 1 ? kSlotC : kSlotP); }
+
+// A fused tile's words in LDS, as fused_eval.h reads them: word w of the tile
+// is lane w % BLOCK of row w / BLOCK; after emission its slots hold the final
+// string word (S0), newlines, the leveled words 0..2 (O, C, P), raw quotes and,
+// after the key tests, the match word (K).
+template <int ROWS, int NSLOT, int BLOCK>
+struct FuseTile {
+  const uint64_t (&wc)[ROWS][NSLOT][BLOCK];
+  const uint64_t* sm;  // [kFuseMaxDepth][nsum]: level l's match words that are not zero
+  const uint64_t* sl;  // [kFuseMaxDepth][nsum]: level l's leveled words that are not zero
+  int nsum;
+  __device__ __forceinline__ uint64_t at(int slot, int w) const { return wc[w / BLOCK][slot][w % BLOCK]; }
+  __device__ __forceinline__ uint64_t S(int w) const { return at(kSlotS0, w); }
+  __device__ __forceinline__ uint64_t NL(int w) const { return at(kSlotNL, w); }
+  __device__ __forceinline__ uint64_t Q(int w) const { return at(kSlotQ, w); }
+  __device__ __forceinline__ uint64_t M(int w) const { return at(kSlotK, w); }
+  __device__ __forceinline__ uint64_t LB(int l, int w) const { return at(fuse_lb_slot(l), w); }
+  __device__ __forceinline__ uint64_t SM(int l, int s) const { return sm[l * nsum + s]; }
+  __device__ __forceinline__ uint64_t SL(int l, int s) const { return sl[l * nsum + s]; }
+};
+
 template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK, int NLVS>
-__device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uint64_t (&wc)[ROWS][NSLOT][BLOCK],
+__device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t (&wc)[ROWS][NSLOT][BLOCK],
                                           const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
                                           int64_t d, bool act, uint64_t (&lv)[NLVS][64],
-                                          uint16_t (&items)[kBracketCap]) {
+                                          uint16_t (&items)[kBracketCap], const FuseEmit& fz) {
+  const bool wr = act && (!fz.on || fz.wr);
   uint64_t valid = ~0ull;
   const int64_t rem = J.N - w * 64;
   if (rem < 64) valid = rem > 0 ? (1ull << rem) - 1ull : 0;
@@ -468,7 +504,7 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
   S = (s ? ~S : S) & valid;
   // the word's record: string, colon and raw-quote words (bytes past the end
   // of the file were loaded as zeros, which are neither ':' nor '"')
-  if (act) {
+  if (wr) {
     uint64_t* rec = J.rec + w * kRecWords;
     rec[0] = S;
     rec[1] = wc[r][kSlotK][tid];
@@ -480,32 +516,39 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, const uin
     if (J.Q) J.Q[w] = wc[r][kSlotQ][tid] & ~E & valid;
   }
   const uint64_t O = wc[r][kSlotO][tid] & ~S, C = wc[r][kSlotC][tid] & ~S, P = wc[r][kSlotP][tid] & ~S;
+  if (fz.on) wc[r][kSlotS0][tid] = S;  // only this lane reads its slots until the next barrier
   for (int lo = 0; lo < J.L; lo += NLVS) {
     uint64_t out[NLVS];
     levels_spread<NLVS>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
 #pragma unroll
     for (int i = 0; i < NLVS; i++)
-      if (act && lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+      if (wr && lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
     // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
     // start at a multiple of 64 (lanes past the file end are inactive, and
     // lane 0 holds the lowest word, so it is active whenever any lane is)
-    if (J.LS) {
+    if (J.LS || fz.on) {
 #pragma unroll
       for (int i = 0; i < NLVS; i++) {
         if (lo + i < J.L) {
           const unsigned long long m = __ballot(out[i] != 0);
-          if ((tid & 63) == 0 && act) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
+          if ((tid & 63) == 0 && act && J.LS) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
+          if (fz.on && lo + i < kFuseMaxDepth) {
+            if ((tid & 63) == 0) fz.sl[(lo + i) * fz.nsum + ((r * BLOCK + tid) >> 6)] = m;
+            wc[r][fuse_lb_slot(lo + i)][tid] = out[i];
+          }
         }
       }
     }
   }
 }
 
-template <int BLOCK, int ROWS, int NLV, bool VERIFY>
+template <int BLOCK, int ROWS, int NLV, bool VERIFY, bool FUSE>
 // The default tile (256 x 3, no verify words) keeps 3 blocks per CU by LDS:
 // registers are held to the 3 wavefronts per SIMD that allows.
 __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 : 1) void index_kernel(
     IndexLaunch launch) {
+  static_assert(!(FUSE && VERIFY), "in-tile evaluation runs without the verify words");
+  static_assert(!FUSE || NLV <= kFuseMaxDepth, "in-tile evaluation keeps at most three levels in LDS");
   constexpr int NW = BLOCK / 64;
   constexpr int TILE_WORDS = BLOCK * ROWS;
   static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
@@ -536,6 +579,15 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   __shared__ uint64_t s_lv[NW][NLVS][64];
   __shared__ uint16_t s_items[NW][kBracketCap];
   __shared__ uint8_t s_lead[ROWS][BLOCK];
+  // in-tile evaluation: key bytes and summaries; after emission the tile's
+  // newline positions reuse the bracket lists and the key candidates the
+  // level scratch
+  constexpr int NSUM = TILE_WORDS / 64;
+  __shared__ uint8_t s_keys[FUSE ? kFuseKeyBytes : 1];
+  __shared__ uint64_t s_fsum[FUSE ? 2 * kFuseMaxDepth * NSUM : 1];  // [match | level][l][NSUM]
+  __shared__ int s_lnl;    // sequence index of the tile's last word holding a '\n'
+  __shared__ int s_ncand;  // key candidates whose bytes are still to be compared
+  static_assert(!FUSE || TILE_WORDS / 3 <= NW * kBracketCap, "the newline positions fit the bracket lists");
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -543,6 +595,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   const unsigned long long lanes_below = (1ull << lane) - 1ull;
   if (launch.run_if && *launch.run_if == 0) return;
   const bool dl = launch.nl_scratch != nullptr;  // deferred newline bookkeeping
+  if (FUSE)
+    for (int k = tid; k < kFuseKeyBytes; k += BLOCK) s_keys[k] = launch.fuse_keys[k];
 
   for (;;) {
     if (tid == 0) {
@@ -550,6 +604,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       s_w0 = -1;
       s_w0_flip = 0;
       s_fnl = TILE_WORDS;
+      s_lnl = -1;
+      s_ncand = 0;
       unsigned t = s_ticket;
       int lo = 0, hi = (int)launch.njobs - 1;
       if (t < launch.total_tiles && launch.jobs) {
@@ -626,6 +682,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
         if (!VERIFY) {
           const unsigned long long nlw = __ballot(w.NL != 0);
           if (nlw && lane == __builtin_ctzll(nlw)) atomicMin(&s_fnl, r * BLOCK + tid);
+          if (FUSE && nlw && lane == 63 - __builtin_clzll(nlw)) atomicMax(&s_lnl, r * BLOCK + tid);
         }
       }
     }
@@ -781,6 +838,11 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     }
 
     // ---- emit the indexes (lane t writes word t of each row: coalesced) ----
+    // A tile with few enough lines evaluates the chain query of the launch on
+    // every line inside it (FUSE); it then writes its index words only around
+    // the lines that cross into its neighbours.
+    const bool ft = FUSE && dl && launch.fuse_depth > 0 && mtile.nl <= fused_line_cap(TILE_WORDS);
+    const int fnl = s_fnl, lnl = s_lnl;
     auto emit_tile = [&](const Carry& cx, bool fx) {
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
@@ -791,7 +853,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
         const int p = ent[r] ? (ent[r] - 2) : cx.p;
         const int s = cx.s ^ mx.q;
         const int64_t d = cx.d + (cx.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
-        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, s_lv[wave], s_items[wave]);
+        const FuseEmit fz{ft, sidx <= fnl || sidx >= lnl, s_fsum + kFuseMaxDepth * NSUM, NSUM};
+        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, s_lv[wave], s_items[wave], fz);
       }
     };
     emit_tile(X, fix);
@@ -810,6 +873,155 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       XL = s_exact;
       // a tile whose first byte is neither '\\' nor '"' needs no escape fix
       if (s_mis) emit_tile(XL, false);
+    }
+    if (FUSE && ft) {
+      // ---- in-tile chain evaluation ----
+      // match summaries start empty; a matching ':' sets its word's bit
+      for (int i = tid; i < kFuseMaxDepth * NSUM; i += BLOCK) s_fsum[i] = 0;
+      __syncthreads();  // every word's string and leveled words are in LDS
+      const int D = launch.fuse_depth;
+      const bool tile0 = lt == 0;
+      const int64_t pos0 = word0 * 64;
+      const FuseTile<ROWS, NSLOT, BLOCK> view{s_wc, s_fsum, s_fsum + kFuseMaxDepth * NSUM, NSUM};
+      uint16_t* nlpos = &s_items[0][0];
+      uint64_t* cands = &s_lv[0][0][0];  // tile-local ':' position | ss << 16 | level << 32
+      constexpr int CCAP = NW * NLVS * 64;
+      auto key_of = [&](int l, int& klen) {
+        int koff = 0;
+#pragma unroll
+        for (int k = 0; k < kFuseMaxDepth; k++)
+          if (k < l) koff += launch.fuse_klen[k];
+        klen = launch.fuse_klen[l];
+        return (const uint8_t*)(s_keys + koff);
+      };
+      // (1) every ':' at a level < D whose key could match (the string,
+      // newline and raw-quote words: this word's and the one before it, read
+      // once; a key test that needs words further back walks the LDS); its
+      // bytes are compared in (2).  The candidates of a wavefront are taken
+      // one per lane at a time and appended with one atomic per step.
+      const int kl0 = launch.fuse_klen[0], kl1 = launch.fuse_klen[1], kl2 = launch.fuse_klen[2];
+      auto note_match = [&](int l, int pc) {
+        const int wq = pc >> 6;
+        atomicOr(reinterpret_cast<unsigned long long*>(&s_wc[wq / BLOCK][kSlotK][wq % BLOCK]), 1ull << (pc & 63));
+        atomicOr(reinterpret_cast<unsigned long long*>(&s_fsum[l * NSUM + (wq >> 6)]), 1ull << (wq & 63));
+      };
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        const int sidx = r * BLOCK + tid;
+        uint64_t mw = 0;
+        uint64_t c0 = 0, c1 = 0, c2 = 0;
+        // words before the tile's first newline belong to a line that began in
+        // an earlier tile, words after its last to one that runs on: neither
+        // is evaluated here (except the file's first and last lines)
+        const bool inside = (sidx >= fnl || tile0) && (sidx <= lnl || lt == J.ntiles - 1);
+        if (word0 + sidx < J.W && inside) {
+          const uint64_t kw = s_wc[r][kSlotK][tid];
+          c0 = s_wc[r][fuse_lb_slot(0)][tid] & kw;
+          if (D > 1) c1 = s_wc[r][fuse_lb_slot(1)][tid] & kw;
+          if (D > 2) c2 = s_wc[r][fuse_lb_slot(2)][tid] & kw;
+        }
+        uint64_t S1 = 0, NL1 = 0, Q1 = 0, S0 = 0, NL0 = 0, Q0 = 0;
+        if (__any((c0 | c1 | c2) != 0)) {
+          S1 = s_wc[r][kSlotS0][tid];
+          NL1 = s_wc[r][kSlotNL][tid];
+          Q1 = s_wc[r][kSlotQ][tid];
+          if (sidx >= 1) {
+            S0 = view.S(sidx - 1);
+            NL0 = view.NL(sidx - 1);
+            Q0 = view.Q(sidx - 1);
+          }
+        }
+        // the usual layout (a key's closing quote right before its ':') for
+        // every ':' of the word at once; the other ':' (and the tile's first
+        // word) take the walk of fused_key_locate
+        uint64_t fast = 0, slow = c0 | c1 | c2;
+        if (sidx >= 1) {
+          const uint64_t cs[3] = {c0, c1, c2};
+          const int kls[3] = {kl0, kl1, kl2};
+          fused_key_split(cs, D, kls, S1, S0, Q1, Q0, &fast, &slow);
+        }
+        uint64_t call = fast | slow;
+        for (;;) {
+          const bool has = call != 0;
+          if (!__any(has)) break;
+          int ss = -1, pc = 0, l = 0;
+          if (has) {
+            const int b = ctz64(call);
+            const uint64_t bit = 1ull << b;
+            l = (c0 & bit) ? 0 : ((c1 & bit) ? 1 : 2);
+            const int klen = l == 0 ? kl0 : (l == 1 ? kl1 : kl2);
+            if (fast & bit) {
+              ss = sidx * 64 + b - 2 - klen;
+            } else {
+              ss = sidx >= 1 ? fused_key_locate_near(S1, NL1, Q1, S0, NL0, Q0, sidx, b, tile0, klen) : -2;
+              if (ss == -2) ss = fused_key_locate(view, sidx * 64 + b, tile0, klen);
+            }
+            pc = sidx * 64 + b;
+            call &= call - 1;
+          }
+          const unsigned long long m = __ballot(ss >= 0);
+          if (m) {
+            const int first = __builtin_ctzll(m);
+            int base = 0;
+            if (lane == first) base = atomicAdd(&s_ncand, __popcll(m));
+            base = __shfl(base, first);
+            if (ss >= 0) {
+              const int slot = base + __popcll(m & lanes_below);
+              if (slot < CCAP) {
+                cands[slot] = (uint64_t)pc | ((uint64_t)ss << 16) | ((uint64_t)l << 32);
+              } else {  // a tile with more candidates than the list holds compares here
+                int kl;
+                const uint8_t* key = key_of(l, kl);
+                if (fused_key_bytes(J.F, pos0 + ss, key, kl)) {
+                  mw |= 1ull << (pc & 63);
+                  atomicOr(reinterpret_cast<unsigned long long*>(&s_fsum[l * NSUM + (sidx >> 6)]), 1ull << (sidx & 63));
+                }
+              }
+            }
+          }
+        }
+        s_wc[r][kSlotK][tid] = mw;  // the match word replaces the colon word (its lane's writes end here)
+        // the tile's newline positions, in order
+        uint64_t nlm = s_wc[r][kSlotNL][tid];
+        int k = texcl[r].nl;
+        while (nlm) {
+          nlpos[k++] = (uint16_t)(sidx * 64 + ctz64(nlm));
+          nlm &= nlm - 1;
+        }
+      }
+      __syncthreads();
+      // (2) the candidates' key bytes, one candidate per lane: one round trip
+      // to memory for the whole tile
+      const int ncand = min(s_ncand, CCAP);
+      for (int i = tid; i < ncand; i += BLOCK) {
+        const uint64_t e = cands[i];
+        const int pc = (int)(e & 0xFFFF), ss = (int)((e >> 16) & 0xFFFF), l = (int)(e >> 32);
+        int kl;
+        const uint8_t* key = key_of(l, kl);
+        if (fused_key_bytes(J.F, pos0 + ss, key, kl)) note_match(l, pc);
+      }
+      __syncthreads();
+      // (3) one lane per line, the lines spread over the wavefronts: slot 0 is
+      // the file's first line (tile 0), slot k + 1 the line starting at the
+      // tile's newline k; a line ending past the tile (or a tile without
+      // newlines) is left to the interpreter.  The result's leading blanks are
+      // skipped by the bookkeeping.
+      const int c = mtile.nl;
+      const bool last = lt == J.ntiles - 1;
+      const int nlocal = last ? (int)(J.N - pos0) : TILE_WORDS * 64;
+      int64_t* sp = launch.nl_scratch + (J.tile0 + lt) * launch.nl_scratch_cap + fused_span_base(TILE_WORDS);
+      for (int base = 0; base <= c; base += BLOCK) {
+        const int sl = base + lane * NW + wave;
+        if (sl > c) continue;
+        const bool valid = sl == 0 ? (tile0 && (c >= 1 || last)) : (sl < c || last);
+        if (!valid) continue;
+        const int a = sl == 0 ? 0 : nlpos[sl - 1];
+        const int b = sl < c ? nlpos[sl] : nlocal;
+        int64_t o[2];
+        fused_chain_line(view, D, a, b, pos0, o);
+        sp[2 * sl] = o[0];
+        sp[2 * sl + 1] = o[1];
+      }
     }
     // newline offsets: at their slots in the newline index, or (deferred) at
     // tile-local slots of the tile's scratch row
@@ -835,13 +1047,17 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
 
 // Zeroes the lookback states and the ticket (a kernel, not hipMemsetAsync, so
 // the index pipeline can be captured into a graph).
-__global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket, const int* run_if) {
+__global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket, const int* run_if,
+                                   unsigned long long* also) {
   if (run_if && *run_if == 0) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long* p = reinterpret_cast<unsigned long long*>(states);
   for (int64_t k = i; k < n * 8; k += stride) p[k] = 0;
-  if (i == 0) *ticket = 0;
+  if (i == 0) {
+    *ticket = 0;
+    if (also) *also = 0;
+  }
 }
 
 // The carry-in a tile stored for the deferred bookkeeping.
@@ -912,6 +1128,10 @@ __global__ __launch_bounds__(kTileBlock) void index_tile_base_kernel(IndexLaunch
       if (J.nl_cap > 0) J.NL[0] = 0;
       if (J.tile_lines) J.tile_lines[0] = 1;
       *lines_out = 1;
+      if (L.fuse_depth > 0 && L.fuse_list_cap > 0) {  // its one line goes to the interpreter
+        L.fuse_list[0] = 0;
+        *L.fuse_count = 1;
+      }
     }
     return;
   }
@@ -930,6 +1150,37 @@ __global__ __launch_bounds__(kTileBlock) void index_tile_base_kernel(IndexLaunch
   }
 }
 
+// In-tile evaluation: the spans of the lines tile t evaluated, from its
+// scratch row to their lines, and the tile's other lines (those starting in it
+// that end past it, or all of them when it holds more lines than it
+// evaluates) onto the interpreter's list.
+__device__ void fuse_place_lines(const IndexLaunch& L, const FileJob& J, int64_t t, int cnt, int64_t before) {
+  const int tw = (int)L.nl_scratch_cap;
+  const bool tile0 = t == 0, last = t == J.ntiles - 1;
+  auto push = [&](int64_t first, int64_t n) {  // lines first .. first + n - 1
+    const int64_t k = (int64_t)atomicAdd(L.fuse_count, (unsigned long long)n);
+    for (int64_t i = 0; i < n && k + i < L.fuse_list_cap; i++) L.fuse_list[k + i] = first + i;
+  };
+  if (cnt > fused_line_cap(tw)) {
+    if (tile0) push(0, 1);
+    if (cnt > 0) push(1 + before, cnt);
+    return;
+  }
+  const int64_t* sp = L.nl_scratch + (J.tile0 + t) * L.nl_scratch_cap + fused_span_base(tw);
+  for (int sl = 0; sl <= cnt; sl++) {
+    const int64_t line = sl == 0 ? 0 : 1 + before + (sl - 1);
+    const bool valid = sl == 0 ? (tile0 && (cnt >= 1 || last)) : (sl < cnt || last);
+    if (valid) {
+      if (line < J.nl_cap) {
+        L.fuse_spans[2 * line] = fused_skip_blanks(J.F, J.N, sp[2 * sl]);
+        L.fuse_spans[2 * line + 1] = sp[2 * sl + 1];
+      }
+    } else if (sl > 0 || tile0) {
+      push(line, 1);
+    }
+  }
+}
+
 // Step 3: tile_lines, and each tile's newline offsets from its scratch row to
 // their slots (tile t's first newline takes slot 1 + the newlines before it).
 // A tile with more newlines than its row holds (lines shorter than 64 bytes on
@@ -945,6 +1196,7 @@ __global__ __launch_bounds__(kTileBlock) void index_tile_lines_kernel(IndexLaunc
   if (t >= J.ntiles) return;
   const int64_t before = block_base[blockIdx.x] + excl;  // newlines before tile t
   if (J.tile_lines) J.tile_lines[t] = t == 0 ? 0 : 1 + before;
+  if (L.fuse_depth > 0) fuse_place_lines(L, J, t, cnt, before);
   if (cnt > L.nl_scratch_cap) {
     atomicOr(overflow, 1);
     return;
@@ -1177,23 +1429,31 @@ static int resident_blocks(const void* fn, int block) {
 
 // The grid is persistent: at most as many blocks as are resident at once, each
 // claiming tiles until none are left, so no block is launched per tile.
-template <int BLOCK, int WPT, int NLV>
-static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
-  static int resident[2] = {0, 0};  // per kernel variant (same value on every device of a process)
-  const void* fn = verify ? reinterpret_cast<const void*>(&index_kernel<BLOCK, WPT, NLV, true>)
-                          : reinterpret_cast<const void*>(&index_kernel<BLOCK, WPT, NLV, false>);
-  int& cap = resident[verify ? 1 : 0];
-  if (cap == 0) cap = resident_blocks(fn, BLOCK);
-  if (grid > cap) grid = cap;
-  if (verify)
-    hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, true>), dim3(grid), dim3(BLOCK), 0, stream, L);
-  else
-    hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, false>), dim3(grid), dim3(BLOCK), 0, stream, L);
+template <int BLOCK, int WPT, int NLV, bool VERIFY, bool FUSE>
+static hipError_t launch_one(const IndexLaunch& L, int grid, hipStream_t stream) {
+  static int resident = 0;  // per kernel variant (same value on every device of a process)
+  if (resident == 0)
+    resident = resident_blocks(reinterpret_cast<const void*>(&index_kernel<BLOCK, WPT, NLV, VERIFY, FUSE>), BLOCK);
+  if (grid > resident) grid = resident;
+  hipLaunchKernelGGL((index_kernel<BLOCK, WPT, NLV, VERIFY, FUSE>), dim3(grid), dim3(BLOCK), 0, stream, L);
   return hipGetLastError();
 }
 
+template <int BLOCK, int WPT, int NLV>
+static hipError_t launch_cfg(const IndexLaunch& L, bool verify, int grid, hipStream_t stream) {
+  if (verify) return launch_one<BLOCK, WPT, NLV, true, false>(L, grid, stream);
+  return launch_one<BLOCK, WPT, NLV, false, false>(L, grid, stream);
+}
+
+// In-tile chain evaluation keeps levels 0..L-1 (L <= 3) in LDS: variants with
+// 2 and 3 level words.
 template <int BLOCK, int WPT>
 static hipError_t launch_nlv(const IndexLaunch& L, int nlv, bool verify, int grid, hipStream_t s) {
+  if (L.fuse_depth > 0) {
+    if (verify || L.one.L > kFuseMaxDepth || !L.nl_scratch) return hipErrorInvalidValue;
+    if (L.one.L <= 2) return launch_one<BLOCK, WPT, 2, false, true>(L, grid, s);
+    return launch_one<BLOCK, WPT, 3, false, true>(L, grid, s);
+  }
   switch (nlv) {
     case 2: return launch_cfg<BLOCK, WPT, 2>(L, verify, grid, s);
     case 4: return launch_cfg<BLOCK, WPT, 4>(L, verify, grid, s);
@@ -1224,12 +1484,12 @@ hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool 
 }
 
 hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream,
-                       const int* run_if) {
+                       const int* run_if, unsigned long long* also) {
   int64_t n8 = ntiles * 8;
   int grid = (int)((n8 + 255) / 256);
   if (grid < 1) grid = 1;
   if (grid > 4096) grid = 4096;
-  hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket, run_if);
+  hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket, run_if, also);
   return hipGetLastError();
 }
 

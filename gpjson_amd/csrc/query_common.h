@@ -67,6 +67,12 @@ struct QueryLaunch {
   unsigned long long* long_count;
   int64_t long_cap;
   int64_t long_min;
+  // When set (one job, lines_dev set): the launch evaluates only the lines
+  // line_list[0 .. min(*list_count, list_cap)), in any order (the lines the
+  // index kernel left to the interpreter after evaluating the others itself).
+  const int64_t* line_list;
+  const unsigned long long* list_count;
+  int64_t list_cap;
   // Every job's IR lies in [ir_base, ir_base + ir_bytes).  When ir_bytes fits
   // kIrLdsBytes, each block copies it to LDS once and interprets from there.
   const uint8_t* ir_base;

@@ -498,8 +498,9 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
   __syncthreads();
   auto ir_of = [&](const QueryJob& J) { return ir_lds ? s_ir + (J.ir - launch.ir_base) : J.ir; };
   const int64_t common = launch.lines_dev ? *launch.lines_dev : 0;
-  const int64_t total = launch.lines_dev ? common * launch.njobs
-                                         : (launch.total_dev ? *launch.total_dev : launch.total_items);
+  int64_t total = launch.lines_dev ? common * launch.njobs
+                                   : (launch.total_dev ? *launch.total_dev : launch.total_items);
+  if (launch.line_list) total = min((int64_t)*launch.list_count, launch.list_cap);
   // Several lines per group amortize a claim over short lines, but with few
   // (long) lines they serialize work one group could share: halve the claim
   // until every group of the grid gets at least one full claim.
@@ -536,8 +537,10 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
       const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
       const uint8_t* ir = ir_of(J);
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-        if (defer(J, common, it, it)) continue;
-        eval_line(g, J, W, ir, common, it, J.out + it * (int64_t)J.R * 2, le, ci, nslots);
+        const int64_t i = launch.line_list ? launch.line_list[it] : it;
+        if (i < 0 || i >= common) continue;
+        if (defer(J, common, i, it)) continue;
+        eval_line(g, J, W, ir, common, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots);
       }
       continue;
     }

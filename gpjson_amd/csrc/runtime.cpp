@@ -223,7 +223,7 @@ int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels,
   const int64_t ntiles = L.total_tiles;
   const int nlv = nlv_for_levels(levels);
   int64_t* block_sums = L.nl_scratch + ntiles * L.nl_scratch_cap;
-  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream));
+  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream, nullptr, L.fuse_count));
   GJ_HIP(zero_u64((unsigned long long*)flags, 8, stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, stream));
   if (ntiles > 0) GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)ntiles, stream));
@@ -235,18 +235,55 @@ int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels,
   IndexLaunch L2 = L;
   L2.speculate = 0;
   L2.run_if = flags;
-  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream, flags));
+  GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream, flags, L.fuse_count));
   if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)ntiles, stream));
   GJ_HIP(index_finish(L2, block_sums, lines_dev, flags + 16, flags + 8, stream));
   return GPJSON_OK;
 }
 
-int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune) {
+bool fuse_spec_from_ir(const uint8_t* ir, int64_t len, FuseSpec* spec) {
+  const int D = chain_depth(ir, len);
+  if (D < 1 || D > kFuseMaxDepth) return false;
+  int64_t pc = 0;
+  int off = 0;
+  for (int l = 0; l < D; l++) {
+    pc++;  // KEY
+    uint32_t v = 0;
+    int sh = 0;
+    for (;;) {
+      const uint8_t b = ir[pc++];
+      v |= (uint32_t)(b & 0x7F) << sh;
+      if (!(b & 0x80)) break;
+      sh += 7;
+    }
+    if (off + (int64_t)v > kFuseKeyBytes) return false;
+    // the in-tile key test relies on keys without a newline byte
+    if (std::memchr(ir + pc, '\n', v)) return false;
+    std::memcpy(spec->keys + off, ir + pc, v);
+    spec->klen[l] = (int)v;
+    off += (int)v;
+    pc += v + 1;  // the key, then DOWN
+  }
+  spec->depth = D;
+  return true;
+}
+
+int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune,
+                  const FuseSpec* fuse) {
   gpjson_ctx* ctx = idx->ctx;
   const IndexTune& t = tune ? *tune : ctx->itune;
   IndexLaunch L{};
   int rc = index_setup(idx, t, &L);
   if (rc) return rc;
+  if (fuse && fuse->depth > 0) {
+    L.fuse_depth = fuse->depth;
+    for (int l = 0; l < kFuseMaxDepth; l++) L.fuse_klen[l] = fuse->klen[l];
+    std::memcpy(L.fuse_keys, fuse->keys, kFuseKeyBytes);
+    L.fuse_spans = fuse->spans;
+    L.fuse_list = fuse->list;
+    L.fuse_list_cap = fuse->list_cap;
+    L.fuse_count = fuse->count;
+  }
   // bookkeeping flags, after the ticket: [16] a speculated carry-in was wrong
   // (it gates the second pass), [24] a tile overflowed its scratch row, [32]
   // the second pass's check (not read)
@@ -1141,6 +1178,10 @@ struct gpjson_plan {
   std::vector<int64_t> ir_lens;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // index kernel, query kernel
   bool timed_pending = false;
+  // one member-chain query: its lines inside index tiles are evaluated by the
+  // index kernel; list holds [count (64 B)][lines left to the interpreter]
+  gj::FuseSpec fuse;
+  int64_t* list = nullptr;
 };
 
 extern "C" {
@@ -1162,6 +1203,25 @@ int gpjson_plan_create(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* i
   rc = dev_alloc((void**)&p->spans, (size_t)nq * (size_t)std::max<int64_t>(p->idx->nl_cap, 1) * R * 2 * 8);
   p->itune = ctx->itune;
   if (rc == GPJSON_OK) rc = prepare_queries(ctx, p->idx, irs, ir_lens, nq, R, p->spans, &p->table, &p->pq);
+  if (rc == GPJSON_OK && nq == 1 && ctx->qtune.fuse && !p->pq.chain && !p->pq.launch.tiled &&
+      fuse_spec_from_ir(irs[0], ir_lens[0], &p->fuse)) {
+    // lines the interpreter may get: at most every line (and the empty file's one)
+    p->fuse.list_cap = p->idx->nl_cap + 1;
+    rc = dev_alloc((void**)&p->list, 64 + (size_t)p->fuse.list_cap * 8);
+    if (rc == GPJSON_OK) {
+      p->fuse.count = (unsigned long long*)p->list;
+      p->fuse.list = p->list + 8;
+      p->fuse.spans = p->spans;
+      QueryLaunch& ql = p->pq.launch;
+      ql.line_list = p->fuse.list;
+      ql.list_count = p->fuse.count;
+      ql.list_cap = p->fuse.list_cap;
+      ql.long_items = nullptr;  // the listed lines are few: no second long-line launch
+      ql.long_count = nullptr;
+    }
+  } else {
+    p->fuse.depth = 0;
+  }
   for (int k = 0; k < 4 && rc == GPJSON_OK; k++)
     if (hipEventCreate(&p->ev[k]) != hipSuccess) rc = fail(GPJSON_ERR_HIP, "hipEventCreate");
   if (rc) {
@@ -1178,7 +1238,8 @@ int gpjson_plan_run(gpjson_plan* p, int timed) {
   GJ_HIP(hipSetDevice(ctx->device));
   // the plan keeps the tile shape it was prepared with (its tiled interpreter
   // launch is sized by that index's tiles), whatever the context's shape is now
-  int rc = enqueue_index(p->idx, timed ? p->ev[0] : nullptr, timed ? p->ev[1] : nullptr, &p->itune);
+  int rc = enqueue_index(p->idx, timed ? p->ev[0] : nullptr, timed ? p->ev[1] : nullptr, &p->itune,
+                         p->fuse.depth > 0 ? &p->fuse : nullptr);
   if (rc) return rc;
   rc = launch_prepared(ctx, p->pq, timed ? p->ev[2] : nullptr, timed ? p->ev[3] : nullptr);
   if (rc) return rc;
@@ -1195,6 +1256,22 @@ int gpjson_plan_timing(gpjson_plan* p, double* index_ms, double* query_ms) {
   GJ_HIP(hipEventElapsedTime(&b, p->ev[2], p->ev[3]));
   if (index_ms) *index_ms = a;
   if (query_ms) *query_ms = b;
+  return GPJSON_OK;
+}
+
+int gpjson_plan_fused(gpjson_plan* p, int* fused, int64_t* interpreted_lines) {
+  if (!p) return fail(GPJSON_ERR_INVALID_ARG, "plan is NULL");
+  if (fused) *fused = p->fuse.depth > 0 ? 1 : 0;
+  if (interpreted_lines) {
+    *interpreted_lines = -1;
+    if (p->fuse.depth > 0) {
+      GJ_HIP(hipSetDevice(p->ctx->device));
+      GJ_HIP(hipStreamSynchronize(p->ctx->stream));
+      unsigned long long c = 0;
+      GJ_HIP(hipMemcpy(&c, p->fuse.count, 8, hipMemcpyDeviceToHost));
+      *interpreted_lines = (int64_t)c;
+    }
+  }
   return GPJSON_OK;
 }
 
@@ -1225,6 +1302,7 @@ void gpjson_plan_free(gpjson_plan* p) {
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
   if (p->spans) (void)hipFree(p->spans);
+  if (p->list) (void)hipFree(p->list);
   p->table.release();
   if (p->idx) gpjson_index_free(p->idx);
   delete p;

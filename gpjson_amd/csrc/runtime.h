@@ -15,8 +15,9 @@ namespace gj {
 // Kernel entry points (index_kernel.hip, query_kernel.hip).
 hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
                         hipStream_t stream);
+// Zeroes the tile states and the ticket (and *also, when set) with a kernel.
 hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream,
-                       const int* run_if = nullptr);
+                       const int* run_if = nullptr, unsigned long long* also = nullptr);
 hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
 // Deferred newline bookkeeping of a one-file launch (IndexLaunch.nl_scratch):
 // tile_lines, the line count, the newline index and the speculation check
@@ -87,6 +88,11 @@ struct QueryTune {
   int group = 1;          // lanes per line
   int waves_per_cu = 16;  // persistent grid size factor
   int lines_per_wave = 4; // lines each group takes per wavefront claim (at most)
+  // A prepared plain pass of one member-chain query evaluates the lines inside
+  // each index tile in the index kernel and interprets only the others.  Off
+  // by default: on 4 GiB of tweets the fused index kernel takes 2.6-3.3 ms
+  // against 1.9 ms for the plain one plus 0.4-1.2 ms of interpreter.
+  int fuse = 0;
 };
 
 int nlv_for_levels(int L);
@@ -170,8 +176,22 @@ int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t befor
                     hipEvent_t after_kernel = nullptr);
 // Enqueues reset + index + line count for one index (no host sync).
 // tune: tile shape to use (NULL: the context's current shape).
+// In-tile evaluation of one member-chain query (IndexLaunch.fuse_*): its keys,
+// and the device arrays the bookkeeping fills.
+struct FuseSpec {
+  int depth = 0;
+  int klen[kFuseMaxDepth] = {0, 0, 0};
+  uint8_t keys[kFuseKeyBytes] = {};
+  int64_t* spans = nullptr;             // int64[lines][2]
+  int64_t* list = nullptr;              // lines left to the interpreter
+  int64_t list_cap = 0;
+  unsigned long long* count = nullptr;  // their number
+};
+// Fills spec's keys from a member chain of at most kFuseMaxDepth keys whose
+// key bytes fit kFuseKeyBytes; false for any other IR.
+bool fuse_spec_from_ir(const uint8_t* ir, int64_t len, FuseSpec* spec);
 int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel = nullptr, hipEvent_t after_kernel = nullptr,
-                  const IndexTune* tune = nullptr);
+                  const IndexTune* tune = nullptr, const FuseSpec* fuse = nullptr);
 // Enqueues one interpreter launch of nq queries over idx with the line count
 // read on the device; spans_dev receives int64[nq][lines][R][2].
 int enqueue_queries_dev(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* const* irs,

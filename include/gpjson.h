@@ -214,6 +214,9 @@ int gpjson_plan_create(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *i
 int gpjson_plan_run(gpjson_plan *plan, int timed);
 /* Waits, then returns index / query time in ms of the last timed run. */
 int gpjson_plan_timing(gpjson_plan *plan, double *index_ms, double *query_ms);
+/* Whether the plan evaluates lines inside index tiles (gpjson_set_fusion), and
+ * (waits) how many lines the last run left to the interpreter (-1 if not). */
+int gpjson_plan_fused(gpjson_plan *plan, int *fused, int64_t *interpreted_lines);
 int64_t gpjson_plan_lines(const gpjson_plan *plan);
 int gpjson_plan_results_per_line(const gpjson_plan *plan);
 /* Waits, then copies int64[nq][lines][R][2] spans of the last run to host memory. */
@@ -324,6 +327,16 @@ int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *qu
                       int *query_group, int *query_waves_per_cu, int *lines_per_wave);
 int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query_mode, int query_block,
                       int query_group, int query_waves_per_cu, int lines_per_wave);
+/* In-tile evaluation (no reference counterpart; off by default): a prepared
+ * plain pass (gpjson_plan_create) of ONE member-chain query $.k0.k1[.k2] (at
+ * most 3 keys, 120 key bytes) evaluates every line that starts and ends inside
+ * an index tile in the index kernel itself, from the tile's masks in LDS, and
+ * writes the tile's records and leveled words only around the lines that cross
+ * into its neighbours; the interpreter then runs on those lines alone.  Spans
+ * are the same either way.  Plans prepared while it is off interpret every
+ * line from the full index. */
+int gpjson_set_fusion(gpjson_ctx *ctx, int enable);
+int gpjson_get_fusion(gpjson_ctx *ctx, int *enabled);
 /* Times every supported shape on buf with the given queries and installs the
  * fastest; report (optional) gets one text line per candidate. */
 int gpjson_autotune(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens, int nq,
