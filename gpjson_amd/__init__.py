@@ -16,6 +16,7 @@ from .api import (  # noqa: F401
     ResultArray,
     ResultItem,
     UnsupportedJSONPathException,
+    FallbackRequired,
     compile_path,
     eval,
 )
@@ -23,6 +24,6 @@ from ._native import NativeLibraryMissing  # noqa: F401
 
 __all__ = [
     "GPJSON", "Buffer", "CompiledPath", "Context", "GPJSONException", "Index", "JSONPathException",
-    "Result", "ResultArray", "ResultItem", "UnsupportedJSONPathException", "compile_path", "eval",
+    "Result", "ResultArray", "ResultItem", "UnsupportedJSONPathException", "FallbackRequired", "compile_path", "eval",
     "NativeLibraryMissing",
 ]

@@ -21,6 +21,7 @@ GPJSON_ERR_UNSUPPORTED = 4
 GPJSON_ERR_IO = 5
 GPJSON_ERR_CAPACITY = 6
 GPJSON_ERR_NO_MEMORY = 7
+GPJSON_FALLBACK_REQUIRED = 8
 
 GPJSON_INDEX_NEWLINE = 0
 GPJSON_INDEX_STRING = 1

@@ -46,10 +46,19 @@ class UnsupportedJSONPathException(JSONPathException):
     """JSONPath outside the supported subset (J/jsonpath/UnsupportedJSONPathException.java)."""
 
 
+class FallbackRequired(UnsupportedJSONPathException):
+    """query() met a path outside the supported subset with fallback allowed: the
+    reference answers it with jayway json-path on the CPU (J/functions/
+    QueryFunction.java:122-128, 267-314); this library runs nothing and leaves
+    that evaluation to the caller's own JSONPath library."""
+
+
 def _check(rc: int) -> None:
     if rc == N.GPJSON_OK:
         return
     msg = N.last_error()
+    if rc == N.GPJSON_FALLBACK_REQUIRED:
+        raise FallbackRequired(msg, rc)
     if rc == N.GPJSON_ERR_UNSUPPORTED:
         raise UnsupportedJSONPathException(msg, rc)
     if rc == N.GPJSON_ERR_PARSE:

@@ -6,8 +6,7 @@
 // the tile shape (threads per block x 64-byte words per thread, i.e. the chunk
 // of input one tile covers), the interpreter's block size, its persistent grid
 // size and lines claimed per wavefront are measured on the caller's own input
-// and the fastest shape is installed in the context.  Results are plain data
-// (the Python layer caches them per device / input-size / depth bucket).
+// and the fastest shape is installed in the context.
 #include <algorithm>
 #include <cstdio>
 #include <string>

@@ -1090,6 +1090,26 @@ __device__ int tile_block_scan(int v, int* excl) {
   return total;
 }
 
+// Block-wide exclusive scan of one int64 per thread (kTileBlock threads;
+// Hillis-Steele in LDS); returns the block total.  The block sums of step 2
+// need 64 bits: 256 of them may hold more than 2^31 newlines.
+__device__ int64_t tile_block_scan64(int64_t v, int64_t* excl) {
+  __shared__ int64_t s_v[kTileBlock];
+  const int t = threadIdx.x;
+  s_v[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kTileBlock; off <<= 1) {
+    const int64_t add = t >= off ? s_v[t - off] : 0;
+    __syncthreads();
+    s_v[t] += add;
+    __syncthreads();
+  }
+  *excl = s_v[t] - v;
+  const int64_t total = s_v[kTileBlock - 1];
+  __syncthreads();
+  return total;
+}
+
 // Step 1: newline counts (from the published aggregates) summed per block.
 // Every tile's stored carry-in is checked against its predecessor's carry-out
 // (the predecessor's aggregate applied to its carry-in): from tile 0's exact
@@ -1138,9 +1158,9 @@ __global__ __launch_bounds__(kTileBlock) void index_tile_base_kernel(IndexLaunch
   int64_t carry = 0;
   for (int64_t c0 = 0; c0 < nb; c0 += kTileBlock) {
     const int64_t i = c0 + threadIdx.x;
-    const int v = i < nb ? (int)block_sums[i] : 0;  // a block's newlines fit an int
-    int excl;
-    const int total = tile_block_scan(v, &excl);
+    const int64_t v = i < nb ? block_sums[i] : 0;
+    int64_t excl;
+    const int64_t total = tile_block_scan64(v, &excl);
     if (i < nb) block_sums[i] = carry + excl;
     carry += total;
   }
@@ -1288,9 +1308,9 @@ __global__ __launch_bounds__(kTileBlock) void batch_tile_base_kernel(IndexLaunch
   int64_t carry = 0;
   for (int64_t c0 = 0; c0 < nb; c0 += kTileBlock) {
     const int64_t i = c0 + threadIdx.x;
-    const int v = i < nb ? (int)block_sums[i] : 0;
-    int excl;
-    const int total = tile_block_scan(v, &excl);
+    const int64_t v = i < nb ? block_sums[i] : 0;
+    int64_t excl;
+    const int64_t total = tile_block_scan64(v, &excl);
     if (i < nb) block_sums[i] = carry + excl;
     carry += total;
   }

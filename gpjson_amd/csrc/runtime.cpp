@@ -989,8 +989,9 @@ int gpjson_query_file(gpjson_ctx* ctx, const char* file, const char* const* path
     if (st == PathStatus::kUnsupported) {
       if (flags & GPJSON_QUERY_DISABLE_FALLBACK)
         return fail(GPJSON_ERR_UNSUPPORTED, "Unsupported JSON path (" + std::to_string(i) + ", fallback disabled): " + err);
-      return fail(GPJSON_ERR_UNSUPPORTED,
-                  "Unsupported JSON path (" + std::to_string(i) + "): " + err + " (no CPU fallback in this library)");
+      return fail(GPJSON_FALLBACK_REQUIRED,
+                  "Unsupported JSON path (" + std::to_string(i) + "): " + err +
+                      "; fallback allowed: evaluate the query with the caller's JSONPath library");
     }
     if (st == PathStatus::kParseError)
       return fail(GPJSON_ERR_PARSE, "Failed to parse JSON path (" + std::to_string(i) + "): " + err);

@@ -40,7 +40,12 @@ enum {
   GPJSON_ERR_UNSUPPORTED = 4,  /* UnsupportedJSONPathException (reference falls back to jayway) */
   GPJSON_ERR_IO = 5,           /* file missing / changed while reading */
   GPJSON_ERR_CAPACITY = 6,     /* caller buffer too small or a stated limit exceeded */
-  GPJSON_ERR_NO_MEMORY = 7     /* device or pinned allocation failed / HBM budget exceeded */
+  GPJSON_ERR_NO_MEMORY = 7,    /* device or pinned allocation failed / HBM budget exceeded */
+  /* gpjson_query_file: a path is outside the supported subset and the options
+     allow the reference's fallback (functions/QueryFunction.java:122-128,
+     267-314): nothing ran on the GPU, and the caller evaluates the query with
+     its own JSONPath library (the reference uses jayway json-path) */
+  GPJSON_FALLBACK_REQUIRED = 8
 };
 
 /* Index export selectors for gpjson_index_export. */

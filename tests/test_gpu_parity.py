@@ -222,8 +222,13 @@ def test_multi_query_and_options(tmp_path):
         assert list(res[0][0]) == ['"nl"', None]       # fresh -1 per query
         assert list(res[1][0]) == ['"13"', "14"]
         assert list(res[2][0]) == [None, None]
-        with pytest.raises(gpjson_amd.UnsupportedJSONPathException):
+        with pytest.raises(gpjson_amd.UnsupportedJSONPathException) as e:
             g.query(str(f), "$..lang", {"disable_fallback": True})
+        assert not isinstance(e.value, gpjson_amd.FallbackRequired)
+        # fallback allowed: the caller is told to evaluate the query itself
+        with pytest.raises(gpjson_amd.FallbackRequired) as e:
+            g.query(str(f), ["$.user.lang", "$..lang"])
+        assert e.value.code == 8 and "(1)" in str(e.value)
         with pytest.raises(gpjson_amd.JSONPathException):
             g.query(str(f), "$.a b")
         with pytest.raises(TypeError):

@@ -54,3 +54,26 @@ def test_four_gib_against_numpy_and_oracle(ctx):
         # the high lines do carry results with offsets past 2^31
         assert int((got[high:, 0, 0] >= (1 << 31)).sum()) > 0
     buf.free()
+
+
+def test_more_than_2_31_newlines(ctx):
+    """A file of 2^31 + 2^27 newlines (every line empty): the line count and
+    newline offsets pass 2^31 inside one group of 65 536 index tiles, where the
+    bookkeeping's scan of per-256-tile newline counts must be 64-bit.  Every
+    tile holds more newlines than its scratch row, so the rescan path builds
+    the whole newline index."""
+    n = (1 << 31) + (1 << 27)
+    print(f"[scale] {n} newlines", flush=True)
+    data = np.full(n, 10, dtype=np.uint8)
+    buf = ctx.stage(data)
+    del data
+    idx = ctx.index(buf, 1)
+    assert idx.lines == n + 1
+    nl = idx.newline_index()
+    idx.free()
+    buf.free()
+    assert nl.size == n + 1 and nl[0] == 0
+    step = 1 << 27
+    for a in range(0, n, step):  # nl[k] = k - 1 for k >= 1
+        b = min(a + step, n)
+        assert np.array_equal(nl[1 + a:1 + b], np.arange(a, b, dtype=np.int64)), a
