@@ -339,6 +339,19 @@ def batch_section(args) -> dict:
     log(f"plain: {plain_dt * 1e3:.2f} ms per pass over all files, {plain_gbps:.1f} GB/s")
 
     batch = ctx.batch(bufs, irs)
+    batch_shapes = batch.get_config()
+    tune_s = None
+    if not args.no_batch_autotune:
+        # the batch's own shapes (tile = chunk of input per work item, block,
+        # interpreter shape, lines per claim), timed on the batch itself
+        t = time.perf_counter()
+        report = batch.autotune(args.batch_tune_iters)
+        tune_s = time.perf_counter() - t
+        lines = report.strip().splitlines()
+        for line in lines[:-1]:
+            log("batch autotune: " + line)
+        log(f"{lines[-1]} ({tune_s:.1f} s)")
+        batch_shapes = batch.get_config()
     for _ in range(2):
         batch.run(graph=True)
     ctx.synchronize()
@@ -350,6 +363,25 @@ def batch_section(args) -> dict:
     value = nbytes / batch_dt / 1e9
     _, total_ms = batch.timing()
     log(f"batched: {batch_dt * 1e3:.2f} ms per run, {value:.1f} GB/s (events {total_ms:.2f} ms)")
+
+    # the plain runs again with the batch's tuned shapes, for a like-for-like ratio
+    plain_same = None
+    if batch_shapes != shapes:
+        ctx.set_config(*batch_shapes.values())
+        tuned_plans = [ctx.plan(b, q) for b, q in zip(bufs, irs)]
+        for p in tuned_plans:
+            p.run()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            for p in tuned_plans:
+                p.run()
+        ctx.synchronize()
+        plain_same = nbytes / ((time.perf_counter() - t0) / args.batch_steps) / 1e9
+        for p in tuned_plans:
+            p.free()
+        ctx.set_config(*shapes.values())
+        log(f"plain with the batch's shapes: {plain_same:.1f} GB/s")
 
     # bit-exact: every (file, query) of the batch equals the plain plan
     mism = 0
@@ -387,6 +419,10 @@ def batch_section(args) -> dict:
         "plain_GBps": round(plain_gbps, 3),
         "plain_ms_per_pass": round(plain_dt * 1e3, 3),
         "batched_over_plain": round(value / plain_gbps, 3),
+        "plain_GBps_batch_shapes": round(plain_same, 3) if plain_same else round(plain_gbps, 3),
+        "batched_over_plain_batch_shapes": round(value / (plain_same or plain_gbps), 3),
+        "batch_launch_shapes": batch_shapes,
+        "batch_autotune_s": round(tune_s, 1) if tune_s is not None else None,
         "bit_exact_vs_plain": mism == 0,
         "bit_exact_vs_oracle_sample": o_mism == 0,
         "oracle_sample": f"{len(sample_data)} files ({o_bytes} bytes), all 8 queries each, oracle on {threads} threads",
@@ -666,6 +702,9 @@ def main() -> int:
     ap.add_argument("--batch-files", type=int, default=256)
     ap.add_argument("--batch-total", type=int, default=16 << 30)
     ap.add_argument("--batch-steps", type=int, default=3, help="batched and 256-file plain runs timed")
+    ap.add_argument("--no-batch-autotune", action="store_true",
+                    help="run the batch with the context's shapes instead of its own autotuned ones")
+    ap.add_argument("--batch-tune-iters", type=int, default=2, help="timed runs per candidate shape")
     ap.add_argument("--no-batch", action="store_true", help="default mode: leave out the batched section")
     ap.add_argument("--serve-files", type=int, default=64)
     ap.add_argument("--serve-total", type=int, default=100 << 30)

@@ -102,6 +102,9 @@ SIGNATURES = {
     "gpjson_batch_results_per_line": (c_int, [c_void_p, c_int, c_int]),
     "gpjson_batch_spans": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64]),
     "gpjson_batch_free": (None, [c_void_p]),
+    "gpjson_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
+    "gpjson_batch_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int),
+                                        P(c_int)]),
     "gpjson_server_create": (c_int, [c_int, c_int64, c_int, P(c_void_p)]),
     "gpjson_server_add": (c_int, [c_void_p, c_char_p, c_void_p, c_int64]),
     "gpjson_server_admit": (c_int, [c_void_p, c_char_p]),
@@ -120,6 +123,10 @@ SIGNATURES = {
     "gpjson_set_fusion": (c_int, [c_void_p, c_int]),
     "gpjson_get_fusion": (c_int, [c_void_p, P(c_int)]),
     "gpjson_autotune": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, c_char_p, c_size_t]),
+    "gpjson_tune_cache_clear": (c_int, [c_void_p]),
+    "gpjson_tune_cache_size": (c_int64, [c_void_p]),
+    "gpjson_tune_cache_save": (c_int, [c_void_p, c_char_p]),
+    "gpjson_tune_cache_load": (c_int, [c_void_p, c_char_p, P(c_int)]),
     "gpjson_query_noalloc": (c_int, [c_void_p, c_void_p, c_int64, P(c_void_p), P(c_int64), c_int, c_void_p, c_int64,
                                      c_void_p, c_int64, c_void_p]),
     "gpjson_time_plain": (c_int, [c_void_p, c_void_p, P(c_void_p), P(c_int64), c_int, c_int, P(c_double),

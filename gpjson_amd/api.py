@@ -22,6 +22,7 @@ parity tests the way the reference's KernelTest exposes its stage classes
 from __future__ import annotations
 
 import ctypes
+import os
 import weakref
 from typing import List, Optional, Sequence, Tuple, Union
 
@@ -242,7 +243,8 @@ class Context:
         return bool(v.value)
 
     def autotune(self, buf: "Buffer", irs: Sequence[bytes], iterations: int = 3) -> str:
-        """Times every launch shape on buf and installs the fastest; returns the report."""
+        """Times every launch shape on buf and records the fastest in the tuning
+        cache for inputs of buf's size bucket at the queries' depth; returns the report."""
         nq = len(irs)
         bufs = [ctypes.create_string_buffer(bytes(ir), max(len(ir), 1)) for ir in irs]
         ptrs = (ctypes.c_void_p * nq)(*[ctypes.cast(b, ctypes.c_void_p) for b in bufs])
@@ -250,6 +252,21 @@ class Context:
         rep = ctypes.create_string_buffer(16384)
         _check(self._lib.gpjson_autotune(self._h, buf.handle, ptrs, lens, nq, iterations, rep, 16384))
         return rep.value.decode()
+
+    def tune_cache_size(self) -> int:
+        return int(self._lib.gpjson_tune_cache_size(self._h))
+
+    def tune_cache_clear(self) -> None:
+        _check(self._lib.gpjson_tune_cache_clear(self._h))
+
+    def tune_cache_save(self, path: str) -> None:
+        _check(self._lib.gpjson_tune_cache_save(self._h, os.fsencode(path)))
+
+    def tune_cache_load(self, path: str) -> int:
+        """Loads the entries of this device's architecture; returns their number."""
+        n = ctypes.c_int()
+        _check(self._lib.gpjson_tune_cache_load(self._h, os.fsencode(path), ctypes.byref(n)))
+        return n.value
 
     def time_copy(self, nbytes: int, iterations: int = 5) -> float:
         """ms per device-to-device streaming copy of nbytes (reads and writes nbytes)."""
@@ -329,6 +346,20 @@ class Batch:
 
     def run(self, graph: bool = True, timed: bool = False) -> None:
         _check(N.load().gpjson_batch_run(self._h, 1 if graph else 0, 1 if timed else 0))
+
+    def autotune(self, iterations: int = 2) -> str:
+        """Times the batch's own runs under every launch shape, keeps the fastest
+        (and records it in the context's tuning cache); returns the report."""
+        rep = ctypes.create_string_buffer(32768)
+        _check(N.load().gpjson_batch_autotune(self._h, iterations, rep, 32768))
+        return rep.value.decode()
+
+    def get_config(self) -> dict:
+        v = [ctypes.c_int() for _ in range(7)]
+        _check(N.load().gpjson_batch_get_config(self._h, *[ctypes.byref(x) for x in v]))
+        keys = ["index_block", "index_wpt", "query_mode", "query_block", "query_group", "query_waves_per_cu",
+                "lines_per_group"]
+        return {k: x.value for k, x in zip(keys, v)}
 
     def timing(self):
         a, t = ctypes.c_double(), ctypes.c_double()

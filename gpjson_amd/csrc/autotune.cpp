@@ -1,14 +1,20 @@
-// autotune.cpp -- launch-shape search for the index and interpreter kernels.
+// autotune.cpp -- launch-shape search for the index and interpreter kernels,
+// and the tuning cache that keeps its results.
 //
 // The reference hard-codes 8x1024 threads for the index kernels and 512x1024
 // for find_value (J/stages/CombinedIndex.java:11-14, J/functions/
 // QueryFunction.java:229; J/ = gpjson/src/main/java/com/koenv/gpjson/).  Here
 // the tile shape (threads per block x 64-byte words per thread, i.e. the chunk
 // of input one tile covers), the interpreter's block size, its persistent grid
-// size and lines claimed per wavefront are measured on the caller's own input
-// and the fastest shape is installed in the context.
+// size and lines claimed per wavefront are measured on the caller's own input.
+// The fastest shapes go into the context's tuning cache under (input size
+// bucket, levels, plain or batch); every later plain pass, one-shot query and
+// batch with the same key runs with them.  The cache can be saved to a text
+// file and loaded into another context of the same architecture.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -16,15 +22,80 @@
 
 using namespace gj;
 
+namespace gj {
+
+const std::vector<IndexTune>& index_candidates() {
+  static const std::vector<IndexTune> c = {{256, 1}, {256, 2}, {256, 3}, {256, 4},
+                                           {512, 1}, {512, 2}, {128, 2}, {128, 4}};
+  return c;
+}
+
+// Interpreter shapes: line mode over lane groups and claim sizes, long lines
+// to a wavefront each, and the member-chain evaluator (used only when every
+// query of a launch is a member chain).
+std::vector<QueryTune> query_candidates(const QueryTune& base) {
+  std::vector<QueryTune> c;
+  auto add = [&](int mode, int block, int group, int wpc, int lpw) {
+    QueryTune q = base;
+    q.mode = mode;
+    q.block = block;
+    q.group = group;
+    q.waves_per_cu = wpc;
+    q.lines_per_wave = lpw;
+    c.push_back(q);
+  };
+  for (int group : {1, 4, 16})
+    for (int block : {128, 256}) add(kTiledMode, block, group, base.waves_per_cu, 1);
+  for (int group : {1, 4, 16, 64})
+    for (int wpc : {16, 32})
+      for (int lpw : {1, 4}) add(kLineMode, 256, group, wpc, lpw);
+  for (int group : {1, 4})
+    for (int wpc : {16, 32}) add(kLineLongMode, 256, group, wpc, group == 1 ? 4 : 1);
+  for (int group : {1, 4, 16})
+    for (int wpc : {8, 16, 32}) add(kChainMode, 256, group, wpc, 1);
+  return c;
+}
+
+int size_bucket(int64_t n) { return n > 1 ? 63 - __builtin_clzll((unsigned long long)n) : 0; }
+
+void shapes_for(const gpjson_ctx* ctx, int64_t n, int L, int batch, IndexTune* it, QueryTune* qt) {
+  IndexTune i = ctx->itune;
+  QueryTune q = ctx->qtune;
+  if (!ctx->tuning) {
+    const auto e = ctx->tune_cache.find(TuneKey{size_bucket(n), L, batch});
+    if (e != ctx->tune_cache.end()) {
+      i = e->second.itune;
+      q = e->second.qtune;
+      q.fuse = ctx->qtune.fuse;
+    }
+  }
+  if (it) *it = i;
+  if (qt) *qt = q;
+}
+
+}  // namespace gj
+
 namespace {
 
-struct IdxCand {
-  int block, wpt;
-};
-const IdxCand kIdx[] = {{256, 1}, {256, 2}, {256, 3}, {256, 4}, {512, 1}, {512, 2}, {128, 2}, {128, 4}};
+void copy_report(const std::string& rep, char* report, size_t report_cap) {
+  if (report && report_cap) {
+    const size_t n = std::min(report_cap - 1, rep.size());
+    std::copy(rep.begin(), rep.begin() + n, report);
+    report[n] = 0;
+  }
+}
 
-struct QCand {
-  int mode, block, group, wpc, lpw;
+// Restores the context's tuning state on every exit of an autotuner.
+struct TuningScope {
+  gpjson_ctx* ctx;
+  IndexTune keep_i;
+  QueryTune keep_q;
+  explicit TuningScope(gpjson_ctx* c) : ctx(c), keep_i(c->itune), keep_q(c->qtune) { ctx->tuning = true; }
+  ~TuningScope() {
+    ctx->itune = keep_i;
+    ctx->qtune = keep_q;
+    ctx->tuning = false;
+  }
 };
 
 }  // namespace
@@ -65,6 +136,7 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
   ctx->qtune.group = query_group;
   ctx->qtune.waves_per_cu = query_waves_per_cu;
   ctx->qtune.lines_per_wave = lines_per_wave;
+  ctx->tune_cache.clear();  // shapes set by hand apply to every input
   return GPJSON_OK;
 }
 
@@ -82,113 +154,144 @@ int gpjson_get_fusion(gpjson_ctx* ctx, int* enabled) {
 
 // Times every candidate shape on buf with the given queries (index kernel and
 // interpreter separately, `iterations` timed runs each after one warm-up) and
-// installs the fastest.  `report` (optional) receives one line per candidate.
+// records the fastest in the tuning cache under (bucket of buf's size, the
+// queries' depth, plain).  `report` (optional) receives one line per candidate.
 int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs, const int64_t* ir_lens, int nq,
                     int iterations, char* report, size_t report_cap) {
   if (!ctx || !buf || !irs || !ir_lens || nq < 1 || iterations < 1)
     return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
   GJ_HIP(hipSetDevice(ctx->device));
+  int L = 0, R = 1;
+  ir_shape(irs, ir_lens, nq, &L, &R);
+  TuningScope scope(ctx);
   std::string rep;
-  const IndexTune keep_i = ctx->itune;
-  const QueryTune keep_q = ctx->qtune;
-  double best = 1e30;
-  IndexTune best_i = keep_i;
   char line[256];
-  for (const IdxCand& c : kIdx) {
-    ctx->itune.block = c.block;
-    ctx->itune.wpt = c.wpt;
+  // one plan per candidate (a plan caches its launch shape); each is timed
+  // `iterations` times after a warm-up run
+  auto time_plan = [&](double* index_ms, double* query_ms) -> int {
     gpjson_plan* p = nullptr;
     int rc = gpjson_plan_create(ctx, buf, irs, ir_lens, nq, &p);
-    if (rc) {
-      ctx->itune = keep_i;
-      ctx->qtune = keep_q;
-      return rc;
-    }
-    double sum = 0;
+    if (rc) return rc;
+    double si = 0, sq = 0;
     rc = gpjson_plan_run(p, 0);
     for (int it = 0; it < iterations && rc == GPJSON_OK; it++) {
       double a = 0, b = 0;
       rc = gpjson_plan_run(p, 1);
       if (rc == GPJSON_OK) rc = gpjson_plan_timing(p, &a, &b);
-      sum += a;
+      si += a;
+      sq += b;
     }
     gpjson_plan_free(p);
-    if (rc) {
-      ctx->itune = keep_i;
-      ctx->qtune = keep_q;
-      return rc;
-    }
-    double ms = sum / iterations;
+    *index_ms = si / iterations;
+    *query_ms = sq / iterations;
+    return rc;
+  };
+  double best = 1e30;
+  IndexTune best_i = ctx->itune;
+  for (const IndexTune& c : index_candidates()) {
+    ctx->itune = c;
+    double ims = 0, qms = 0;
+    const int rc = time_plan(&ims, &qms);
+    if (rc) return rc;
     std::snprintf(line, sizeof(line), "index block=%d wpt=%d tile=%lld B: %.4f ms\n", c.block, c.wpt,
-                  (long long)index_tile_bytes(c.block, c.wpt), ms);
+                  (long long)index_tile_bytes(c.block, c.wpt), ims);
     rep += line;
-    if (ms < best) {
-      best = ms;
-      best_i = ctx->itune;
+    if (ims < best) {
+      best = ims;
+      best_i = c;
     }
   }
   ctx->itune = best_i;
-  std::vector<QCand> qc;
-  for (int group : {1, 4, 16})
-    for (int block : {128, 256}) qc.push_back({1, block, group, 0, 1});
-  for (int group : {1, 4, 16, 64})
-    for (int wpc : {16, 32})
-      for (int lpw : {1, 4}) qc.push_back({0, 256, group, wpc, lpw});
-  // long lines to a wavefront each
-  for (int group : {1, 4})
-    for (int wpc : {16, 32}) qc.push_back({3, 256, group, wpc, group == 1 ? 4 : 1});
-  // member-chain evaluator (used only when every query is a member chain)
-  for (int group : {1, 4, 16})
-    for (int wpc : {8, 16, 32}) qc.push_back({2, 256, group, wpc, 1});
-  int rc = GPJSON_OK;
   double bestq = 1e30;
-  QueryTune best_q = keep_q;
-  for (const QCand& c : qc) {
-    ctx->qtune.mode = c.mode;
-    ctx->qtune.block = c.block;
-    ctx->qtune.group = c.group;
-    if (c.wpc) ctx->qtune.waves_per_cu = c.wpc;
-    ctx->qtune.lines_per_wave = c.lpw;
-    // a plan caches the launch shape: prepare one per candidate
-    gpjson_plan* p = nullptr;
-    rc = gpjson_plan_create(ctx, buf, irs, ir_lens, nq, &p);
-    if (rc) break;
-    double sum = 0;
-    rc = gpjson_plan_run(p, 0);
-    for (int it = 0; it < iterations && rc == GPJSON_OK; it++) {
-      double a = 0, b = 0;
-      rc = gpjson_plan_run(p, 1);
-      if (rc == GPJSON_OK) rc = gpjson_plan_timing(p, &a, &b);
-      sum += b;
-    }
-    gpjson_plan_free(p);
-    if (rc) break;
-    double ms = sum / iterations;
+  QueryTune best_q = ctx->qtune;
+  for (const QueryTune& c : query_candidates(ctx->qtune)) {
+    ctx->qtune = c;
+    double ims = 0, qms = 0;
+    const int rc = time_plan(&ims, &qms);
+    if (rc) return rc;
     std::snprintf(line, sizeof(line), "query mode=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n",
-                  c.mode, c.block, c.group, c.wpc, c.lpw, ms);
+                  c.mode, c.block, c.group, c.waves_per_cu, c.lines_per_wave, qms);
     rep += line;
-    if (ms < bestq) {
-      bestq = ms;
-      best_q = ctx->qtune;
+    if (qms < bestq) {
+      bestq = qms;
+      best_q = c;
     }
   }
-  if (rc) {
-    ctx->itune = keep_i;
-    ctx->qtune = keep_q;
-    return rc;
-  }
-  ctx->qtune = best_q;
+  const TuneKey key{size_bucket(buf->n), L, 0};
+  ctx->tune_cache[key] = TuneEntry{best_i, best_q, best + bestq};
   std::snprintf(line, sizeof(line),
-                "best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms)\n", best_i.block,
-                best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
-                best_q.lines_per_wave, bestq);
+                "best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms); "
+                "cached for inputs of [2^%d, 2^%d) bytes at %d levels\n",
+                best_i.block, best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
+                best_q.lines_per_wave, bestq, key.bucket, key.bucket + 1, L);
   rep += line;
-  if (report && report_cap) {
-    size_t n = std::min(report_cap - 1, rep.size());
-    std::copy(rep.begin(), rep.begin() + n, report);
-    report[n] = 0;
-  }
+  copy_report(rep, report, report_cap);
   return GPJSON_OK;
+}
+
+int gpjson_tune_cache_clear(gpjson_ctx* ctx) {
+  if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
+  ctx->tune_cache.clear();
+  return GPJSON_OK;
+}
+
+int64_t gpjson_tune_cache_size(const gpjson_ctx* ctx) { return ctx ? (int64_t)ctx->tune_cache.size() : -1; }
+
+// Text format: a header line, then one line per entry:
+//   arch bucket levels batch index_block index_wpt query_mode query_block
+//   query_group waves_per_cu lines_per_wave ms
+int gpjson_tune_cache_save(const gpjson_ctx* ctx, const char* path) {
+  if (!ctx || !path) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(GPJSON_ERR_IO, std::string("Failed to open file ") + path);
+  std::fprintf(f, "gpjson-tune-cache 1\n");
+  for (const auto& kv : ctx->tune_cache) {
+    const TuneKey& k = kv.first;
+    const TuneEntry& e = kv.second;
+    std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n", ctx->arch.c_str(), k.bucket, k.levels, k.batch,
+                 e.itune.block, e.itune.wpt, e.qtune.mode, e.qtune.block, e.qtune.group, e.qtune.waves_per_cu,
+                 e.qtune.lines_per_wave, e.ms);
+  }
+  const bool ok = std::fflush(f) == 0;
+  std::fclose(f);
+  return ok ? GPJSON_OK : fail(GPJSON_ERR_IO, std::string("Failed to write file ") + path);
+}
+
+// Loads the entries of ctx's architecture (others are skipped) over the cache's
+// own; *loaded (optional) gets their number.  Entries with shapes this build
+// does not support are rejected.
+int gpjson_tune_cache_load(gpjson_ctx* ctx, const char* path, int* loaded) {
+  if (!ctx || !path) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(GPJSON_ERR_IO, std::string("Failed to open file ") + path);
+  char buf[512];
+  int n = 0, rc = GPJSON_OK;
+  if (!std::fgets(buf, sizeof(buf), f) || std::strncmp(buf, "gpjson-tune-cache 1", 19) != 0)
+    rc = fail(GPJSON_ERR_INVALID_ARG, std::string("not a tuning cache file: ") + path);
+  while (rc == GPJSON_OK && std::fgets(buf, sizeof(buf), f)) {
+    char arch[128];
+    TuneKey k;
+    TuneEntry e;
+    if (std::sscanf(buf, "%127s %d %d %d %d %d %d %d %d %d %d %lf", arch, &k.bucket, &k.levels, &k.batch,
+                    &e.itune.block, &e.itune.wpt, &e.qtune.mode, &e.qtune.block, &e.qtune.group,
+                    &e.qtune.waves_per_cu, &e.qtune.lines_per_wave, &e.ms) != 12) {
+      rc = fail(GPJSON_ERR_INVALID_ARG, std::string("malformed tuning cache line: ") + buf);
+      break;
+    }
+    if (ctx->arch != arch) continue;
+    if (!index_config_supported(e.itune.block, e.itune.wpt) || !query_config_supported(e.qtune.block, e.qtune.group) ||
+        e.qtune.mode < kLineMode || e.qtune.mode > kLineLongMode || e.qtune.waves_per_cu < 1 ||
+        e.qtune.waves_per_cu > 64 || e.qtune.lines_per_wave < 1 || e.qtune.lines_per_wave > 1024 || k.levels < 0 ||
+        k.levels > GPJSON_MAX_LEVELS || k.bucket < 0 || k.bucket > 62 || (k.batch != 0 && k.batch != 1)) {
+      rc = fail(GPJSON_ERR_INVALID_ARG, std::string("unsupported shapes in tuning cache line: ") + buf);
+      break;
+    }
+    ctx->tune_cache[k] = e;
+    n++;
+  }
+  std::fclose(f);
+  if (loaded) *loaded = n;
+  return rc;
 }
 
 }  // extern "C"

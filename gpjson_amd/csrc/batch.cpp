@@ -27,8 +27,10 @@
 // After the first (uncaptured) run has sized the arrays, the sequence is
 // captured once into a hipGraph and replayed; nothing is allocated per run.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 
 #include "runtime.h"
 
@@ -45,7 +47,11 @@ struct gpjson_batch {
   std::vector<int64_t> job_span_off;  // element offset of each job's spans in the span arena
   std::vector<int> file_first_job;
   int64_t total_tiles = 0;
+  int64_t total_bytes = 0;
+  int max_levels = 0;
   int64_t ir_total = 0;  // IR bytes of every job, contiguous in dev_ir
+  std::vector<uint8_t> ir_host;    // the same bytes on the host (a new layout uploads them again)
+  std::vector<int64_t> ir_lens_host;
   bool chain = false;    // every job is a member chain: chain evaluator
   // device memory
   DevBuf arena_rec, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
@@ -94,9 +100,13 @@ static int levels_of(const uint8_t* ir, int64_t len, int* R) {
   return maxd;
 }
 
-// Lays out the arenas for the current capacities and uploads the tables.
-static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps, const uint8_t* const* irs,
-                  const int64_t* ir_lens) {
+// Lays out the arenas for the current capacities and shapes and uploads the
+// tables.
+static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
+  std::vector<const uint8_t*> irv(b->njobs);
+  for (int64_t j = 0, off = 0; j < b->njobs; off += b->ir_lens_host[j], j++) irv[j] = b->ir_host.data() + off;
+  const uint8_t* const* irs = irv.data();
+  const int64_t* ir_lens = b->ir_lens_host.data();
   gpjson_ctx* ctx = b->ctx;
   const int64_t tile_bytes = index_tile_bytes(b->itune.block, b->itune.wpt);
   int64_t s_words = 0, lb_words = 0, nl_elems = 0, span_elems = 0, tiles = 0;
@@ -287,11 +297,10 @@ int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, co
     return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
   *out = nullptr;
   GJ_HIP(hipSetDevice(ctx->device));
-  std::unique_ptr<gpjson_batch> b(new gpjson_batch());
+  // a batch that fails half-built releases what it holds
+  std::unique_ptr<gpjson_batch, void (*)(gpjson_batch*)> b(new gpjson_batch(), gpjson_batch_free);
   b->ctx = ctx;
   b->nfiles = nfiles;
-  b->itune = ctx->itune;
-  b->qtune = ctx->qtune;
   b->bufs.assign(bufs, bufs + nfiles);
   b->files.assign(nfiles, FileJob{});
   int j = 0;
@@ -302,6 +311,7 @@ int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, co
     b->file_first_job.push_back(j);
     int L = 0;
     for (int q = 0; q < nq_per_file[f]; q++, j++) {
+      if (!irs[j] || ir_lens[j] < 1) return fail(GPJSON_ERR_INVALID_ARG, "empty query program");
       int R = 1;
       int l = levels_of(irs[j], ir_lens[j], &R);
       L = std::max(L, l);
@@ -315,10 +325,20 @@ int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, co
   b->file_first_job.push_back(j);
   b->njobs = j;
   b->qjobs.assign(j, QueryJob{});
+  for (int k = 0; k < j; k++) {
+    b->ir_host.insert(b->ir_host.end(), irs[k], irs[k] + ir_lens[k]);
+    b->ir_lens_host.push_back(ir_lens[k]);
+  }
+  for (int f = 0; f < nfiles; f++) {
+    b->total_bytes += bufs[f]->n;
+    b->max_levels = std::max(b->max_levels, b->files[f].L);
+  }
+  // launch shapes: the tuning cache's for a batch of this size and depth, else the context's
+  shapes_for(ctx, b->total_bytes, b->max_levels, 1, &b->itune, &b->qtune);
   for (int k = 0; k < 4; k++) GJ_HIP(hipEventCreate(&b->ev[k]));
   std::vector<int64_t> caps(nfiles);
   for (int f = 0; f < nfiles; f++) caps[f] = bufs[f]->n / 256 + 4096;
-  int rc = layout(b.get(), caps, irs, ir_lens);
+  int rc = layout(b.get(), caps);
   if (rc) return rc;
   // first run sizes the newline arrays exactly (grows them if a file overflowed)
   rc = enqueue_batch(b.get(), nullptr, nullptr, nullptr);
@@ -328,7 +348,7 @@ int gpjson_batch_create(gpjson_ctx* ctx, gpjson_buf* const* bufs, int nfiles, co
     // the bookkeeping counts every file's lines even when its newline array
     // overflowed: lay out again with exact capacities
     for (int f = 0; f < nfiles; f++) caps[f] = std::max<int64_t>(b->lines[f], caps[f]);
-    rc = layout(b.get(), caps, irs, ir_lens);
+    rc = layout(b.get(), caps);
     if (!rc) rc = enqueue_batch(b.get(), nullptr, nullptr, nullptr);
     if (!rc) rc = read_back_lines(b.get(), &ov);
     if (!rc && ov) rc = fail(GPJSON_ERR_CAPACITY, "newline capacity still exceeded");
@@ -413,6 +433,130 @@ int gpjson_batch_results_per_line(const gpjson_batch* b, int file, int q) {
   const int j = b->file_first_job[file] + q;
   if (q < 0 || j >= b->file_first_job[file + 1]) return -1;
   return b->job_R[j];
+}
+
+// Times the batch's own runs (eager, with events: index + bookkeeping, then
+// plan + interpreter) under every index tile shape with the current
+// interpreter shape, then every interpreter shape with the fastest tile, and
+// keeps the fastest pair: the arenas are laid out again for it, the captured
+// graph (if any) is dropped so the next graph run captures the new launches,
+// and the pair goes into the context's tuning cache under (bucket of the
+// batch's total bytes, its deepest level, batch).  Each candidate is run once
+// untimed and `iterations` times timed.
+int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t report_cap) {
+  if (!b || iterations < 1) return fail(GPJSON_ERR_INVALID_ARG, "bad argument");
+  gpjson_ctx* ctx = b->ctx;
+  GJ_HIP(hipSetDevice(ctx->device));
+  GJ_HIP(hipStreamSynchronize(ctx->stream));
+  drop_graph(b);
+  b->timed_graph = false;
+  std::vector<int64_t> caps(b->nfiles);
+  for (int f = 0; f < b->nfiles; f++) caps[f] = b->files[f].nl_cap;
+  const IndexTune keep_i = b->itune;
+  const QueryTune keep_q = b->qtune;
+  // lays out for the current shapes and times them
+  auto time_shapes = [&](double* index_ms, double* query_ms) -> int {
+    int rc = layout(b, caps);
+    if (rc) return rc;
+    rc = enqueue_batch(b, nullptr, nullptr, nullptr);
+    double si = 0, sq = 0;
+    for (int it = 0; it < iterations && rc == GPJSON_OK; it++) {
+      rc = enqueue_batch(b, b->ev[0], b->ev[1], b->ev[2]);
+      if (rc) break;
+      GJ_HIP(hipEventSynchronize(b->ev[2]));
+      float a = 0, t = 0;
+      GJ_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+      GJ_HIP(hipEventElapsedTime(&t, b->ev[1], b->ev[2]));
+      si += a;
+      sq += t;
+    }
+    *index_ms = si / iterations;
+    *query_ms = sq / iterations;
+    return rc;
+  };
+  std::string rep;
+  char line[256];
+  double best = 1e30;
+  IndexTune best_i = keep_i;
+  int rc = GPJSON_OK;
+  for (const IndexTune& c : index_candidates()) {
+    b->itune = c;
+    double ims = 0, qms = 0;
+    rc = time_shapes(&ims, &qms);
+    if (rc) break;
+    std::snprintf(line, sizeof(line), "batch index block=%d wpt=%d tile=%lld B: %.4f ms\n", c.block, c.wpt,
+                  (long long)index_tile_bytes(c.block, c.wpt), ims);
+    rep += line;
+    if (ims < best) {
+      best = ims;
+      best_i = c;
+    }
+  }
+  b->itune = best_i;
+  double bestq = 1e30;
+  QueryTune best_q = keep_q;
+  // the batch runs no tiled launch, and the chain evaluator only when every
+  // job is a member chain and all programs fit its LDS: such candidates would
+  // time line mode again
+  bool chains = b->ir_total <= kIrLdsBytes;
+  for (int64_t j = 0, off = 0; j < b->njobs && chains; off += b->ir_lens_host[j], j++)
+    chains = chain_depth(b->ir_host.data() + off, b->ir_lens_host[j]) > 0;
+  if (rc == GPJSON_OK) {
+    for (const QueryTune& c : query_candidates(keep_q)) {
+      if (c.mode == kTiledMode || (c.mode == kChainMode && !chains)) continue;
+      b->qtune = c;
+      double ims = 0, qms = 0;
+      rc = time_shapes(&ims, &qms);
+      if (rc) break;
+      std::snprintf(line, sizeof(line),
+                    "batch query mode=%d block=%d group=%d waves_per_cu=%d lines_per_group=%d: %.4f ms\n", c.mode,
+                    c.block, c.group, c.waves_per_cu, c.lines_per_wave, qms);
+      rep += line;
+      if (qms < bestq) {
+        bestq = qms;
+        best_q = c;
+      }
+    }
+  }
+  if (rc) {
+    // back to the shapes the batch had, laid out as before
+    b->itune = keep_i;
+    b->qtune = keep_q;
+    (void)layout(b, caps);
+    return rc;
+  }
+  b->qtune = best_q;
+  rc = layout(b, caps);
+  if (!rc) rc = enqueue_batch(b, nullptr, nullptr, nullptr);  // the spans of the chosen shapes
+  if (rc) return rc;
+  GJ_HIP(hipStreamSynchronize(ctx->stream));
+  const TuneKey key{size_bucket(b->total_bytes), b->max_levels, 1};
+  ctx->tune_cache[key] = TuneEntry{best_i, best_q, best + bestq};
+  std::snprintf(line, sizeof(line),
+                "batch best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms); "
+                "cached for batches of [2^%d, 2^%d) bytes at %d levels\n",
+                best_i.block, best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
+                best_q.lines_per_wave, bestq, key.bucket, key.bucket + 1, b->max_levels);
+  rep += line;
+  if (report && report_cap) {
+    const size_t n = std::min(report_cap - 1, rep.size());
+    std::copy(rep.begin(), rep.begin() + n, report);
+    report[n] = 0;
+  }
+  return GPJSON_OK;
+}
+
+int gpjson_batch_get_config(const gpjson_batch* b, int* index_block, int* index_wpt, int* query_mode,
+                            int* query_block, int* query_group, int* query_waves_per_cu, int* lines_per_wave) {
+  if (!b) return fail(GPJSON_ERR_INVALID_ARG, "batch is NULL");
+  if (index_block) *index_block = b->itune.block;
+  if (index_wpt) *index_wpt = b->itune.wpt;
+  if (query_mode) *query_mode = b->qtune.mode;
+  if (query_block) *query_block = b->qtune.block;
+  if (query_group) *query_group = b->qtune.group;
+  if (query_waves_per_cu) *query_waves_per_cu = b->qtune.waves_per_cu;
+  if (lines_per_wave) *lines_per_wave = b->qtune.lines_per_wave;
+  return GPJSON_OK;
 }
 
 void gpjson_batch_free(gpjson_batch* b) {

@@ -271,7 +271,9 @@ bool fuse_spec_from_ir(const uint8_t* ir, int64_t len, FuseSpec* spec) {
 int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_kernel, const IndexTune* tune,
                   const FuseSpec* fuse) {
   gpjson_ctx* ctx = idx->ctx;
-  const IndexTune& t = tune ? *tune : ctx->itune;
+  IndexTune t;
+  if (tune) t = *tune;
+  else shapes_for(ctx, idx->n, idx->L, 0, &t, nullptr);
   IndexLaunch L{};
   int rc = index_setup(idx, t, &L);
   if (rc) return rc;
@@ -315,11 +317,13 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   std::memset(hbase, 0, jobs_off);
   QueryJob* hj = (QueryJob*)(hbase + jobs_off);
   int64_t off = (int64_t)(jobs_off + jobs_bytes);
+  QueryTune qt;
+  shapes_for(ctx, idx->n, idx->L, 0, nullptr, &qt);
   // Tiled mode needs the line count on the host (span offsets per query); the
   // line-based mode reads it on the device.
-  bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
+  bool chain = qt.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int q = 0; q < nq && chain; q++) chain = chain_depth(irs[q], ir_lens[q]) > 0;
-  const bool tiled = !chain && ctx->qtune.mode == kTiledMode && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
+  const bool tiled = !chain && qt.mode == kTiledMode && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
   for (int q = 0; q < nq; q++) {
     QueryJob j{};
     j.F = idx->buf->d;
@@ -352,11 +356,11 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.lines_dev = tiled ? nullptr : idx->lines_dev;
   L.total_dev = nullptr;
   L.counter = (unsigned long long*)dbase;
-  L.lines_per_wave = ctx->qtune.lines_per_wave;
+  L.lines_per_wave = qt.lines_per_wave;
   L.tiled = tiled ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
-  if (!tiled && !chain && ctx->qtune.mode == kLineLongMode && ctx->qtune.group != 64) {
+  if (!tiled && !chain && qt.mode == kLineLongMode && qt.group != 64) {
     L.long_count = (unsigned long long*)(dbase + long_off);
     L.long_items = (int64_t*)(dbase + long_off + 64);
     L.long_cap = long_cap;
@@ -365,11 +369,11 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
   out->chain = chain;
-  out->block = ctx->qtune.block;
-  out->group = ctx->qtune.group;
+  out->block = qt.block;
+  out->group = qt.group;
   out->levels = idx->L;
-  const int waves_per_block = ctx->qtune.block / 64;
-  out->grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
+  const int waves_per_block = qt.block / 64;
+  out->grid = std::max(1, ctx->num_cus * qt.waves_per_cu / waves_per_block);
   if (tiled) out->grid = (int)std::max<int64_t>(1, std::min<int64_t>(L.total_items, (int64_t)ctx->num_cus * 4));
   out->long_grid = ctx->num_cus * 2;
   return GPJSON_OK;
@@ -475,6 +479,9 @@ int gpjson_ctx_create(int device, gpjson_ctx** out) {
   ctx->device = device;
   GJ_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   GJ_HIP(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+  hipDeviceProp_t prop;
+  GJ_HIP(hipGetDeviceProperties(&prop, device));
+  ctx->arch = prop.gcnArchName;
   const int rc = noalloc_scratch_setup(device, &ctx->noalloc_scratch);
   if (rc) {
     (void)hipStreamDestroy(ctx->stream);
@@ -745,7 +752,8 @@ static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, 
     if (!ctx->copy_stream) GJ_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
     GJ_HIP(hipStreamSynchronize(ctx->stream));
     GJ_HIP(hipStreamSynchronize(ctx->copy_stream));
-    const IndexTune t = ctx->itune;
+    IndexTune t;
+    shapes_for(ctx, n, levels, 0, &t, nullptr);
     IndexLaunch L{};
     rc = index_setup(idx.get(), t, &L);
     if (rc) return rc;
@@ -1202,7 +1210,7 @@ int gpjson_plan_create(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* i
   int rc = gpjson_index_build(ctx, buf, L, 0, &p->idx);
   if (rc) return rc;
   rc = dev_alloc((void**)&p->spans, (size_t)nq * (size_t)std::max<int64_t>(p->idx->nl_cap, 1) * R * 2 * 8);
-  p->itune = ctx->itune;
+  shapes_for(ctx, buf->n, L, 0, &p->itune, nullptr);  // the shape the index was just built with
   if (rc == GPJSON_OK) rc = prepare_queries(ctx, p->idx, irs, ir_lens, nq, R, p->spans, &p->table, &p->pq);
   if (rc == GPJSON_OK && nq == 1 && ctx->qtune.fuse && !p->pq.chain && !p->pq.launch.tiled &&
       fuse_spec_from_ir(irs[0], ir_lens[0], &p->fuse)) {

@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gpjson.h"
@@ -97,6 +99,26 @@ struct QueryTune {
 
 int nlv_for_levels(int L);
 
+// Tuning cache: the launch shapes the autotuner measured fastest, keyed by the
+// input size's power-of-two bucket (floor(log2 n)), the level count and the
+// kind of run (0: a plain pass over one file, 1: a batch, n = its total bytes);
+// one context serves one device, so its architecture completes the key.
+struct TuneKey {
+  int bucket = 0, levels = 0, batch = 0;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(bucket, levels, batch) < std::tie(o.bucket, o.levels, o.batch);
+  }
+};
+struct TuneEntry {
+  IndexTune itune;
+  QueryTune qtune;
+  double ms = 0;  // the winning shapes' measured time per run
+};
+int size_bucket(int64_t n);
+// The autotuner's candidates (autotune.cpp).
+const std::vector<IndexTune>& index_candidates();
+std::vector<QueryTune> query_candidates(const QueryTune& base);
+
 // Grow-only device allocation owned by a context.
 struct DevBuf {
   void* p = nullptr;
@@ -125,6 +147,9 @@ struct gpjson_ctx {
   size_t host_scratch_size = 0;
   gj::IndexTune itune;
   gj::QueryTune qtune;
+  std::string arch;                               // gcnArchName of the device
+  std::map<gj::TuneKey, gj::TuneEntry> tune_cache;  // filled by the autotuners
+  bool tuning = false;                            // an autotuner is timing candidates: use itune/qtune
   void* noalloc_scratch = nullptr;  // this device's fixed scratch array (gpjson_query_noalloc)
 };
 
@@ -158,6 +183,10 @@ struct gpjson_index {
 };
 
 namespace gj {
+// The shapes for a run over n bytes at L levels (batch: 1 for a batch): the
+// tuning cache's entry when there is one, else the context's shapes; the
+// in-tile evaluation switch always comes from the context.
+void shapes_for(const gpjson_ctx* ctx, int64_t n, int L, int batch, IndexTune* it, QueryTune* qt);
 // Device job table + launch descriptor for nq queries over one index.
 struct PreparedQueries {
   QueryLaunch launch{};

@@ -265,8 +265,10 @@ int gpjson_query_noalloc(gpjson_ctx *ctx, const uint8_t *input, int64_t n, const
 
 /* Prepares a batch of nfiles resident buffers.  File f runs nq_per_file[f]
  * queries; their IRs are irs[k..k+nq_per_file[f]) in file order.  Each file's
- * index has the depth of its deepest query.  The first run happens here and
- * sizes every array; later runs allocate nothing. */
+ * index has the depth of its deepest query.  Launch shapes come from the
+ * context's tuning cache for (bucket of the batch's total bytes, its deepest
+ * level, batch), else from the context.  The first run happens here and sizes
+ * every array; later runs allocate nothing. */
 int gpjson_batch_create(gpjson_ctx *ctx, gpjson_buf *const *bufs, int nfiles, const uint8_t *const *irs,
                         const int64_t *ir_lens, const int *nq_per_file, gpjson_batch **out);
 /* Enqueues one run: one persistent index launch over the tiles of every file
@@ -280,6 +282,17 @@ int gpjson_batch_lines(gpjson_batch *batch, int file, int64_t *lines);
 int gpjson_batch_results_per_line(const gpjson_batch *batch, int file, int q);
 /* Waits, then copies int64[lines][R][2] spans of (file, query q) to host memory. */
 int gpjson_batch_spans(gpjson_batch *batch, int file, int q, int64_t *spans, int64_t cap_elems);
+/* Batch autotuner: times the batch's own runs under every index tile shape
+ * (the chunk of input a tile covers = block x wpt x 64 bytes) and every
+ * interpreter shape (mode, block, lanes per line, grid, lines per claim),
+ * installs the fastest pair in the batch (its arrays are laid out again, a
+ * captured graph is dropped and captured anew on the next graph run; the
+ * spans are those of a run with the new shapes) and records it in the tuning
+ * cache.  Allocates; waits.  report (optional) gets one line per candidate. */
+int gpjson_batch_autotune(gpjson_batch *batch, int iterations, char *report, size_t report_cap);
+/* The batch's launch shapes (the fields of gpjson_get_config). */
+int gpjson_batch_get_config(const gpjson_batch *batch, int *index_block, int *index_wpt, int *query_mode,
+                            int *query_block, int *query_group, int *query_waves_per_cu, int *lines_per_wave);
 void gpjson_batch_free(gpjson_batch *batch);
 
 /* ---- serving (no counterpart in the reference) ---------------------------- */
@@ -342,10 +355,24 @@ int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query
  * line from the full index. */
 int gpjson_set_fusion(gpjson_ctx *ctx, int enable);
 int gpjson_get_fusion(gpjson_ctx *ctx, int *enabled);
-/* Times every supported shape on buf with the given queries and installs the
- * fastest; report (optional) gets one text line per candidate. */
+/* Times every supported shape on buf with the given queries and records the
+ * fastest in the context's tuning cache under (floor(log2 of buf's size), the
+ * queries' depth, plain pass): every later plan, one-shot query and index
+ * build of an input in that size bucket at that depth uses it; the context's
+ * own shapes (gpjson_set_config) serve every other input.  report (optional)
+ * gets one text line per candidate.  No reference counterpart: the reference
+ * hard-codes its launch shapes (stages/CombinedIndex.java:11-14,
+ * functions/QueryFunction.java:229). */
 int gpjson_autotune(gpjson_ctx *ctx, gpjson_buf *buf, const uint8_t *const *irs, const int64_t *ir_lens, int nq,
                     int iterations, char *report, size_t report_cap);
+/* The tuning cache (no reference counterpart).  gpjson_set_config clears it.
+ * A saved cache is a text file with one line per entry, tagged with the
+ * device architecture; loading takes the entries of this context's
+ * architecture and rejects shapes this build does not support. */
+int gpjson_tune_cache_clear(gpjson_ctx *ctx);
+int64_t gpjson_tune_cache_size(const gpjson_ctx *ctx);
+int gpjson_tune_cache_save(const gpjson_ctx *ctx, const char *path);
+int gpjson_tune_cache_load(gpjson_ctx *ctx, const char *path, int *loaded);
 
 /* ---- measurement --------------------------------------------------------- */
 

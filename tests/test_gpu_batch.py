@@ -103,3 +103,80 @@ def _batch_equals_plain(ctx, files, qsets, oracle):
         idx = ctx.index(bufs[0], 2)
         idx.free()
     batch.free()
+
+
+def test_batch_autotune_and_tuning_cache(ctx, batch_inputs, tmp_path):
+    """The batch autotuner re-lays the batch out for every candidate shape and
+    keeps the fastest: the spans right after it, after a direct run and after
+    graph replays equal the oracle's.  Its choice goes into the tuning cache,
+    where a second batch of the same files finds it; the cache survives a save,
+    clear and load."""
+    files, qsets, oracle = batch_inputs
+    ctx.tune_cache_clear()
+    bufs = [ctx.stage(d) for _, d in files]
+    irs = [[c.ir for c in qs] for qs in qsets]
+    batch = ctx.batch(bufs, irs)
+    try:
+        print("[batch] autotune", flush=True)
+        report = batch.autotune(1)
+        assert "batch best index" in report
+        tuned = batch.get_config()
+        assert ctx.tune_cache_size() == 1
+
+        def check(tag):
+            for f in range(len(files)):
+                for q in range(len(qsets[f])):
+                    assert np.array_equal(batch.spans(f, q), oracle[f][q]), (tag, f, q, files[f][0], tuned)
+
+        check("after autotune")
+        batch.run(graph=False)
+        check("direct run")
+        for r in range(2):
+            batch.run(graph=True)
+            ctx.synchronize()
+            check(f"graph replay {r}")
+        second = ctx.batch(bufs, irs)
+        try:
+            assert second.get_config() == tuned
+        finally:
+            second.free()
+        path = str(tmp_path / "tune.txt")
+        ctx.tune_cache_save(path)
+        ctx.tune_cache_clear()
+        assert ctx.tune_cache_size() == 0
+        assert ctx.tune_cache_load(path) == 1
+        assert ctx.tune_cache_size() == 1
+    finally:
+        batch.free()
+        ctx.tune_cache_clear()
+        for b in bufs:
+            b.free()
+
+
+def test_plain_autotune_cache(ctx):
+    """gpjson_autotune records its choice for the input's size bucket and
+    depth; plans and one-shot queries that then run with it equal the oracle."""
+    from gpjson_amd import compile_path, datagen
+    from oracle import oracle as O
+    data = datagen.generate("tweets", 71, 3 << 20)
+    cp = compile_path("$.user.lang")
+    nl = O.newline_index(data.tobytes())
+    s = O.string_index(data.tobytes())
+    lb = O.leveled_index(data.tobytes(), s, cp.max_depth)
+    want = O.find_value(data.tobytes(), nl, s, lb, cp.max_depth, cp.ir, 1).reshape(nl.size, 1, 2)
+    ctx.tune_cache_clear()
+    buf = ctx.stage(data)
+    try:
+        report = ctx.autotune(buf, [cp.ir], 1)
+        assert "cached for inputs of [2^21, 2^22) bytes at 2 levels" in report, report[-300:]
+        assert ctx.tune_cache_size() == 1
+        plan = ctx.plan(buf, [cp.ir])
+        plan.run()
+        assert np.array_equal(plan.spans()[0], want)
+        plan.free()
+        idx = ctx.index(buf, cp.max_depth)
+        assert np.array_equal(ctx.run_queries(idx, [cp.ir], 1)[0], want)
+        idx.free()
+    finally:
+        ctx.tune_cache_clear()
+        buf.free()
