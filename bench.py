@@ -278,9 +278,11 @@ def batch_sizes(n: int, total: int, seed: int, lo: int = 1 << 20, hi: int = 512 
 def batch_section(args) -> dict:
     """256 files of uneven size (tweets and adversarial mixed), 8 member-chain queries
     per file: one batched run (persistent index launch + one interpreter launch,
-    replayed as a hipGraph) against 256 sequential plain runs (index + query per
-    file).  Every (file, query) is compared with the plain path bit for bit, and a
-    sample of files with the multithreaded oracle."""
+    replayed as a hipGraph, launch shapes from the batch autotuner) against 256
+    sequential plain runs (one index + one launch of the file's 8 queries per
+    file), and against the drop-in's one-query passes (index + query per
+    (file, query) pair, 2048 passes).  Every (file, query) is compared with the
+    plain path bit for bit, and a sample of files with the multithreaded oracle."""
     import numpy as np
 
     import gpjson_amd
@@ -321,7 +323,7 @@ def batch_section(args) -> dict:
         for line in report.strip().splitlines():
             log("autotune: " + line)
         log(f"autotune on file {big} took {time.perf_counter() - t:.1f} s")
-    shapes = ctx.get_config()
+    shapes = {**ctx.get_config(), "query_order": ctx.get_query_order()}
     log(f"launch shapes: {shapes}")
 
     # plain: one index + one interpreter launch (8 queries) per file, file after file
@@ -337,9 +339,27 @@ def batch_section(args) -> dict:
     plain_dt = (time.perf_counter() - t0) / args.batch_steps
     plain_gbps = nbytes / plain_dt / 1e9
     log(f"plain: {plain_dt * 1e3:.2f} ms per pass over all files, {plain_gbps:.1f} GB/s")
+    # the drop-in's own plain path, one file and one query per index + query
+    # pass: 8 passes per file answer what one pass above does (plans are made
+    # outside the timed runs, a file's 8 at a time)
+    one_dt = 0.0
+    for b, q in zip(bufs, irs):
+        ps = [ctx.plan(b, [ir]) for ir in q]
+        for p in ps:
+            p.run()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for p in ps:
+            p.run()
+        ctx.synchronize()
+        one_dt += time.perf_counter() - t0
+        for p in ps:
+            p.free()
+    log(f"plain, one query per pass: {one_dt * 1e3:.2f} ms for every (file, query) pair "
+        f"({sum(len(q) for q in irs)} passes)")
 
     batch = ctx.batch(bufs, irs)
-    batch_shapes = batch.get_config()
+    batch_shapes = {**batch.get_config(), "query_order": batch.query_order()}
     tune_s = None
     if not args.no_batch_autotune:
         # the batch's own shapes (tile = chunk of input per work item, block,
@@ -351,7 +371,7 @@ def batch_section(args) -> dict:
         for line in lines[:-1]:
             log("batch autotune: " + line)
         log(f"{lines[-1]} ({tune_s:.1f} s)")
-        batch_shapes = batch.get_config()
+        batch_shapes = {**batch.get_config(), "query_order": batch.query_order()}
     for _ in range(2):
         batch.run(graph=True)
     ctx.synchronize()
@@ -367,7 +387,8 @@ def batch_section(args) -> dict:
     # the plain runs again with the batch's tuned shapes, for a like-for-like ratio
     plain_same = None
     if batch_shapes != shapes:
-        ctx.set_config(*batch_shapes.values())
+        ctx.set_config(*list(batch_shapes.values())[:7])
+        ctx.set_query_order(batch_shapes["query_order"])
         tuned_plans = [ctx.plan(b, q) for b, q in zip(bufs, irs)]
         for p in tuned_plans:
             p.run()
@@ -380,7 +401,8 @@ def batch_section(args) -> dict:
         plain_same = nbytes / ((time.perf_counter() - t0) / args.batch_steps) / 1e9
         for p in tuned_plans:
             p.free()
-        ctx.set_config(*shapes.values())
+        ctx.set_config(*list(shapes.values())[:7])
+        ctx.set_query_order(shapes["query_order"])
         log(f"plain with the batch's shapes: {plain_same:.1f} GB/s")
 
     # bit-exact: every (file, query) of the batch equals the plain plan
@@ -419,6 +441,8 @@ def batch_section(args) -> dict:
         "plain_GBps": round(plain_gbps, 3),
         "plain_ms_per_pass": round(plain_dt * 1e3, 3),
         "batched_over_plain": round(value / plain_gbps, 3),
+        "plain_one_query_ms_per_pass": round(one_dt * 1e3, 3),
+        "batched_over_plain_one_query": round(one_dt / batch_dt, 3),
         "plain_GBps_batch_shapes": round(plain_same, 3) if plain_same else round(plain_gbps, 3),
         "batched_over_plain_batch_shapes": round(value / (plain_same or plain_gbps), 3),
         "batch_launch_shapes": batch_shapes,

@@ -103,6 +103,9 @@ SIGNATURES = {
     "gpjson_batch_spans": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64]),
     "gpjson_batch_free": (None, [c_void_p]),
     "gpjson_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
+    "gpjson_batch_query_order": (c_int, [c_void_p]),
+    "gpjson_set_query_order": (c_int, [c_void_p, c_int]),
+    "gpjson_get_query_order": (c_int, [c_void_p, P(c_int)]),
     "gpjson_batch_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int),
                                         P(c_int)]),
     "gpjson_server_create": (c_int, [c_int, c_int64, c_int, P(c_void_p)]),

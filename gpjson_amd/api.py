@@ -253,6 +253,16 @@ class Context:
         _check(self._lib.gpjson_autotune(self._h, buf.handle, ptrs, lens, nq, iterations, rep, 16384))
         return rep.value.decode()
 
+    def set_query_order(self, qblock: int) -> None:
+        """0: several queries over one file run query after query; B > 0: they take
+        turns over blocks of B lines.  Clears the tuning cache."""
+        _check(self._lib.gpjson_set_query_order(self._h, qblock))
+
+    def get_query_order(self) -> int:
+        v = ctypes.c_int()
+        _check(self._lib.gpjson_get_query_order(self._h, ctypes.byref(v)))
+        return v.value
+
     def tune_cache_size(self) -> int:
         return int(self._lib.gpjson_tune_cache_size(self._h))
 
@@ -360,6 +370,9 @@ class Batch:
         keys = ["index_block", "index_wpt", "query_mode", "query_block", "query_group", "query_waves_per_cu",
                 "lines_per_group"]
         return {k: x.value for k, x in zip(keys, v)}
+
+    def query_order(self) -> int:
+        return int(N.load().gpjson_batch_query_order(self._h))
 
     def timing(self):
         a, t = ctypes.c_double(), ctypes.c_double()

@@ -42,6 +42,7 @@ std::vector<QueryTune> query_candidates(const QueryTune& base) {
     q.group = group;
     q.waves_per_cu = wpc;
     q.lines_per_wave = lpw;
+    q.qblock = 0;  // query after query; the block orders are tried on the best shape
     c.push_back(q);
   };
   for (int group : {1, 4, 16})
@@ -140,6 +141,22 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
   return GPJSON_OK;
 }
 
+int gpjson_set_query_order(gpjson_ctx* ctx, int qblock) {
+  if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
+  if (qblock < 0 || qblock > (1 << 20)) return fail(GPJSON_ERR_INVALID_ARG, "qblock must be in [0, 2^20]");
+  GJ_HIP(hipSetDevice(ctx->device));
+  GJ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->qtune.qblock = qblock;
+  ctx->tune_cache.clear();  // shapes set by hand apply to every input
+  return GPJSON_OK;
+}
+
+int gpjson_get_query_order(gpjson_ctx* ctx, int* qblock) {
+  if (!ctx || !qblock) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
+  *qblock = ctx->qtune.qblock;
+  return GPJSON_OK;
+}
+
 int gpjson_set_fusion(gpjson_ctx* ctx, int enable) {
   if (!ctx) return fail(GPJSON_ERR_INVALID_ARG, "ctx is NULL");
   ctx->qtune.fuse = enable ? 1 : 0;
@@ -217,13 +234,31 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
       best_q = c;
     }
   }
+  if (nq > 1 && best_q.mode != kChainMode && best_q.mode != kTiledMode) {
+    // several queries: their work order (taking turns over blocks of lines)
+    for (int qb : {16, 64, 256, 1024, 4096}) {
+      QueryTune c = best_q;
+      c.qblock = qb;
+      ctx->qtune = c;
+      double ims = 0, qms = 0;
+      const int rc = time_plan(&ims, &qms);
+      if (rc) return rc;
+      std::snprintf(line, sizeof(line), "query (best shape) queries taking turns over blocks of %d lines: %.4f ms\n",
+                    qb, qms);
+      rep += line;
+      if (qms < bestq) {
+        bestq = qms;
+        best_q = c;
+      }
+    }
+  }
   const TuneKey key{size_bucket(buf->n), L, 0};
   ctx->tune_cache[key] = TuneEntry{best_i, best_q, best + bestq};
   std::snprintf(line, sizeof(line),
-                "best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms); "
+                "best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d qblock %d (%.4f ms); "
                 "cached for inputs of [2^%d, 2^%d) bytes at %d levels\n",
                 best_i.block, best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
-                best_q.lines_per_wave, bestq, key.bucket, key.bucket + 1, L);
+                best_q.lines_per_wave, best_q.qblock, bestq, key.bucket, key.bucket + 1, L);
   rep += line;
   copy_report(rep, report, report_cap);
   return GPJSON_OK;
@@ -239,18 +274,18 @@ int64_t gpjson_tune_cache_size(const gpjson_ctx* ctx) { return ctx ? (int64_t)ct
 
 // Text format: a header line, then one line per entry:
 //   arch bucket levels batch index_block index_wpt query_mode query_block
-//   query_group waves_per_cu lines_per_wave ms
+//   query_group waves_per_cu lines_per_wave qblock ms
 int gpjson_tune_cache_save(const gpjson_ctx* ctx, const char* path) {
   if (!ctx || !path) return fail(GPJSON_ERR_INVALID_ARG, "NULL argument");
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(GPJSON_ERR_IO, std::string("Failed to open file ") + path);
-  std::fprintf(f, "gpjson-tune-cache 1\n");
+  std::fprintf(f, "gpjson-tune-cache 2\n");
   for (const auto& kv : ctx->tune_cache) {
     const TuneKey& k = kv.first;
     const TuneEntry& e = kv.second;
-    std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n", ctx->arch.c_str(), k.bucket, k.levels, k.batch,
+    std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %.6f\n", ctx->arch.c_str(), k.bucket, k.levels, k.batch,
                  e.itune.block, e.itune.wpt, e.qtune.mode, e.qtune.block, e.qtune.group, e.qtune.waves_per_cu,
-                 e.qtune.lines_per_wave, e.ms);
+                 e.qtune.lines_per_wave, e.qtune.qblock, e.ms);
   }
   const bool ok = std::fflush(f) == 0;
   std::fclose(f);
@@ -266,22 +301,23 @@ int gpjson_tune_cache_load(gpjson_ctx* ctx, const char* path, int* loaded) {
   if (!f) return fail(GPJSON_ERR_IO, std::string("Failed to open file ") + path);
   char buf[512];
   int n = 0, rc = GPJSON_OK;
-  if (!std::fgets(buf, sizeof(buf), f) || std::strncmp(buf, "gpjson-tune-cache 1", 19) != 0)
+  if (!std::fgets(buf, sizeof(buf), f) || std::strncmp(buf, "gpjson-tune-cache 2", 19) != 0)
     rc = fail(GPJSON_ERR_INVALID_ARG, std::string("not a tuning cache file: ") + path);
   while (rc == GPJSON_OK && std::fgets(buf, sizeof(buf), f)) {
     char arch[128];
     TuneKey k;
     TuneEntry e;
-    if (std::sscanf(buf, "%127s %d %d %d %d %d %d %d %d %d %d %lf", arch, &k.bucket, &k.levels, &k.batch,
+    if (std::sscanf(buf, "%127s %d %d %d %d %d %d %d %d %d %d %d %lf", arch, &k.bucket, &k.levels, &k.batch,
                     &e.itune.block, &e.itune.wpt, &e.qtune.mode, &e.qtune.block, &e.qtune.group,
-                    &e.qtune.waves_per_cu, &e.qtune.lines_per_wave, &e.ms) != 12) {
+                    &e.qtune.waves_per_cu, &e.qtune.lines_per_wave, &e.qtune.qblock, &e.ms) != 13) {
       rc = fail(GPJSON_ERR_INVALID_ARG, std::string("malformed tuning cache line: ") + buf);
       break;
     }
     if (ctx->arch != arch) continue;
     if (!index_config_supported(e.itune.block, e.itune.wpt) || !query_config_supported(e.qtune.block, e.qtune.group) ||
         e.qtune.mode < kLineMode || e.qtune.mode > kLineLongMode || e.qtune.waves_per_cu < 1 ||
-        e.qtune.waves_per_cu > 64 || e.qtune.lines_per_wave < 1 || e.qtune.lines_per_wave > 1024 || k.levels < 0 ||
+        e.qtune.waves_per_cu > 64 || e.qtune.lines_per_wave < 1 || e.qtune.lines_per_wave > 1024 ||
+        e.qtune.qblock < 0 || e.qtune.qblock > (1 << 20) || k.levels < 0 ||
         k.levels > GPJSON_MAX_LEVELS || k.bucket < 0 || k.bucket > 62 || (k.batch != 0 && k.batch != 1)) {
       rc = fail(GPJSON_ERR_INVALID_ARG, std::string("unsupported shapes in tuning cache line: ") + buf);
       break;

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 
@@ -49,7 +50,7 @@ struct gpjson_batch {
   int64_t total_tiles = 0;
   int64_t total_bytes = 0;
   int max_levels = 0;
-  int64_t ir_total = 0;  // IR bytes of every job, contiguous in dev_ir
+  int64_t ir_total = 0;  // bytes of the distinct programs, contiguous in dev_ir
   std::vector<uint8_t> ir_host;    // the same bytes on the host (a new layout uploads them again)
   std::vector<int64_t> ir_lens_host;
   bool chain = false;    // every job is a member chain: chain evaluator
@@ -160,20 +161,26 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
     J.E = nullptr;
     J.Q = nullptr;
   }
-  // IR bytes
+  // IR bytes, one copy per distinct program (files sharing a query set share
+  // its programs), so a batch's programs usually fit the interpreter's LDS
+  std::map<std::string, int64_t> ir_at;
+  std::vector<int64_t> ir_off(b->njobs);
   int64_t ir_total = 0;
-  for (int j = 0; j < b->njobs; j++) ir_total += ir_lens[j];
+  for (int j = 0; j < b->njobs; j++) {
+    const auto ins = ir_at.emplace(std::string((const char*)irs[j], (size_t)ir_lens[j]), ir_total);
+    if (ins.second) ir_total += ir_lens[j];
+    ir_off[j] = ins.first->second;
+  }
   b->ir_total = ir_total;
   b->chain = b->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
   for (int j = 0; j < b->njobs && b->chain; j++) b->chain = chain_depth(irs[j], ir_lens[j]) > 0;
   rc = b->dev_ir.ensure((size_t)ir_total + 16);
   if (rc) return rc;
   std::vector<uint8_t> flat((size_t)ir_total + 16);
-  int64_t off = 0;
+  for (const auto& kv : ir_at) std::memcpy(flat.data() + kv.second, kv.first.data(), kv.first.size());
   for (int j = 0; j < b->njobs; j++) {
     QueryJob& q = b->qjobs[j];
     const FileJob& J = b->files[b->job_file[j]];
-    std::memcpy(flat.data() + off, irs[j], (size_t)ir_lens[j]);
     q.F = J.F;
     q.N = J.N;
     q.NL = J.NL;
@@ -185,11 +192,12 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
     q.Wp = J.Wp;
     q.L = J.L;
     q.R = b->job_R[j];
-    q.ir = (const uint8_t*)b->dev_ir.p + off;
+    q.ir = (const uint8_t*)b->dev_ir.p + ir_off[j];
     q.ir_len = ir_lens[j];
     q.out = (int64_t*)b->arena_spans.p + b->job_span_off[j];
     q.item0 = 0;
-    off += ir_lens[j];
+    q.qf = b->job_q[j];
+    q.nqf = b->file_first_job[b->job_file[j] + 1] - b->file_first_job[b->job_file[j]];
   }
   uint8_t* misc = (uint8_t*)b->dev_misc.p;
   b->total_dev = (int64_t*)misc;
@@ -258,6 +266,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
+  Q.qblock = b->chain ? 0 : b->qtune.qblock;
   const bool long_lines = !b->chain && b->qtune.mode == kLineLongMode && b->qtune.group != 64;
   if (long_lines) {
     Q.long_count = (unsigned long long*)b->arena_long.p;
@@ -518,6 +527,24 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
       }
     }
   }
+  // the work order: the queries of a file taking turns over blocks of lines
+  if (rc == GPJSON_OK && !(best_q.mode == kChainMode && chains)) {
+    for (int qb : {16, 64, 256, 1024, 4096}) {
+      QueryTune c = best_q;
+      c.qblock = qb;
+      b->qtune = c;
+      double ims = 0, qms = 0;
+      rc = time_shapes(&ims, &qms);
+      if (rc) break;
+      std::snprintf(line, sizeof(line), "batch query (best shape) queries taking turns over blocks of %d lines: %.4f ms\n",
+                    qb, qms);
+      rep += line;
+      if (qms < bestq) {
+        bestq = qms;
+        best_q = c;
+      }
+    }
+  }
   if (rc) {
     // back to the shapes the batch had, laid out as before
     b->itune = keep_i;
@@ -533,10 +560,10 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
   const TuneKey key{size_bucket(b->total_bytes), b->max_levels, 1};
   ctx->tune_cache[key] = TuneEntry{best_i, best_q, best + bestq};
   std::snprintf(line, sizeof(line),
-                "batch best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d (%.4f ms); "
-                "cached for batches of [2^%d, 2^%d) bytes at %d levels\n",
+                "batch best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d qblock %d "
+                "(%.4f ms); cached for batches of [2^%d, 2^%d) bytes at %d levels\n",
                 best_i.block, best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
-                best_q.lines_per_wave, bestq, key.bucket, key.bucket + 1, b->max_levels);
+                best_q.lines_per_wave, best_q.qblock, bestq, key.bucket, key.bucket + 1, b->max_levels);
   rep += line;
   if (report && report_cap) {
     const size_t n = std::min(report_cap - 1, rep.size());
@@ -545,6 +572,8 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
   }
   return GPJSON_OK;
 }
+
+int gpjson_batch_query_order(const gpjson_batch* b) { return b ? b->qtune.qblock : -1; }
 
 int gpjson_batch_get_config(const gpjson_batch* b, int* index_block, int* index_wpt, int* query_mode,
                             int* query_block, int* query_group, int* query_waves_per_cu, int* lines_per_wave) {

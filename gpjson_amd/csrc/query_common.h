@@ -39,6 +39,10 @@ struct QueryJob {
   const int64_t* tile_lines;
   int64_t ntiles;
   int64_t tile_bytes;
+  // batched launches (QueryLaunch.qblock): this job is query qf of the nqf
+  // queries of its file, whose jobs are consecutive and share the file's lines
+  int qf;
+  int nqf;
 };
 
 struct QueryLaunch {
@@ -77,7 +81,33 @@ struct QueryLaunch {
   // kIrLdsBytes, each block copies it to LDS once and interprets from there.
   const uint8_t* ir_base;
   int64_t ir_bytes;
+  // Work order of several queries over one file's lines (line mode).  0: query
+  // after query (items q * lines + i).  B > 0: the queries take turns over
+  // blocks of B lines (items of block c: query 0 on lines [cB, cB + B), then
+  // query 1 on the same lines, ...; the last block holds the lines left), so
+  // the queries of a line run close together and share its cached words.
+  int qblock;
 };
+
+// Item r of a file's nq * lines work items when its queries take turns over
+// blocks of B lines: the query q and line i it stands for.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline void split_item(int64_t r, int64_t lines, int64_t nq, int64_t B, int64_t* q, int64_t* i) {
+  const int64_t full = lines / B;  // whole blocks
+  const int64_t span = nq * B;     // items of a whole block
+  if (r < full * span) {
+    const int64_t c = r / span, w = r - c * span;
+    *q = w / B;
+    *i = c * B + (w - *q * B);
+  } else {
+    const int64_t t = lines - full * B;  // lines of the last block (> 0 when items are left)
+    const int64_t w = r - full * span;
+    *q = w / t;
+    *i = full * B + (w - *q * t);
+  }
+}
 
 constexpr int kIrLdsBytes = 4096;
 // Lines at least this long are evaluated by a whole wavefront (line mode).

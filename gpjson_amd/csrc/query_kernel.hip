@@ -545,13 +545,23 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
       continue;
     }
     // several jobs: each lane keeps its current job in registers and reloads
-    // it only when its lines cross into the next job (lines of a claim are
+    // it only when its items cross into another job (items of a claim are
     // consecutive), so a line's walk does not start with a load of its job
     if (launch.lines_dev) {
-      int64_t qcur = (int64_t)g0 / common;
+      auto split = [&](int64_t it, int64_t& q, int64_t& i) {
+        if (launch.qblock > 0) {
+          split_item(it, common, launch.njobs, launch.qblock, &q, &i);
+        } else {
+          q = it / common;
+          i = it - q * common;
+        }
+      };
+      int64_t qcur, i0;
+      split((int64_t)g0, qcur, i0);
       QueryJob J = launch.jobs[qcur];
       for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-        const int64_t q = it / common, i = it - q * common;
+        int64_t q, i;
+        split(it, q, i);
         if (q != qcur) {
           qcur = q;
           J = launch.jobs[q];
@@ -568,6 +578,31 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
       if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
     }
     QueryJob J = launch.jobs[lo];
+    if (launch.qblock > 0 && J.nqf > 0) {
+      // the queries of a file take turns over blocks of its lines: the file's
+      // items are [base, base + nq * lines), and its jobs j0 .. j0 + nq - 1
+      int64_t j0 = lo - J.qf, nq = J.nqf, lines = J.lines;
+      int64_t base = launch.jobs[j0].item0, jcur = lo;
+      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+        while (it >= base + nq * lines && j0 + nq < launch.njobs) {  // into the next file
+          j0 += nq;
+          const QueryJob& F0 = launch.jobs[j0];
+          nq = F0.nqf;
+          lines = F0.lines;
+          base = F0.item0;
+        }
+        int64_t q, i;
+        split_item(it - base, lines, nq, launch.qblock, &q, &i);
+        if (j0 + q != jcur) {
+          jcur = j0 + q;
+          J = launch.jobs[jcur];
+        }
+        if (defer(J, lines, i, it)) continue;
+        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), lines, i,
+                  J.out + i * (int64_t)J.R * 2, le, ci, nslots);
+      }
+      continue;
+    }
     int64_t next0 = lo + 1 < launch.njobs ? launch.jobs[lo + 1].item0 : INT64_MAX;  // first item of the next job
     for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
       if (it >= next0) {
@@ -610,7 +645,13 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
     if (k >= count) break;
     const int64_t it = launch.long_items[k];
     if (launch.lines_dev) {
-      const int64_t q = it / common, i = it - q * common;
+      int64_t q, i;
+      if (launch.qblock > 0) {
+        split_item(it, common, launch.njobs, launch.qblock, &q, &i);
+      } else {
+        q = it / common;
+        i = it - q * common;
+      }
       const QueryJob& J = launch.jobs[q];
       int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
       eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
@@ -620,8 +661,14 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
         const int mid = (lo + hi + 1) >> 1;
         if (launch.jobs[mid].item0 <= it) lo = mid; else hi = mid - 1;
       }
+      int64_t i = it - launch.jobs[lo].item0;
+      if (launch.qblock > 0 && launch.jobs[lo].nqf > 0) {  // the same split as the first launch
+        const int64_t j0 = lo - launch.jobs[lo].qf;
+        int64_t q;
+        split_item(it - launch.jobs[j0].item0, launch.jobs[lo].lines, launch.jobs[lo].nqf, launch.qblock, &q, &i);
+        lo = (int)(j0 + q);
+      }
       const QueryJob& J = launch.jobs[lo];
-      const int64_t i = it - J.item0;
       eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
                 nslots);
     }

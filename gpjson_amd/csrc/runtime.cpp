@@ -360,6 +360,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.tiled = tiled ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
+  L.qblock = (nq > 1 && !tiled && !chain) ? qt.qblock : 0;
   if (!tiled && !chain && qt.mode == kLineLongMode && qt.group != 64) {
     L.long_count = (unsigned long long*)(dbase + long_off);
     L.long_items = (int64_t*)(dbase + long_off + 64);

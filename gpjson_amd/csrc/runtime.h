@@ -90,6 +90,7 @@ struct QueryTune {
   int group = 1;          // lanes per line
   int waves_per_cu = 16;  // persistent grid size factor
   int lines_per_wave = 4; // lines each group takes per wavefront claim (at most)
+  int qblock = 0;         // several queries: they take turns over blocks of this many lines (0: query after query)
   // A prepared plain pass of one member-chain query evaluates the lines inside
   // each index tile in the index kernel and interprets only the others.  Off
   // by default: on 4 GiB of tweets the fused index kernel takes 2.6-3.3 ms

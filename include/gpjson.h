@@ -290,6 +290,8 @@ int gpjson_batch_spans(gpjson_batch *batch, int file, int q, int64_t *spans, int
  * spans are those of a run with the new shapes) and records it in the tuning
  * cache.  Allocates; waits.  report (optional) gets one line per candidate. */
 int gpjson_batch_autotune(gpjson_batch *batch, int iterations, char *report, size_t report_cap);
+/* The batch's work order (gpjson_set_query_order), -1 for a NULL batch. */
+int gpjson_batch_query_order(const gpjson_batch *batch);
 /* The batch's launch shapes (the fields of gpjson_get_config). */
 int gpjson_batch_get_config(const gpjson_batch *batch, int *index_block, int *index_wpt, int *query_mode,
                             int *query_block, int *query_group, int *query_waves_per_cu, int *lines_per_wave);
@@ -353,6 +355,15 @@ int gpjson_set_config(gpjson_ctx *ctx, int index_block, int index_wpt, int query
  * into its neighbours; the interpreter then runs on those lines alone.  Spans
  * are the same either way.  Plans prepared while it is off interpret every
  * line from the full index. */
+/* Work order of a launch with several queries over one file's lines (no
+ * reference counterpart; the reference runs one find_value launch per query,
+ * functions/QueryFunction.java:201-241): qblock 0 runs query after query over
+ * all lines; qblock B > 0 lets the queries take turns over blocks of B lines,
+ * so the queries of a line run close together and share its cached index
+ * words.  Results are the same either way.  Setting it clears the tuning
+ * cache, as gpjson_set_config does. */
+int gpjson_set_query_order(gpjson_ctx *ctx, int qblock);
+int gpjson_get_query_order(gpjson_ctx *ctx, int *qblock);
 int gpjson_set_fusion(gpjson_ctx *ctx, int enable);
 int gpjson_get_fusion(gpjson_ctx *ctx, int *enabled);
 /* Times every supported shape on buf with the given queries and records the

@@ -34,6 +34,9 @@ def _files(seed, nfiles):
     out.insert(3, ("soup", soup))
     out.insert(7, ("short", b"".join(b'{"id":%d}\n' % (i % 10) for i in range(33_000))))
     out.insert(9, ("empty", b""))
+    # lines of 9-40 KB (mode 3 hands them to a wavefront each)
+    out.append(("long", b"".join(b'{"id":%d,"pad":"%s","user":{"lang":"l%d","name":"n"},"lang":"x"}\n'
+                                 % (k, b"y" * (9000 + 997 * k), k % 7) for k in range(32))))
     return out
 
 
@@ -55,15 +58,19 @@ def batch_inputs():
     return files, qsets, ref
 
 
-@pytest.mark.parametrize("mode", [0, 3])
-def test_batch_equals_plain(ctx, mode, batch_inputs):
+@pytest.mark.parametrize("mode,qblock", [(0, 0), (3, 0), (0, 8), (3, 4)])
+def test_batch_equals_plain(ctx, mode, qblock, batch_inputs):
+    """qblock > 0: the queries of a file take turns over blocks of lines, in the
+    batch and in the plain multi-query launches alike."""
     cfg = ctx.get_config()
     try:
         ctx.set_config(cfg["index_block"], cfg["index_wpt"], mode, cfg["query_block"], cfg["query_group"],
                        cfg["query_waves_per_cu"], cfg["lines_per_group"])
+        ctx.set_query_order(qblock)
         _batch_equals_plain(ctx, *batch_inputs)
     finally:
         ctx.set_config(*cfg.values())
+        ctx.set_query_order(0)
 
 
 def _batch_equals_plain(ctx, files, qsets, oracle):
@@ -77,6 +84,12 @@ def _batch_equals_plain(ctx, files, qsets, oracle):
         idx = ctx.index(buf, L)
         plain.append([ctx.run_queries(idx, [c.ir], c.num_results)[0] for c in qs])
         plain_lines.append(idx.lines)
+        # all of the file's queries in one plain launch (the work order applies)
+        R = max(c.num_results for c in qs)
+        together = ctx.run_queries(idx, [c.ir for c in qs], R)
+        for q, c in enumerate(qs):
+            assert np.array_equal(together[q][:, :c.num_results], oracle[len(plain) - 1][q]), (
+                "plain multi-query vs oracle", len(plain) - 1, q, k)
         idx.free()
 
     for f in range(len(files)):
