@@ -459,6 +459,8 @@ def run_batch(args) -> int:
     r = batch_section(args)
     result = common_fields(args, r["GBps"], r["ms_per_run"] / 1e3 * args.batch_steps)
     result["steps"] = args.batch_steps
+    result["warmup"] = 2
+    result["ms_per_step"] = r["ms_per_run"]
     result.update({
         "data": f"synthetic seeded tweets + adversarial NDJSON (seed {args.seed}), resident in HBM",
         "config": {"workload": "batched path: " + r["workload"], "input_bytes": r["input_bytes"],
@@ -604,6 +606,9 @@ def run_serve(args) -> int:
     r = serve_section(args)
     value = r["capacity_GBps"]
     result = common_fields(args, value, args.serve_burst / r["capacity_requests_per_s"])
+    result["steps"] = 1  # one timed closed burst (after one untimed burst)
+    result["warmup"] = 1
+    result["ms_per_step"] = round(args.serve_burst / r["capacity_requests_per_s"] * 1e3, 4)
     result.update({
         "data": f"synthetic seeded tweets NDJSON (seed {args.seed}), registered in host memory, resident in HBM",
         "config": {"workload": "serving: " + r["workload"] + "; value = closed-burst capacity in input GB/s",
