@@ -228,8 +228,8 @@ class Context:
 
     def set_config(self, index_block: int, index_wpt: int, query_mode: int, query_block: int, query_group: int,
                    query_waves_per_cu: int, lines_per_group: int) -> None:
-        """query_mode: 0 line (lane group per line), 1 tiled, 2 member-chain evaluator (else line),
-        3 line with lines of 8 KB or more handed to a wavefront each."""
+        """query_mode: 0 line (lane group per line), 3 line with lines of 8 KB or more handed to a
+        wavefront each.  Clears the tuning cache."""
         _check(self._lib.gpjson_set_config(self._h, index_block, index_wpt, query_mode, query_block, query_group,
                                            query_waves_per_cu, lines_per_group))
 

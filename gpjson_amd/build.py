@@ -26,7 +26,6 @@ SOURCES = [
     "server.cpp",
     "autotune.cpp",
     "diag.hip",
-    "chain_kernel.hip",
     "gather.hip",
     "noalloc.hip",
 ]

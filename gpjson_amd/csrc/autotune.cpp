@@ -30,9 +30,8 @@ const std::vector<IndexTune>& index_candidates() {
   return c;
 }
 
-// Interpreter shapes: line mode over lane groups and claim sizes, long lines
-// to a wavefront each, and the member-chain evaluator (used only when every
-// query of a launch is a member chain).
+// Interpreter shapes: line mode over lane groups, grid sizes and claim sizes,
+// and long lines to a wavefront each.
 std::vector<QueryTune> query_candidates(const QueryTune& base) {
   std::vector<QueryTune> c;
   auto add = [&](int mode, int block, int group, int wpc, int lpw) {
@@ -45,15 +44,11 @@ std::vector<QueryTune> query_candidates(const QueryTune& base) {
     q.qblock = 0;  // query after query; the block orders are tried on the best shape
     c.push_back(q);
   };
-  for (int group : {1, 4, 16})
-    for (int block : {128, 256}) add(kTiledMode, block, group, base.waves_per_cu, 1);
   for (int group : {1, 4, 16, 64})
     for (int wpc : {16, 32})
       for (int lpw : {1, 4}) add(kLineMode, 256, group, wpc, lpw);
   for (int group : {1, 4})
     for (int wpc : {16, 32}) add(kLineLongMode, 256, group, wpc, group == 1 ? 4 : 1);
-  for (int group : {1, 4, 16})
-    for (int wpc : {8, 16, 32}) add(kChainMode, 256, group, wpc, 1);
   return c;
 }
 
@@ -126,8 +121,8 @@ int gpjson_set_config(gpjson_ctx* ctx, int index_block, int index_wpt, int query
     return fail(GPJSON_ERR_INVALID_ARG, "query block must be 64, 128 or 256 and group 1, 4, 16 or 64");
   if (query_waves_per_cu < 1 || query_waves_per_cu > 64 || lines_per_wave < 1 || lines_per_wave > 1024)
     return fail(GPJSON_ERR_INVALID_ARG, "query shape out of range");
-  if (query_mode < kLineMode || query_mode > kLineLongMode)
-    return fail(GPJSON_ERR_INVALID_ARG, "query mode must be 0 (line), 1 (tiled), 2 (chain) or 3 (line, long lines to a wavefront)");
+  if (query_mode != kLineMode && query_mode != kLineLongMode)
+    return fail(GPJSON_ERR_INVALID_ARG, "query mode must be 0 (line) or 3 (line, long lines to a wavefront)");
   GJ_HIP(hipSetDevice(ctx->device));
   GJ_HIP(hipStreamSynchronize(ctx->stream));
   ctx->itune.block = index_block;
@@ -234,7 +229,7 @@ int gpjson_autotune(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* irs,
       best_q = c;
     }
   }
-  if (nq > 1 && best_q.mode != kChainMode && best_q.mode != kTiledMode) {
+  if (nq > 1) {
     // several queries: their work order (taking turns over blocks of lines)
     for (int qb : {16, 64, 256, 1024, 4096}) {
       QueryTune c = best_q;
@@ -315,7 +310,7 @@ int gpjson_tune_cache_load(gpjson_ctx* ctx, const char* path, int* loaded) {
     }
     if (ctx->arch != arch) continue;
     if (!index_config_supported(e.itune.block, e.itune.wpt) || !query_config_supported(e.qtune.block, e.qtune.group) ||
-        e.qtune.mode < kLineMode || e.qtune.mode > kLineLongMode || e.qtune.waves_per_cu < 1 ||
+        (e.qtune.mode != kLineMode && e.qtune.mode != kLineLongMode) || e.qtune.waves_per_cu < 1 ||
         e.qtune.waves_per_cu > 64 || e.qtune.lines_per_wave < 1 || e.qtune.lines_per_wave > 1024 ||
         e.qtune.qblock < 0 || e.qtune.qblock > (1 << 20) || k.levels < 0 ||
         k.levels > GPJSON_MAX_LEVELS || k.bucket < 0 || k.bucket > 62 || (k.batch != 0 && k.batch != 1)) {

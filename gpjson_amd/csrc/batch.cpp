@@ -53,7 +53,6 @@ struct gpjson_batch {
   int64_t ir_total = 0;  // bytes of the distinct programs, contiguous in dev_ir
   std::vector<uint8_t> ir_host;    // the same bytes on the host (a new layout uploads them again)
   std::vector<int64_t> ir_lens_host;
-  bool chain = false;    // every job is a member chain: chain evaluator
   // device memory
   DevBuf arena_rec, arena_lb, arena_nl, arena_spans, dev_files, dev_qjobs, dev_misc, dev_ir, states;
   DevBuf arena_long;      // [2 counters, 64 B][long-line work items]
@@ -172,8 +171,6 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
     ir_off[j] = ins.first->second;
   }
   b->ir_total = ir_total;
-  b->chain = b->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
-  for (int j = 0; j < b->njobs && b->chain; j++) b->chain = chain_depth(irs[j], ir_lens[j]) > 0;
   rc = b->dev_ir.ensure((size_t)ir_total + 16);
   if (rc) return rc;
   std::vector<uint8_t> flat((size_t)ir_total + 16);
@@ -266,8 +263,8 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   Q.lines_per_wave = b->qtune.lines_per_wave;
   Q.ir_base = (const uint8_t*)b->dev_ir.p;
   Q.ir_bytes = b->ir_total;
-  Q.qblock = b->chain ? 0 : b->qtune.qblock;
-  const bool long_lines = !b->chain && b->qtune.mode == kLineLongMode && b->qtune.group != 64;
+  Q.qblock = b->qtune.qblock;
+  const bool long_lines = b->qtune.mode == kLineLongMode && b->qtune.group != 64;
   if (long_lines) {
     Q.long_count = (unsigned long long*)b->arena_long.p;
     Q.long_items = (int64_t*)((uint8_t*)b->arena_long.p + 64);
@@ -275,8 +272,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
     Q.long_min = kLongLineBytes;
     GJ_HIP(zero_u64(Q.long_count, 2, s));
   }
-  if (b->chain) GJ_HIP(chain_launch(Q, b->qtune.block, b->qtune.group, b->query_grid, s));
-  else GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s, ctx->num_cus * 2));
+  GJ_HIP(query_launch(Q, b->qtune.block, b->qtune.group, maxL, b->query_grid, s, ctx->num_cus * 2));
   if (e2) GJ_HIP(hipEventRecord(e2, s));
   return GPJSON_OK;
 }
@@ -504,15 +500,8 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
   b->itune = best_i;
   double bestq = 1e30;
   QueryTune best_q = keep_q;
-  // the batch runs no tiled launch, and the chain evaluator only when every
-  // job is a member chain and all programs fit its LDS: such candidates would
-  // time line mode again
-  bool chains = b->ir_total <= kIrLdsBytes;
-  for (int64_t j = 0, off = 0; j < b->njobs && chains; off += b->ir_lens_host[j], j++)
-    chains = chain_depth(b->ir_host.data() + off, b->ir_lens_host[j]) > 0;
   if (rc == GPJSON_OK) {
     for (const QueryTune& c : query_candidates(keep_q)) {
-      if (c.mode == kTiledMode || (c.mode == kChainMode && !chains)) continue;
       b->qtune = c;
       double ims = 0, qms = 0;
       rc = time_shapes(&ims, &qms);
@@ -528,7 +517,7 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
     }
   }
   // the work order: the queries of a file taking turns over blocks of lines
-  if (rc == GPJSON_OK && !(best_q.mode == kChainMode && chains)) {
+  if (rc == GPJSON_OK) {
     for (int qb : {16, 64, 256, 1024, 4096}) {
       QueryTune c = best_q;
       c.qblock = qb;

@@ -221,9 +221,6 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
     qj.ir_len = ir_lens[q];
     qj.out = spans;  // job q writes at spans + q * lines * R * 2
     qj.item0 = 0;
-    qj.tile_lines = j.tile_lines;
-    qj.ntiles = ntiles;
-    qj.tile_bytes = index_tile_bytes(t.block, t.wpt);
     if (ir_lens[q]) std::memcpy(tab.ir + off, irs[q], (size_t)ir_lens[q]);
     off += ir_lens[q];
   }

@@ -34,11 +34,7 @@ struct QueryJob {
   const uint8_t* ir;  // device copy of the IR bytes
   int64_t ir_len;
   int64_t* out;       // int64[lines][R][2]
-  int64_t item0;      // first global work item (line, or tile in tiled mode) of this job
-  // tiled mode: lines starting in index tile t are [tile_lines[t], tile_lines[t+1])
-  const int64_t* tile_lines;
-  int64_t ntiles;
-  int64_t tile_bytes;
+  int64_t item0;      // first global work item (line) of this job
   // batched launches (QueryLaunch.qblock): this job is query qf of the nqf
   // queries of its file, whose jobs are consecutive and share the file's lines
   int qf;
@@ -59,9 +55,6 @@ struct QueryLaunch {
   const int64_t* total_dev;
   unsigned long long* counter;    // zeroed before launch
   int lines_per_wave;             // lines each lane group takes per wavefront claim
-  // tiled mode: work items are index tiles (total_items of them, jobs' item0 =
-  // tile prefix); a block stages a tile's leveled and string words in LDS.
-  int tiled;
   // Long lines (line mode): a lane group that claims a line longer than
   // long_min bytes appends the work item to long_items (up to long_cap items)
   // instead of walking it alone; a second launch then evaluates each of those

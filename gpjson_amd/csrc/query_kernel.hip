@@ -78,8 +78,7 @@ struct Group {
   }
 };
 
-// Where the leveled and string words come from: device memory, or an LDS copy
-// of the words [wlo, wlo + maxw) of the current tile.
+// Where the leveled and string words come from: device memory.
 struct GlobalWords {
   const uint64_t* LB;
   int64_t Wp;
@@ -90,17 +89,6 @@ struct GlobalWords {
   __device__ __forceinline__ uint64_t s(int64_t wi) const { return rec_s(rec, wi); }
   __device__ __forceinline__ bool has_summary() const { return LS != nullptr; }
   __device__ __forceinline__ uint64_t ls(int lvl, int64_t si) const { return LS[(int64_t)lvl * Ws + si]; }
-};
-
-struct LdsWords {
-  const uint64_t* w;  // [L + 1][maxw]: levels, then the string index
-  int64_t wlo;
-  int maxw;
-  int L;
-  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return w[lvl * maxw + (int)(wi - wlo)]; }
-  __device__ __forceinline__ uint64_t s(int64_t wi) const { return w[L * maxw + (int)(wi - wlo)]; }
-  __device__ __forceinline__ bool has_summary() const { return false; }
-  __device__ __forceinline__ uint64_t ls(int, int64_t) const { return ~0ull; }
 };
 
 // Summary words of level lvl for the blocks of 64 words that hold word nb and
@@ -676,70 +664,6 @@ __global__ __launch_bounds__(BLOCK) void query_long_kernel(QueryLaunch launch, i
 }
 
 
-// Tiled interpreter: a block takes one index tile of one job, copies the
-// leveled words of levels [0, L) and the string words covering the tile's
-// lines (from the first line's start to the last line's end) into LDS with
-// coalesced loads, and evaluates those lines from LDS.  Input bytes are read
-// from device memory (only around candidate keys).  A tile whose lines span
-// more than maxw words is evaluated straight from device memory.
-template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void query_tiled_kernel(QueryLaunch launch, int nslots, int maxw) {
-  constexpr int NG = BLOCK / G;
-  constexpr int kIrCap = 512;
-  extern __shared__ int64_t q_lds[];
-  __shared__ uint8_t s_ir[kIrCap];
-  __shared__ unsigned long long s_item;
-  const Group<G> g;
-  const int grp = threadIdx.x / G;
-  int64_t* le = q_lds + (int64_t)grp * nslots;
-  int* ci = reinterpret_cast<int*>(q_lds + (int64_t)NG * nslots) + (int64_t)grp * nslots;
-  uint64_t* words = reinterpret_cast<uint64_t*>(q_lds + (int64_t)NG * nslots + ((int64_t)NG * nslots + 1) / 2);
-  const int64_t total = launch.total_items;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(launch.counter, 1ull);
-    __syncthreads();
-    const int64_t item = (int64_t)s_item;
-    if (item >= total) break;
-    int lo = 0, hi = (int)launch.njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (launch.jobs[mid].item0 <= item) lo = mid; else hi = mid - 1;
-    }
-    const QueryJob& J = launch.jobs[lo];
-    const int64_t t = item - J.item0;
-    const int64_t la = J.tile_lines[t], lb = J.tile_lines[t + 1];
-    const int64_t lines = J.tile_lines[J.ntiles];
-    const uint8_t* ir = J.ir;
-    if (J.ir_len <= kIrCap) {
-      for (int k = threadIdx.x; k < J.ir_len; k += BLOCK) s_ir[k] = J.ir[k];
-      ir = s_ir;
-    }
-    if (la < lb) {
-      const int64_t lo_b = la == 0 ? 0 : J.NL[la];
-      const int64_t hi_b = lb < lines ? J.NL[lb] : J.N;
-      const int64_t wlo = lo_b >> 6;
-      const int64_t nw = hi_b > lo_b ? ((hi_b - 1) >> 6) - wlo + 1 : 1;
-      const bool staged = nw <= maxw;
-      if (staged) {
-        for (int lv = 0; lv < J.L; lv++) {
-          const uint64_t* src = J.LB + (int64_t)lv * J.Wp;
-          for (int64_t k = threadIdx.x; k < nw; k += BLOCK) words[lv * maxw + k] = src[wlo + k];
-        }
-        for (int64_t k = threadIdx.x; k < nw; k += BLOCK) words[J.L * maxw + k] = rec_s(J.rec, wlo + k);
-      }
-      __syncthreads();
-      const LdsWords LW{words, wlo, maxw, J.L};
-      const GlobalWords GW{J.LB, J.Wp, J.rec, J.LS, J.Ws};
-      for (int64_t i = la + grp; i < lb; i += NG) {
-        int64_t* out = J.out + i * (int64_t)J.R * 2;
-        if (staged) eval_line(g, J, LW, ir, lines, i, out, le, ci, nslots);
-        else eval_line(g, J, GW, ir, lines, i, out, le, ci, nslots);
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Batched runs: fills each query job's line count and first work item from the
 // per-file line counts of the newline bookkeeping, so the interpreter launch
 // needs no host round trip.  One block of kPlanBlock threads: the jobs' line
@@ -795,20 +719,9 @@ bool query_config_supported(int block, int group) {
   return (block == 64 || block == 128 || block == 256) && (group == 1 || group == 4 || group == 16 || group == 64);
 }
 
-// LDS of the tiled kernel: level slots plus (levels + 1) rows of maxw words.
-constexpr size_t kTiledLdsBudget = 56 * 1024;
-
 template <int BLOCK, int G>
 static hipError_t launch_bg(const QueryLaunch& L, int nslots, int grid, hipStream_t stream) {
   const size_t slots = (size_t)(BLOCK / G) * nslots * 12 + 8;
-  if (L.tiled) {
-    const int levels = nslots - 2;
-    int maxw = (int)((kTiledLdsBudget - slots) / ((size_t)(levels + 1) * 8));
-    maxw &= ~7;
-    const size_t lds = slots + (size_t)(levels + 1) * maxw * 8;
-    hipLaunchKernelGGL((query_tiled_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), lds, stream, L, nslots, maxw);
-    return hipGetLastError();
-  }
   hipLaunchKernelGGL((query_kernel<BLOCK, G>), dim3(grid), dim3(BLOCK), slots, stream, L, nslots);
   return hipGetLastError();
 }
@@ -838,7 +751,7 @@ hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, 
     case 256: e = launch_b<256>(L, group, nslots, grid, stream); break;
     default: break;
   }
-  if (e != hipSuccess || !L.long_items || L.tiled || group == 64) return e;
+  if (e != hipSuccess || !L.long_items || group == 64) return e;
   const size_t slots = (size_t)(256 / 64) * nslots * 12 + 8;
   hipLaunchKernelGGL(query_long_kernel<256>, dim3(long_grid > 0 ? long_grid : 1), dim3(256), slots, stream, L,
                      nslots);

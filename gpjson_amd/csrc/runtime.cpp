@@ -241,6 +241,29 @@ int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels,
   return GPJSON_OK;
 }
 
+int chain_depth(const uint8_t* ir, int64_t len) {
+  int64_t pc = 0;
+  int D = 0;
+  while (pc < len && ir[pc] == kMoveToKey) {
+    pc++;
+    int v = 0, shift = 0;
+    for (;;) {
+      if (pc >= len) return 0;
+      const uint8_t b = ir[pc++];
+      v |= (int)(b & 0x7F) << shift;
+      if (!(b & 0x80)) break;
+      shift += 7;
+      if (shift > 28) return 0;
+    }
+    pc += v;
+    if (pc >= len || ir[pc] != kMoveDown) return 0;
+    pc++;
+    D++;
+  }
+  if (D == 0 || pc + 2 != len || ir[pc] != kStoreResult || ir[pc + 1] != kEnd) return 0;
+  return D;
+}
+
 bool fuse_spec_from_ir(const uint8_t* ir, int64_t len, FuseSpec* spec) {
   const int D = chain_depth(ir, len);
   if (D < 1 || D > kFuseMaxDepth) return false;
@@ -319,17 +342,13 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   int64_t off = (int64_t)(jobs_off + jobs_bytes);
   QueryTune qt;
   shapes_for(ctx, idx->n, idx->L, 0, nullptr, &qt);
-  // Tiled mode needs the line count on the host (span offsets per query); the
-  // line-based mode reads it on the device.
-  bool chain = qt.mode == kChainMode && ir_total <= kIrLdsBytes;
-  for (int q = 0; q < nq && chain; q++) chain = chain_depth(irs[q], ir_lens[q]) > 0;
-  const bool tiled = !chain && qt.mode == kTiledMode && idx->L <= 4 && idx->ntiles > 0 && idx->lines > 0;
+  // the line count is read on the device
   for (int q = 0; q < nq; q++) {
     QueryJob j{};
     j.F = idx->buf->d;
     j.N = idx->n;
     j.NL = idx->NL;
-    j.lines = tiled ? idx->lines : 0;
+    j.lines = 0;
     j.rec = idx->rec;
     j.LB = idx->LB;
     j.LS = idx->LS;
@@ -339,11 +358,8 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
     j.R = R;
     j.ir = dbase + off;
     j.ir_len = ir_lens[q];
-    j.out = tiled ? spans_dev + (int64_t)q * idx->lines * R * 2 : spans_dev;
-    j.item0 = tiled ? (int64_t)q * idx->ntiles : 0;
-    j.tile_lines = idx->tile_lines;
-    j.ntiles = idx->ntiles;
-    j.tile_bytes = idx->tile_bytes;
+    j.out = spans_dev;
+    j.item0 = 0;
     std::memcpy(hbase + off, irs[q], (size_t)ir_lens[q]);
     off += ir_lens[q];
     hj[q] = j;
@@ -353,29 +369,25 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
   L.jobs = (const QueryJob*)(dbase + jobs_off);
   L.njobs = nq;
   L.total_items = 0;
-  L.lines_dev = tiled ? nullptr : idx->lines_dev;
+  L.lines_dev = idx->lines_dev;
   L.total_dev = nullptr;
   L.counter = (unsigned long long*)dbase;
   L.lines_per_wave = qt.lines_per_wave;
-  L.tiled = tiled ? 1 : 0;
   L.ir_base = dbase + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
-  L.qblock = (nq > 1 && !tiled && !chain) ? qt.qblock : 0;
-  if (!tiled && !chain && qt.mode == kLineLongMode && qt.group != 64) {
+  L.qblock = nq > 1 ? qt.qblock : 0;
+  if (qt.mode == kLineLongMode && qt.group != 64) {
     L.long_count = (unsigned long long*)(dbase + long_off);
     L.long_items = (int64_t*)(dbase + long_off + 64);
     L.long_cap = long_cap;
     L.long_min = kLongLineBytes;
   }
-  if (tiled) L.total_items = (int64_t)nq * idx->ntiles;
   out->launch = L;
-  out->chain = chain;
   out->block = qt.block;
   out->group = qt.group;
   out->levels = idx->L;
   const int waves_per_block = qt.block / 64;
   out->grid = std::max(1, ctx->num_cus * qt.waves_per_cu / waves_per_block);
-  if (tiled) out->grid = (int)std::max<int64_t>(1, std::min<int64_t>(L.total_items, (int64_t)ctx->num_cus * 4));
   out->long_grid = ctx->num_cus * 2;
   return GPJSON_OK;
 }
@@ -384,8 +396,7 @@ int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t befor
   GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
   if (pq.launch.long_count) GJ_HIP(zero_u64(pq.launch.long_count, 2, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
-  if (pq.chain) GJ_HIP(chain_launch(pq.launch, pq.block, pq.group, pq.grid, ctx->stream));
-  else GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream, pq.long_grid));
+  GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream, pq.long_grid));
   if (after_kernel) GJ_HIP(hipEventRecord(after_kernel, ctx->stream));
   return GPJSON_OK;
 }
@@ -1213,7 +1224,7 @@ int gpjson_plan_create(gpjson_ctx* ctx, gpjson_buf* buf, const uint8_t* const* i
   rc = dev_alloc((void**)&p->spans, (size_t)nq * (size_t)std::max<int64_t>(p->idx->nl_cap, 1) * R * 2 * 8);
   shapes_for(ctx, buf->n, L, 0, &p->itune, nullptr);  // the shape the index was just built with
   if (rc == GPJSON_OK) rc = prepare_queries(ctx, p->idx, irs, ir_lens, nq, R, p->spans, &p->table, &p->pq);
-  if (rc == GPJSON_OK && nq == 1 && ctx->qtune.fuse && !p->pq.chain && !p->pq.launch.tiled &&
+  if (rc == GPJSON_OK && nq == 1 && ctx->qtune.fuse &&
       fuse_spec_from_ir(irs[0], ir_lens[0], &p->fuse)) {
     // lines the interpreter may get: at most every line (and the empty file's one)
     p->fuse.list_cap = p->idx->nl_cap + 1;
@@ -1246,8 +1257,9 @@ int gpjson_plan_run(gpjson_plan* p, int timed) {
   if (!p) return fail(GPJSON_ERR_INVALID_ARG, "plan is NULL");
   gpjson_ctx* ctx = p->ctx;
   GJ_HIP(hipSetDevice(ctx->device));
-  // the plan keeps the tile shape it was prepared with (its tiled interpreter
-  // launch is sized by that index's tiles), whatever the context's shape is now
+  // the plan keeps the tile shape it was prepared with, whatever the context's
+  // shapes or tuning cache say now: a prepared plan runs the same launches
+  // every time
   int rc = enqueue_index(p->idx, timed ? p->ev[0] : nullptr, timed ? p->ev[1] : nullptr, &p->itune,
                          p->fuse.depth > 0 ? &p->fuse : nullptr);
   if (rc) return rc;

@@ -35,10 +35,8 @@ bool index_config_supported(int block, int wpt);
 int64_t index_tile_bytes(int block, int wpt);
 hipError_t query_launch(const QueryLaunch& L, int block, int group, int levels, int grid, hipStream_t stream,
                         int long_grid = 0);
-// chain_kernel.hip: member-chain programs, (KEY k DOWN)^D STORE END; the
-// launch's IR must fit kIrLdsBytes.
-int chain_depth(const uint8_t* ir, int64_t len);  // D, or 0 if ir is not a member chain
-hipError_t chain_launch(const QueryLaunch& L, int block, int group, int grid, hipStream_t stream);
+// D for a member-chain program (KEY k DOWN)^D STORE END, else 0.
+int chain_depth(const uint8_t* ir, int64_t len);
 bool query_config_supported(int block, int group);
 // gather.hip: result bytes gathered into one buffer (phase 1: sizes and the
 // total; phase 2: offsets and bytes).  block_sums holds gather_blocks(n) int64.
@@ -72,18 +70,16 @@ struct IndexTune {
   int wpt = 3;
 };
 
-// Interpreter modes: one lane group per line (the general interpreter); tiled
-// (a block stages one index tile's words in LDS, levels <= 4); chain (the
-// member-chain evaluator when every program of a launch is a member chain,
-// else line mode); line-long (line mode, and lines of kLongLineBytes or more
-// go to a second launch with a wavefront per line).
-enum QueryMode { kLineMode = 0, kTiledMode = 1, kChainMode = 2, kLineLongMode = 3 };
+// Interpreter modes: one lane group per line (the general interpreter);
+// line-long (line mode, and lines of kLongLineBytes or more go to a second
+// launch with a wavefront per line).  Modes 1 and 2 (a tiled interpreter and a
+// member-chain evaluator) lost every autotuner run and were removed.
+enum QueryMode { kLineMode = 0, kLineLongMode = 3 };
 
-// Defaults: the shape the autotuner measured fastest on 4 GiB of tweets for
-// $.user.lang: line mode, one lane per line, 4 lines per claim (2.86 ms; 3.31 ms
-// with 1 line per claim, 7.2 ms tiled, 7.0 ms on the chain evaluator, which
-// wins only for chains that end near the line start, e.g. $.id 0.74 ms against
-// 1.05 ms).
+// Defaults: the shape the autotuner measures fastest on 4 GiB of tweets for
+// $.user.lang: line mode, one lane per line, 16 waves per CU, 4 lines per claim
+// (1.11 ms; 1.25 ms with long lines to a wavefront, 1.92 ms with 1 line per
+// claim, 1.90 ms with 4 lanes per line).
 struct QueryTune {
   int mode = kLineMode;
   int block = 256;
@@ -191,7 +187,6 @@ void shapes_for(const gpjson_ctx* ctx, int64_t n, int L, int batch, IndexTune* i
 // Device job table + launch descriptor for nq queries over one index.
 struct PreparedQueries {
   QueryLaunch launch{};
-  bool chain = false;  // run on the member-chain evaluator
   int long_grid = 0;   // blocks of the long-line launch
   int grid = 0;
   int block = 256;

@@ -77,7 +77,6 @@ struct Request {
   int64_t off = 0;             // element offset in that buffer
   bool collecting = false;     // a waiter is copying from the slot, or a client holds a view of it
   int64_t file_bytes = 0;
-  bool chain = false;          // a member chain (chain evaluator)
 };
 
 struct Batch {
@@ -353,10 +352,7 @@ int launch_batch(gpjson_server* s, Batch& b) {
   GJ_HIP(zero_u64(L.counter, 1, stream));
   const int waves_per_block = ctx->qtune.block / 64;
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
-  bool chain = ctx->qtune.mode == kChainMode && ir_total <= kIrLdsBytes;
-  for (int i = 0; i < n && chain; i++) chain = b.reqs[i]->chain;
-  if (chain) GJ_HIP(chain_launch(L, ctx->qtune.block, ctx->qtune.group, grid, stream));
-  else GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, stream));
+  GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, stream));
   GJ_HIP(hipMemcpyAsync(b.host_spans, b.spans.p, (size_t)elems * 8, hipMemcpyDeviceToHost, stream));
   GJ_HIP(hipEventRecord(b.done, stream));
   return GPJSON_OK;
@@ -516,7 +512,6 @@ int gpjson_server_submit(gpjson_server* s, const char* key, const char* path, ui
   if (it == s->files.end()) return fail(GPJSON_ERR_INVALID_ARG, std::string("unknown file: ") + key);
   if (r->cp.max_depth > s->levels)
     return fail(GPJSON_ERR_CAPACITY, "path deeper than the server's index levels");
-  r->chain = chain_depth(r->cp.ir.data(), (int64_t)r->cp.ir.size()) > 0;
   r->ticket = s->next_ticket++;
   r->key = key;
   r->path = path;

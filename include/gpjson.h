@@ -334,13 +334,11 @@ void gpjson_server_destroy(gpjson_server *server);
 
 /* Current launch shapes of the context: index tile = index_block threads x
  * index_wpt 64-byte words; interpreter mode (query_mode: 0 = one lane group
- * per line; 1 = tiled, a block stages one index tile's leveled and string words
- * in LDS, up to 4 levels; 2 = member-chain evaluator when every query of a
- * launch is a member chain $.a.b..., else mode 0; 3 = mode 0, and lines of
- * 8 KB or more are handed to a second launch that gives each a wavefront),
- * block size, lanes per line (query_group: 1, 4, 16 or 64; modes 0, 1 and 3),
- * persistent grid in wavefronts per CU and lines a lane group claims at a time
- * (modes 0 and 3).  The reference
+ * per line; 3 = mode 0, and lines of 8 KB or more are handed to a second
+ * launch that gives each a wavefront; modes 1 and 2 no longer exist and are
+ * rejected), block size, lanes per line (query_group: 1, 4, 16 or 64),
+ * persistent grid in wavefronts per CU and lines a lane group claims at a
+ * time.  gpjson_set_config clears the tuning cache.  The reference
  * hard-codes 8x1024 (stages/CombinedIndex.java:11-14) and 512x1024 threads with
  * one thread per line (functions/QueryFunction.java:229). */
 int gpjson_get_config(gpjson_ctx *ctx, int *index_block, int *index_wpt, int *query_mode, int *query_block,

@@ -13,9 +13,23 @@ import pytest
 
 from conftest import fixture_bytes
 from oracle import oracle as O
-from test_chain_cpu import ADV_PATHS, SOUP_PATHS, TWEET_PATHS, byte_bitmap
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+TWEET_PATHS = ["$.user.lang", "$.id", "$.place.country_code", "$.user.stats.followers", "$.user.name",
+               "$.user.stats.friends", "$.retweet_count", "$.user.profile.colors.background", "$.user.id",
+               "$.place.bounding_box.type", "$.lang", "$.nokey", "$.user.nokey", "$.entities"]
+ADV_PATHS = ["$.id", "$.s1", "$.chain.pad", "$.chain.c1.pad", "$.chain.c1.c2", "$.chain.c1.c2.c3.c4", "$.deep.d.d",
+             "$.s2", "$.chain.c1.c2.c3.c4.c5.c6.c7.c8.c9.c10.c11.c12.c13.c14.c15.c16.c17.c18.c19", "$.arr"]
+SOUP_PATHS = ["$.a", "$.a.b", "$.x.y.z", "$.b", "$['a b']"]
+
+
+def byte_bitmap(f: np.ndarray, c: int) -> np.ndarray:
+    """Bit per byte of f that equals c, packed LSB-first into u64 words."""
+    W = (len(f) + 63) // 64
+    bits = np.zeros(max(W, 1) * 64, dtype=np.uint8)
+    bits[:len(f)] = f == c
+    return np.packbits(bits, bitorder="little").view("<u8").astype(np.uint64)
 TILES = (3, 8, 64, 256, 768)
 
 

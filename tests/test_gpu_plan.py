@@ -69,7 +69,7 @@ def test_plan_keeps_tile_shape(ctx, files):
     cfg = ctx.get_config()
     buf = ctx.stage(data)
     try:
-        ctx.set_config(256, 4, 1, 256, 4, 32, 1)  # tiled interpreter, 64 KiB index tiles
+        ctx.set_config(256, 4, 3, 256, 4, 32, 1)  # long lines to a wavefront, 64 KiB index tiles
         plan = ctx.plan(buf, [c.ir for c in cps])
         ctx.set_config(256, 1, 0, 64, 1, 32, 1)   # other tile size and line mode afterwards
         plan.run()

@@ -1,7 +1,7 @@
 """Every launch shape the autotuner may pick gives the oracle's bit-exact answers:
-all index tile shapes, and the interpreter in tiled and line mode with every
-lane-group size, with and without the long-line launch, and the member-chain
-evaluator."""
+all index tile shapes, and the interpreter in line mode with every lane-group
+size, with and without the long-line launch, on mixed query sets and on member
+chains alone and together."""
 import numpy as np
 import pytest
 
@@ -10,10 +10,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 INDEX_SHAPES = [(256, 1), (256, 2), (256, 3), (256, 4), (512, 1), (512, 2), (128, 2), (128, 4)]
-# (mode, block, group); mode 2 with these mixed query sets runs in line mode;
-# mode 3 hands the adversarial input's lines (4-64 KB) to the long-line launch
-QUERY_SHAPES = [(1, 256, 1), (1, 256, 4), (1, 128, 16), (1, 256, 64), (0, 256, 1), (0, 256, 4), (0, 64, 16),
-                (0, 256, 64), (2, 256, 1), (3, 256, 1), (3, 256, 4), (3, 128, 16)]
+# (mode, block, group); mode 3 hands the adversarial input's lines (4-64 KB) to
+# the long-line launch
+QUERY_SHAPES = [(0, 256, 1), (0, 128, 1), (0, 256, 4), (0, 64, 16), (0, 256, 64), (3, 256, 1), (3, 256, 4),
+                (3, 128, 16)]
 
 
 @pytest.fixture(scope="module")
@@ -32,7 +32,7 @@ def inputs():
 def test_index_shapes(ctx, inputs, block, wpt):
     cfg = ctx.get_config()
     try:
-        ctx.set_config(block, wpt, 1, 256, 4, 32, 1)
+        ctx.set_config(block, wpt, 0, 256, 4, 32, 1)
         for name, data in inputs.items():
             print(f"[shapes] index {block}x{wpt} {name}", flush=True)
             buf = ctx.stage(data)
@@ -55,17 +55,17 @@ PATHS = {
 }
 
 
-@pytest.mark.parametrize("tiled,block,group", QUERY_SHAPES)
-def test_query_shapes(ctx, inputs, tiled, block, group):
+@pytest.mark.parametrize("mode,block,group", QUERY_SHAPES)
+def test_query_shapes(ctx, inputs, mode, block, group):
     from gpjson_amd import compile_path
     cfg = ctx.get_config()
     try:
-        ctx.set_config(cfg["index_block"], cfg["index_wpt"], tiled, block, group, 32, 2)
+        ctx.set_config(cfg["index_block"], cfg["index_wpt"], mode, block, group, 32, 2)
         for name, data in inputs.items():
             cps = [compile_path(p) for p in PATHS[name]]
             L = max(c.max_depth for c in cps)
             R = max(c.num_results for c in cps)
-            print(f"[shapes] query tiled={tiled} {block}/{group} {name}", flush=True)
+            print(f"[shapes] query mode={mode} {block}/{group} {name}", flush=True)
             buf = ctx.stage(data)
             idx = ctx.index(buf, L)
             got = ctx.run_queries(idx, [c.ir for c in cps], R)
@@ -90,13 +90,13 @@ CHAIN_PATHS = {
 
 
 @pytest.mark.parametrize("block,group,wpc", [(64, 1, 8), (256, 1, 16), (256, 4, 16), (256, 16, 8), (128, 64, 16)])
-def test_chain_mode(ctx, inputs, block, group, wpc):
-    """The member-chain evaluator (every query a member chain) equals the
-    oracle, one query per launch and all queries in one launch."""
+def test_member_chains(ctx, inputs, block, group, wpc):
+    """Member-chain queries equal the oracle, one query per launch and all
+    queries in one launch."""
     from gpjson_amd import compile_path
     cfg = ctx.get_config()
     try:
-        ctx.set_config(cfg["index_block"], cfg["index_wpt"], 2, block, group, wpc, 1)
+        ctx.set_config(cfg["index_block"], cfg["index_wpt"], 0, block, group, wpc, 1)
         for name, data in inputs.items():
             cps = [compile_path(p) for p in CHAIN_PATHS[name]]
             L = max(c.max_depth for c in cps)
