@@ -12,8 +12,10 @@
 //     in-flight batch uses (their device copies are freed; the host copy stays);
 //   - an admission thread stages and indexes files on its own stream while
 //     queries run on the serving stream;
-//   - the batching thread coalesces every pending request whose file is
-//     resident into ONE interpreter launch (continuous batching: requests that
+//   - the batching thread coalesces the pending requests whose file is
+//     resident into ONE interpreter launch (a backlog larger than the free
+//     slots can take is shared among them, kMinSplit requests or more per
+//     launch; continuous batching: requests that
 //     arrive while batches are on the GPU join the next batch; up to kSlots
 //     batches are in flight, each with its own span buffers and its own
 //     stream, so one batch's span copy to the host overlaps the next batch's
@@ -60,6 +62,9 @@ struct ServedFile {
 };
 
 constexpr int kSlots = 4;
+// A backlog is split among the free slots only into batches of at least this
+// many requests.
+constexpr size_t kMinSplit = 16;
 
 struct Request {
   uint64_t ticket = 0;
@@ -382,10 +387,17 @@ void batcher_main(gpjson_server* s) {
           for (int k = 0; k < kSlots; k++)
             if (!s->slots[k].active && s->slots[k].outstanding == 0) { free_slot = k; break; }
         }
-        // requests whose file is resident can run now; others wait for admission
+        // requests whose file is resident can run now; others wait for admission.
+        // A large backlog is shared among the free slots, so that one batch's
+        // span copy to the host overlaps the next batch's launch instead of
+        // one batch holding everything
         if (free_slot >= 0) {
+          size_t free_slots = 0;
+          for (int k = 0; k < kSlots; k++) free_slots += !s->slots[k].active && s->slots[k].outstanding == 0;
+          const size_t cap = std::max<size_t>(kMinSplit, (s->pending.size() + free_slots - 1) / free_slots);
           for (auto it = s->pending.begin(); it != s->pending.end();) {
             ServedFile* f = s->files[(*it)->key].get();
+            if (f->resident && take.size() >= cap) break;
             if (f->resident) {
               take.push_back(*it);
               take_files.push_back(f);
