@@ -104,6 +104,8 @@ SIGNATURES = {
     "gpjson_batch_free": (None, [c_void_p]),
     "gpjson_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
     "gpjson_batch_query_order": (c_int, [c_void_p]),
+    "gpjson_batch_set_level_split": (c_int, [c_void_p, c_int]),
+    "gpjson_batch_get_level_split": (c_int, [c_void_p]),
     "gpjson_set_query_order": (c_int, [c_void_p, c_int]),
     "gpjson_get_query_order": (c_int, [c_void_p, P(c_int)]),
     "gpjson_batch_get_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int), P(c_int),

@@ -374,6 +374,13 @@ class Batch:
     def query_order(self) -> int:
         return int(N.load().gpjson_batch_query_order(self._h))
 
+    def set_level_split(self, enable: bool) -> None:
+        """One index launch per level class (True) or one for every file (False)."""
+        _check(N.load().gpjson_batch_set_level_split(self._h, 1 if enable else 0))
+
+    def level_split(self) -> bool:
+        return N.load().gpjson_batch_get_level_split(self._h) == 1
+
     def timing(self):
         a, t = ctypes.c_double(), ctypes.c_double()
         _check(N.load().gpjson_batch_timing(self._h, ctypes.byref(a), ctypes.byref(t)))

@@ -47,6 +47,16 @@ struct gpjson_batch {
   std::vector<int> job_file, job_q, job_R;
   std::vector<int64_t> job_span_off;  // element offset of each job's spans in the span arena
   std::vector<int> file_first_job;
+  // Level classes: with split_levels, the files whose level count needs a
+  // smaller kernel variant than the deepest file's come first in tile order
+  // and get an index launch of their own (tiles [0, split_tile)); the rest
+  // take a second launch.  Files never span the two launches, so neither
+  // launch waits on the other.  slot_file maps the device file table's order
+  // (tile order) to the caller's file index.
+  bool split_levels = false;
+  int64_t split_tile = 0;
+  int nlv_first = 2, nlv_all = 2;
+  std::vector<int> slot_file, file_slot;
   int64_t total_tiles = 0;
   int64_t total_bytes = 0;
   int max_levels = 0;
@@ -111,7 +121,21 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   const int64_t tile_bytes = index_tile_bytes(b->itune.block, b->itune.wpt);
   int64_t s_words = 0, lb_words = 0, nl_elems = 0, span_elems = 0, tiles = 0;
   std::vector<int64_t> s_off(b->nfiles), lb_off(b->nfiles), nl_off(b->nfiles);
-  for (int f = 0; f < b->nfiles; f++) {
+  // tile order: the smaller level class first when the launch is split
+  b->nlv_all = nlv_for_levels(b->max_levels);
+  b->nlv_first = 2;
+  b->slot_file.clear();
+  for (int pass = 0; pass < 2; pass++)
+    for (int f = 0; f < b->nfiles; f++) {
+      const bool first = b->split_levels && nlv_for_levels(b->files[f].L) < b->nlv_all;
+      if (first == (pass == 0)) b->slot_file.push_back(f);
+      if (first && pass == 0) b->nlv_first = std::max(b->nlv_first, nlv_for_levels(b->files[f].L));
+    }
+  b->file_slot.assign(b->nfiles, 0);
+  for (int k = 0; k < b->nfiles; k++) b->file_slot[b->slot_file[k]] = k;
+  b->split_tile = 0;
+  for (int k = 0; k < b->nfiles; k++) {
+    const int f = b->slot_file[k];
     FileJob& J = b->files[f];
     const int64_t n = b->bufs[f]->n;
     J.N = n;
@@ -121,6 +145,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
     J.ntiles = (n + tile_bytes - 1) / tile_bytes;
     J.tile0 = tiles;
     tiles += J.ntiles;
+    if (b->split_levels && nlv_for_levels(J.L) < b->nlv_all) b->split_tile = tiles;
     s_off[f] = s_words;
     s_words += (J.W + 7) / 8 * 8;
     lb_off[f] = lb_words;
@@ -204,9 +229,14 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   b->lines_dev = (int64_t*)(misc + 4096);
   b->job_file_dev = (int*)(misc + 4096 + (size_t)b->nfiles * 8);
   GJ_HIP(hipMemcpy(b->dev_ir.p, flat.data(), flat.size(), hipMemcpyHostToDevice));
-  GJ_HIP(hipMemcpy(b->dev_files.p, b->files.data(), sizeof(FileJob) * b->nfiles, hipMemcpyHostToDevice));
+  // the device tables in tile order: file table by slot, each job's file slot
+  std::vector<FileJob> table(b->nfiles);
+  for (int k = 0; k < b->nfiles; k++) table[k] = b->files[b->slot_file[k]];
+  std::vector<int> job_slot(b->njobs);
+  for (int j = 0; j < b->njobs; j++) job_slot[j] = b->file_slot[b->job_file[j]];
+  GJ_HIP(hipMemcpy(b->dev_files.p, table.data(), sizeof(FileJob) * b->nfiles, hipMemcpyHostToDevice));
   GJ_HIP(hipMemcpy(b->dev_qjobs.p, b->qjobs.data(), sizeof(QueryJob) * b->njobs, hipMemcpyHostToDevice));
-  GJ_HIP(hipMemcpy(b->job_file_dev, b->job_file.data(), sizeof(int) * b->njobs, hipMemcpyHostToDevice));
+  GJ_HIP(hipMemcpy(b->job_file_dev, job_slot.data(), sizeof(int) * b->njobs, hipMemcpyHostToDevice));
   GJ_HIP(hipMemset(b->dev_misc.p, 0, 4096));
   const int bpc = 2048 / b->itune.block;  // resident blocks per CU the grid aims for
   b->index_grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ctx->num_cus * bpc));
@@ -233,22 +263,41 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   int* flags = (int*)b->bk.p;
   int64_t* block_sums = (int64_t*)((uint8_t*)b->bk.p + 128);
   int64_t* file_base = block_sums + b->total_tiles / 256 + 2;
-  int maxL = 0;
-  for (const auto& f : b->files) maxL = std::max(maxL, f.L);
+  const int maxL = b->max_levels;
   const int nlv = nlv_for_levels(maxL);
+  // the index launches of one pass: one over every tile, or one per level
+  // class (the ticket restarts at the second class's first tile)
+  const bool split = b->split_tile > 0 && b->split_tile < b->total_tiles;
+  auto index_pass = [&](const IndexLaunch& P) -> int {
+    if (b->total_tiles == 0) return GPJSON_OK;
+    if (!split) {
+      GJ_HIP(index_launch(P, b->itune.block, b->itune.wpt, nlv, false, b->index_grid, s));
+      return GPJSON_OK;
+    }
+    IndexLaunch A = P;
+    A.total_tiles = b->split_tile;
+    GJ_HIP(index_launch(A, b->itune.block, b->itune.wpt, b->nlv_first, false,
+                        (int)std::min<int64_t>(b->index_grid, b->split_tile), s));
+    IndexLaunch B = P;
+    B.tile_begin = b->split_tile;
+    GJ_HIP(zero_u64((unsigned long long*)P.ticket, 1, s));
+    GJ_HIP(index_launch(B, b->itune.block, b->itune.wpt, nlv, false,
+                        (int)std::min<int64_t>(b->index_grid, b->total_tiles - b->split_tile), s));
+    return GPJSON_OK;
+  };
   GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s));
   GJ_HIP(zero_u64((unsigned long long*)flags, 16, s));
   GJ_HIP(zero_u64(b->counter, 1, s));
   if (e0) GJ_HIP(hipEventRecord(e0, s));
-  if (b->total_tiles > 0)
-    GJ_HIP(index_launch(L, b->itune.block, b->itune.wpt, nlv, false, b->index_grid, s));
+  int rc = index_pass(L);
+  if (rc) return rc;
   GJ_HIP(index_finish_batch(L, block_sums, file_base, b->lines_dev, flags, flags + 8, s));
   IndexLaunch L2 = L;
   L2.speculate = 0;
   L2.run_if = flags;
   GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s, flags));
-  if (b->total_tiles > 0)
-    GJ_HIP(index_launch(L2, b->itune.block, b->itune.wpt, nlv, false, b->index_grid, s));
+  rc = index_pass(L2);
+  if (rc) return rc;
   GJ_HIP(index_finish_batch(L2, block_sums, file_base, b->lines_dev, flags + 16, flags + 8, s));
   if (e1) GJ_HIP(hipEventRecord(e1, s));
   GJ_HIP(batch_plan((const FileJob*)b->dev_files.p, b->nfiles, (QueryJob*)b->dev_qjobs.p, b->job_file_dev,
@@ -281,8 +330,10 @@ static int read_back_lines(gpjson_batch* b, bool* overflow) {
   int ov = 0;
   GJ_HIP(hipStreamSynchronize(b->ctx->stream));
   GJ_HIP(hipMemcpy(&ov, b->overflow_dev, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int64_t> by_slot(b->nfiles, 0);
+  GJ_HIP(hipMemcpy(by_slot.data(), b->lines_dev, sizeof(int64_t) * b->nfiles, hipMemcpyDeviceToHost));
   b->lines.assign(b->nfiles, 0);
-  GJ_HIP(hipMemcpy(b->lines.data(), b->lines_dev, sizeof(int64_t) * b->nfiles, hipMemcpyDeviceToHost));
+  for (int k = 0; k < b->nfiles; k++) b->lines[b->slot_file[k]] = by_slot[k];
   *overflow = ov != 0;
   return GPJSON_OK;
 }
@@ -498,6 +549,30 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
     }
   }
   b->itune = best_i;
+  // one index launch for all files, or one per level class (when some files
+  // need a smaller kernel variant than the deepest file's)
+  const bool keep_split = b->split_levels;
+  bool best_split = keep_split;
+  bool two_classes = false;
+  for (int f = 0; f < b->nfiles; f++)
+    two_classes |= nlv_for_levels(b->files[f].L) < nlv_for_levels(b->max_levels) && b->bufs[f]->n > 0;
+  if (rc == GPJSON_OK && two_classes) {
+    double t[2] = {0, 0};
+    for (int split = 0; split < 2 && rc == GPJSON_OK; split++) {
+      b->split_levels = split != 0;
+      double qms = 0;
+      rc = time_shapes(&t[split], &qms);
+      if (rc) break;
+      std::snprintf(line, sizeof(line), "batch index (best shape) %s: %.4f ms\n",
+                    split ? "one launch per level class" : "one launch", t[split]);
+      rep += line;
+    }
+    if (rc == GPJSON_OK) {
+      best_split = t[1] < t[0];
+      best = best_split ? t[1] : t[0];
+    }
+  }
+  b->split_levels = best_split;
   double bestq = 1e30;
   QueryTune best_q = keep_q;
   if (rc == GPJSON_OK) {
@@ -538,6 +613,7 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
     // back to the shapes the batch had, laid out as before
     b->itune = keep_i;
     b->qtune = keep_q;
+    b->split_levels = keep_split;
     (void)layout(b, caps);
     return rc;
   }
@@ -549,10 +625,11 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
   const TuneKey key{size_bucket(b->total_bytes), b->max_levels, 1};
   ctx->tune_cache[key] = TuneEntry{best_i, best_q, best + bestq};
   std::snprintf(line, sizeof(line),
-                "batch best index %dx%d (%.4f ms), query mode %d block %d group %d wpc %d lpg %d qblock %d "
+                "batch best index %dx%d%s (%.4f ms), query mode %d block %d group %d wpc %d lpg %d qblock %d "
                 "(%.4f ms); cached for batches of [2^%d, 2^%d) bytes at %d levels\n",
-                best_i.block, best_i.wpt, best, best_q.mode, best_q.block, best_q.group, best_q.waves_per_cu,
-                best_q.lines_per_wave, best_q.qblock, bestq, key.bucket, key.bucket + 1, b->max_levels);
+                best_i.block, best_i.wpt, best_split ? " per level class" : "", best, best_q.mode, best_q.block,
+                best_q.group, best_q.waves_per_cu, best_q.lines_per_wave, best_q.qblock, bestq, key.bucket,
+                key.bucket + 1, b->max_levels);
   rep += line;
   if (report && report_cap) {
     const size_t n = std::min(report_cap - 1, rep.size());
@@ -563,6 +640,27 @@ int gpjson_batch_autotune(gpjson_batch* b, int iterations, char* report, size_t 
 }
 
 int gpjson_batch_query_order(const gpjson_batch* b) { return b ? b->qtune.qblock : -1; }
+
+int gpjson_batch_set_level_split(gpjson_batch* b, int enable) {
+  if (!b) return fail(GPJSON_ERR_INVALID_ARG, "batch is NULL");
+  GJ_HIP(hipSetDevice(b->ctx->device));
+  GJ_HIP(hipStreamSynchronize(b->ctx->stream));
+  drop_graph(b);
+  b->timed_graph = false;
+  b->split_levels = enable != 0;
+  std::vector<int64_t> caps(b->nfiles);
+  for (int f = 0; f < b->nfiles; f++) caps[f] = b->files[f].nl_cap;
+  int rc = layout(b, caps);
+  if (!rc) rc = enqueue_batch(b, nullptr, nullptr, nullptr);  // the spans of the new layout
+  if (rc) return rc;
+  GJ_HIP(hipStreamSynchronize(b->ctx->stream));
+  return GPJSON_OK;
+}
+
+int gpjson_batch_get_level_split(const gpjson_batch* b) {
+  if (!b) return -1;
+  return b->split_tile > 0 && b->split_tile < b->total_tiles ? 1 : 0;
+}
 
 int gpjson_batch_get_config(const gpjson_batch* b, int* index_block, int* index_wpt, int* query_mode,
                             int* query_block, int* query_group, int* query_waves_per_cu, int* lines_per_wave) {

@@ -290,6 +290,14 @@ int gpjson_batch_spans(gpjson_batch *batch, int file, int q, int64_t *spans, int
  * spans are those of a run with the new shapes) and records it in the tuning
  * cache.  Allocates; waits.  report (optional) gets one line per candidate. */
 int gpjson_batch_autotune(gpjson_batch *batch, int iterations, char *report, size_t report_cap);
+/* Index launches of a batch: one over every file's tiles (enable 0), or one per
+ * level class (enable 1): the files that need a smaller kernel variant than
+ * the deepest file's take a launch of their own, so they do not run the
+ * deeper variant.  Lays the batch out again and runs it once (allocates;
+ * waits).  The batch autotuner picks this too.  get: 1 when two launches run,
+ * 0 when one does (one level class), -1 for a NULL batch. */
+int gpjson_batch_set_level_split(gpjson_batch *batch, int enable);
+int gpjson_batch_get_level_split(const gpjson_batch *batch);
 /* The batch's work order (gpjson_set_query_order), -1 for a NULL batch. */
 int gpjson_batch_query_order(const gpjson_batch *batch);
 /* The batch's launch shapes (the fields of gpjson_get_config). */

@@ -166,6 +166,32 @@ def test_batch_autotune_and_tuning_cache(ctx, batch_inputs, tmp_path):
             b.free()
 
 
+def test_level_split(ctx, batch_inputs):
+    """One index launch per level class (the tweets files run the 4-level
+    variant, the adversarial files the deepest one) gives the oracle's spans,
+    on a direct run and on graph replays, and back to one launch again."""
+    files, qsets, oracle = batch_inputs
+    bufs = [ctx.stage(d) for _, d in files]
+    batch = ctx.batch(bufs, [[c.ir for c in qs] for qs in qsets])
+    try:
+        for split in (True, False, True):
+            batch.set_level_split(split)
+            assert batch.level_split() == split
+            for tag in ("after the switch", "direct run", "graph replay", "graph replay"):
+                if tag == "direct run":
+                    batch.run(graph=False)
+                elif tag == "graph replay":
+                    batch.run(graph=True)
+                    ctx.synchronize()
+                for f in range(len(files)):
+                    for q in range(len(qsets[f])):
+                        assert np.array_equal(batch.spans(f, q), oracle[f][q]), (split, tag, f, q, files[f][0])
+    finally:
+        batch.free()
+        for b in bufs:
+            b.free()
+
+
 def test_plain_autotune_cache(ctx):
     """gpjson_autotune records its choice for the input's size bucket and
     depth; plans and one-shot queries that then run with it equal the oracle."""
