@@ -25,8 +25,8 @@ using namespace gj;
 namespace gj {
 
 const std::vector<IndexTune>& index_candidates() {
-  static const std::vector<IndexTune> c = {{256, 1}, {256, 2}, {256, 3}, {256, 4},
-                                           {512, 1}, {512, 2}, {128, 2}, {128, 4}};
+  static const std::vector<IndexTune> c = {{256, 1}, {256, 2}, {256, 3}, {256, 4}, {512, 1},
+                                           {512, 2}, {128, 2}, {128, 3}, {128, 4}};
   return c;
 }
 

@@ -9,7 +9,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-INDEX_SHAPES = [(256, 1), (256, 2), (256, 3), (256, 4), (512, 1), (512, 2), (128, 2), (128, 4)]
+INDEX_SHAPES = [(256, 1), (256, 2), (256, 3), (256, 4), (512, 1), (512, 2), (128, 2), (128, 3), (128, 4)]
 # (mode, block, group); mode 3 hands the adversarial input's lines (4-64 KB) to
 # the long-line launch
 QUERY_SHAPES = [(0, 256, 1), (0, 128, 1), (0, 256, 4), (0, 64, 16), (0, 256, 64), (3, 256, 1), (3, 256, 4),
