@@ -159,6 +159,27 @@ def test_batch_autotune_and_tuning_cache(ctx, batch_inputs, tmp_path):
         assert ctx.tune_cache_size() == 0
         assert ctx.tune_cache_load(path) == 1
         assert ctx.tune_cache_size() == 1
+        # entries of another architecture are skipped; unsupported shapes,
+        # malformed lines and foreign files are refused
+        header, entry = open(path).read().splitlines()[:2]
+        fields = entry.split()
+
+        def write(name, lines):
+            q = str(tmp_path / name)
+            with open(q, "w") as fh:
+                fh.write("\n".join(lines) + "\n")
+            return q
+
+        other = write("other.txt", [header, " ".join(["gfx000"] + fields[1:])])
+        assert ctx.tune_cache_load(other) == 0
+        bad_shape = write("bad.txt", [header, " ".join(fields[:4] + ["100"] + fields[5:])])
+        with pytest.raises(Exception):
+            ctx.tune_cache_load(bad_shape)
+        with pytest.raises(Exception):
+            ctx.tune_cache_load(write("short.txt", [header, " ".join(fields[:6])]))
+        with pytest.raises(Exception):
+            ctx.tune_cache_load(write("foreign.txt", ["something else", entry]))
+        assert ctx.tune_cache_size() == 1
     finally:
         batch.free()
         ctx.tune_cache_clear()
