@@ -104,26 +104,6 @@ GJ_HD uint32_t perm_b32(uint32_t s0, uint32_t s1, uint32_t sel) {
 }
 #endif
 
-// 8x8 bit-matrix transpose of the 8 bytes held in (lo, hi): bit c of byte r
-// moves to bit r of byte c, so byte c afterwards holds bit c of each of the 8
-// input bytes.  Three delta swaps (2x2, then 4x4, then 8x8 blocks; Warren,
-// Hacker's Delight, section 7-3); the last one exchanges nibbles between the
-// two halves.
-GJ_HD void transpose8x8(uint32_t& lo, uint32_t& hi) {
-  uint32_t t;
-  t = (lo ^ (lo >> 7)) & 0x00AA00AAu;
-  lo ^= t ^ (t << 7);
-  t = (hi ^ (hi >> 7)) & 0x00AA00AAu;
-  hi ^= t ^ (t << 7);
-  t = (lo ^ (lo >> 14)) & 0x0000CCCCu;
-  lo ^= t ^ (t << 14);
-  t = (hi ^ (hi >> 14)) & 0x0000CCCCu;
-  hi ^= t ^ (t << 14);
-  t = ((lo >> 4) ^ hi) & 0x0F0F0F0Fu;
-  hi ^= t;
-  lo ^= t << 4;
-}
-
 // 4x4 byte transpose: out[k] = byte k of a0, a1, a2, a3 (in that order).
 GJ_HD void transpose4x4_bytes(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t out[4]) {
   const uint32_t u0 = perm_b32(a1, a0, 0x05010400u);  // a0.0 a1.0 a0.1 a1.1
@@ -136,19 +116,50 @@ GJ_HD void transpose4x4_bytes(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3
   out[3] = perm_b32(u3, u1, 0x07060302u);
 }
 
+// One butterfly stage of the bit-plane extraction: registers a and b differ
+// in one bit of the register index, and the bits of a at positions with bit S
+// of the position set trade places with the bits of b at positions with it
+// clear (m selects positions with bit S clear, SH = 2^S).  Afterwards that
+// register-index bit and position bit have exchanged their meaning.
+template <int SH>
+GJ_HD void plane_swap(uint32_t& a, uint32_t& b, uint32_t m) {
+  const uint32_t na = (a & m) | ((b << SH) & ~m);
+  const uint32_t nb = ((a >> SH) & m) | (b & ~m);
+  a = na;
+  b = nb;
+}
+
 // Bit planes of 32 bytes (d[0..7], byte b of d[i] = position 4i+b): bit j of
-// pl[k] is bit k of byte j.  Each 8-byte group is bit-transposed, then byte k
-// of the four groups' results is gathered into plane k.
+// pl[k] is bit k of byte j.  A bit of byte j (j = j4..j0) starts in register
+// j >> 2 at position 8*(j & 3) + k.  Two 4x4 byte transposes put byte
+// j = t + 8s in slot s of register t, so the position reads (j4 j3 k2 k1 k0)
+// and the register index (j2 j1 j0); three butterfly stages then exchange k0,
+// k1, k2 with j0, j1, j2, which leaves bit k of byte j in register k at
+// position j.  (48 shifts and bitfield inserts and 16 byte permutes per 32
+// bytes, against 84 + 16 for an 8x8 bit transpose of each 8-byte group.)
+GJ_HD uint32_t opaque_mask(uint32_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // an SGPR whose value the optimizer does not see: it would otherwise move
+  // the stages' masks across stages and split bitfield inserts into and/or
+  asm("" : "=s"(m) : "0"(m));
+#endif
+  return m;
+}
+
 GJ_HD void bit_planes32(const uint32_t d[8], uint32_t pl[8]) {
-  uint32_t lo[4], hi[4];
+  uint32_t u[8];
+  transpose4x4_bytes(d[0], d[2], d[4], d[6], u);
+  transpose4x4_bytes(d[1], d[3], d[5], d[7], u + 4);
+  const uint32_t m1 = opaque_mask(0x55555555u), m2 = opaque_mask(0x33333333u), m4 = opaque_mask(0x0F0F0F0Fu);
 #pragma unroll
-  for (int g = 0; g < 4; g++) {
-    lo[g] = d[2 * g];
-    hi[g] = d[2 * g + 1];
-    transpose8x8(lo[g], hi[g]);
-  }
-  transpose4x4_bytes(lo[0], lo[1], lo[2], lo[3], pl);
-  transpose4x4_bytes(hi[0], hi[1], hi[2], hi[3], pl + 4);
+  for (int t = 0; t < 8; t += 2) plane_swap<1>(u[t], u[t + 1], m1);
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+    if (!(t & 2)) plane_swap<2>(u[t], u[t + 2], m2);
+#pragma unroll
+  for (int t = 0; t < 4; t++) plane_swap<4>(u[t], u[t + 4], m4);
+#pragma unroll
+  for (int k = 0; k < 8; k++) pl[k] = u[k];
 }
 
 // Classifies 32 bytes into 32-bit masks from their bit planes: each class is a

@@ -135,6 +135,23 @@ __device__ __forceinline__ int64_t sext(unsigned long long v, int bits) {
   return ((int64_t)(v << (64 - bits))) >> (64 - bits);
 }
 
+// Accesses to the index arrays and the input through global-segment pointers.
+// A file's pointers reach a tile through LDS (its FileJob), which leaves them
+// generic, and generic accesses compile to FLAT instructions: those count
+// against the LDS wait counter as well, so every LDS wait would also wait for
+// them.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4_t g_u32x4_t;
+typedef __attribute__((address_space(1))) unsigned long long g_u64_t;
+typedef __attribute__((address_space(1))) long long g_i64_t;
+
+__device__ __forceinline__ uint4 gload16(const uint8_t* p) {
+  const u32x4_t v = *(const g_u32x4_t*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore64(uint64_t* p, uint64_t v) { *(g_u64_t*)p = v; }
+__device__ __forceinline__ void gstore64i(int64_t* p, int64_t v) { *(g_i64_t*)p = v; }
+
 // Two state words as one 16-byte access that bypasses the non-coherent caches
 // (volatile: sc0 sc1), so every CU sees the other CUs' published words.
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
@@ -506,9 +523,9 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
   // of the file were loaded as zeros, which are neither ':' nor '"')
   if (wr) {
     uint64_t* rec = J.rec + w * kRecWords;
-    rec[0] = S;
-    rec[1] = wc[r][kSlotK][tid];
-    rec[2] = wc[r][kSlotQ][tid];
+    gstore64(rec, S);
+    gstore64(rec + 1, wc[r][kSlotK][tid]);
+    gstore64(rec + 2, wc[r][kSlotQ][tid]);
   }
   if (VERIFY && act) {
     const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
@@ -522,7 +539,7 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
     levels_spread<NLVS>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
 #pragma unroll
     for (int i = 0; i < NLVS; i++)
-      if (wr && lo + i < J.L) J.LB[(int64_t)(lo + i) * J.Wp + w] = out[i];
+      if (wr && lo + i < J.L) gstore64(J.LB + (int64_t)(lo + i) * J.Wp + w, out[i]);
     // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
     // start at a multiple of 64 (lanes past the file end are inactive, and
     // lane 0 holds the lowest word, so it is active whenever any lane is)
@@ -531,7 +548,7 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
       for (int i = 0; i < NLVS; i++) {
         if (lo + i < J.L) {
           const unsigned long long m = __ballot(out[i] != 0);
-          if ((tid & 63) == 0 && act && J.LS) J.LS[(int64_t)(lo + i) * J.Ws + (w >> 6)] = m;
+          if ((tid & 63) == 0 && act && J.LS) gstore64(J.LS + (int64_t)(lo + i) * J.Ws + (w >> 6), m);
           if (fz.on && lo + i < kFuseMaxDepth) {
             if ((tid & 63) == 0) fz.sl[(lo + i) * fz.nsum + ((r * BLOCK + tid) >> 6)] = m;
             wc[r][fuse_lb_slot(lo + i)][tid] = out[i];
@@ -555,7 +572,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   static_assert(TILE_WORDS * 64 <= 65536, "lookback state fields assume tiles of at most 64 KiB (groups of 4 MiB)");
 
   __shared__ unsigned s_ticket;
-  __shared__ int s_job;
+  __shared__ FileJob s_J;          // the file of the block's current tile
   __shared__ int s_esc[ROWS][NW];  // last escape code that is not 0 of each (row, wave)
   __shared__ M4 s_m[ROWS][NW];     // transfer function of each (row, wave)
   __shared__ M4 s_pre[ROWS * NW];  // their exclusive prefixes in row-major order
@@ -598,28 +615,66 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   if (FUSE)
     for (int k = tid; k < kFuseKeyBytes; k += BLOCK) s_keys[k] = launch.fuse_keys[k];
 
+  // Thread 0 claims each tile one tile ahead: the ticket for the block's next
+  // tile is taken when its current tile starts, so the atomic's round trip
+  // overlaps the current tile instead of standing before the next one.  Claims
+  // stay in increasing order per block, so the lowest unfinished tile is always
+  // one a block is working on (its predecessors are all claimed, and finish
+  // first by the same argument): the lookback still cannot wait forever.
+  unsigned next_ticket = 0;  // thread 0 only
+  int cur_job = -1;          // thread 0 only: the job copied to s_J
+  if (tid == 0) next_ticket = atomicAdd(launch.ticket, 1u);
+  // This lane's input word of every row of a tile (bytes past the end of the
+  // file read as zeros).
+  uint4 raw[ROWS][4];
+  auto load_rows = [&](const FileJob& JJ, int64_t w0) {
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int64_t off = (w0 + (int64_t)r * BLOCK + tid) * 64;
+      if (off + 64 <= JJ.N) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) raw[r][q] = gload16(JJ.F + off + 16 * q);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          uint32_t w[4] = {0, 0, 0, 0};
+          for (int b = 0; b < 16; b++) {
+            const int64_t o = off + 16 * q + b;
+            if (o < JJ.N) w[b >> 2] |= (uint32_t)JJ.F[o] << (8 * (b & 3));
+          }
+          raw[r][q] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+      }
+    }
+  };
   for (;;) {
     if (tid == 0) {
-      s_ticket = (unsigned)launch.tile_begin + atomicAdd(launch.ticket, 1u);
+      const unsigned t = (unsigned)launch.tile_begin + next_ticket;
+      s_ticket = t;
       s_w0 = -1;
       s_w0_flip = 0;
       s_fnl = TILE_WORDS;
       s_lnl = -1;
       s_ncand = 0;
-      unsigned t = s_ticket;
-      int lo = 0, hi = (int)launch.njobs - 1;
-      if (t < launch.total_tiles && launch.jobs) {
-        while (lo < hi) {  // last job with tile0 <= t
-          int mid = (lo + hi + 1) >> 1;
-          if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
+      if (t < launch.total_tiles) {
+        next_ticket = atomicAdd(launch.ticket, 1u);
+        int lo = 0, hi = (int)launch.njobs - 1;
+        if (launch.jobs) {
+          while (lo < hi) {  // last job with tile0 <= t
+            int mid = (lo + hi + 1) >> 1;
+            if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
+          }
+        }
+        if (lo != cur_job) {  // a tile of another file (the first tile: always)
+          s_J = launch.jobs ? launch.jobs[lo] : launch.one;
+          cur_job = lo;
         }
       }
-      s_job = lo;
     }
     __syncthreads();
     const unsigned ticket = s_ticket;
     if (ticket >= launch.total_tiles) break;
-    const FileJob J = launch.jobs ? launch.jobs[s_job] : launch.one;
+    const FileJob J = s_J;
     const int64_t lt = (int64_t)ticket - J.tile0;
     const int64_t word0 = lt * (int64_t)TILE_WORDS;
 
@@ -628,26 +683,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     M4 wm0[ROWS];    // the word's transfer function for escape carry-in 0
     int flip[ROWS];  // the byte after the leading backslash run is a quote
     {
-      uint4 raw[ROWS][4];
-#pragma unroll
-      for (int r = 0; r < ROWS; r++) {
-        const int64_t off = (word0 + (int64_t)r * BLOCK + tid) * 64;
-        if (off + 64 <= J.N) {
-          const uint4* src = reinterpret_cast<const uint4*>(J.F + off);
-#pragma unroll
-          for (int q = 0; q < 4; q++) raw[r][q] = src[q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            uint32_t w[4] = {0, 0, 0, 0};
-            for (int b = 0; b < 16; b++) {
-              const int64_t o = off + 16 * q + b;
-              if (o < J.N) w[b >> 2] |= (uint32_t)J.F[o] << (8 * (b & 3));
-            }
-            raw[r][q] = make_uint4(w[0], w[1], w[2], w[3]);
-          }
-        }
-      }
+      load_rows(J, word0);
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
         uint32_t d[16];
@@ -1036,7 +1072,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       int64_t idx = (dl ? 0 : XL.lines) + texcl[r].nl;
       while (nlm) {
         const int b = ctz64(nlm);
-        if (idx < nl_lim) nl_out[idx] = w * 64 + b;
+        if (idx < nl_lim) gstore64i(nl_out + idx, w * 64 + b);
         idx++;
         nlm &= nlm - 1;
       }

@@ -26,6 +26,23 @@ namespace gj {
 namespace {
 
 __device__ __forceinline__ bool is_blank(uint8_t c) { return c == '\n' || c == '\r' || c == '\t' || c == ' '; }
+
+// Loads of the index arrays and the input through global-segment pointers.
+// The jobs' pointers come from a table in device memory and stay generic, and
+// generic loads compile to FLAT instructions, which also count against the
+// LDS wait counter: every read of the level slots in LDS would wait for them.
+typedef __attribute__((address_space(1))) const uint64_t g_cu64_t;
+typedef __attribute__((address_space(1))) const int64_t g_ci64_t;
+typedef __attribute__((address_space(1))) const uint8_t g_cu8_t;
+typedef unsigned int q_u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const q_u32x4_t g_cu32x4_t;
+__device__ __forceinline__ uint64_t gld64(const uint64_t* p) { return *(const g_cu64_t*)p; }
+__device__ __forceinline__ int64_t gldi64(const int64_t* p) { return *(const g_ci64_t*)p; }
+__device__ __forceinline__ uint8_t gld8(const uint8_t* p) { return *(const g_cu8_t*)p; }
+__device__ __forceinline__ uint4 gld16(const uint8_t* p) {
+  const q_u32x4_t v = *(const g_cu32x4_t*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ int ctz64(unsigned long long x) { return __builtin_ctzll(x); }
 // Bits [lo_bit, 63] of a word.
 __device__ __forceinline__ uint64_t mask_from(int lo_bit) { return ~0ull << lo_bit; }
@@ -85,10 +102,12 @@ struct GlobalWords {
   const uint64_t* rec;  // records: string, colon and raw-quote words
   const uint64_t* LS;   // level summaries (may be null: no skipping)
   int64_t Ws;
-  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return LB[(int64_t)lvl * Wp + wi]; }
-  __device__ __forceinline__ uint64_t s(int64_t wi) const { return rec_s(rec, wi); }
+  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return gld64(LB + (int64_t)lvl * Wp + wi); }
+  __device__ __forceinline__ uint64_t s(int64_t wi) const { return gld64(rec + wi * kRecWords); }
+  __device__ __forceinline__ uint64_t colon(int64_t wi) const { return gld64(rec + wi * kRecWords + 1); }
+  __device__ __forceinline__ uint64_t quote(int64_t wi) const { return gld64(rec + wi * kRecWords + 2); }
   __device__ __forceinline__ bool has_summary() const { return LS != nullptr; }
-  __device__ __forceinline__ uint64_t ls(int lvl, int64_t si) const { return LS[(int64_t)lvl * Ws + si]; }
+  __device__ __forceinline__ uint64_t ls(int lvl, int64_t si) const { return gld64(LS + (int64_t)lvl * Ws + si); }
 };
 
 // Summary words of level lvl for the blocks of 64 words that hold word nb and
@@ -189,13 +208,13 @@ __device__ bool key_matches(const uint8_t* F, const uint64_t* rec, const Words& 
   const int64_t ss = se - klen;
   if (ss < start) return false;
   const int64_t qi = ss >> 6;
-  const uint64_t qw = qi == wp ? qw_p : rec_quote(rec, qi);
+  const uint64_t qw = qi == wp ? qw_p : W.quote(qi);
   if (!((qw >> (ss & 63)) & 1ull)) return false;
   // compare 8 bytes per round trip (the loads of a chunk are independent)
   for (int i0 = 0; i0 < klen; i0 += 8) {
     uint8_t f[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) f[i] = i0 + i < klen ? F[ss + 1 + i0 + i] : 0;
+    for (int i = 0; i < 8; i++) f[i] = i0 + i < klen ? gld8(F + ss + 1 + i0 + i) : 0;
     bool same = true;
 #pragma unroll
     for (int i = 0; i < 8; i++) same &= !(i0 + i < klen) || f[i] == key[i0 + i];
@@ -224,9 +243,9 @@ __device__ int64_t group_find_key(const Group<G>& g, const Words& W, const uint6
       sw[u] = 0;
       qw[u] = 0;
       if (wi <= wend) {
-        v[u] = window_word(W, lvl, wi, wbeg, wend, from, to) & rec_colon(rec, wi);
-        sw[u] = rec_s(rec, wi);
-        qw[u] = rec_quote(rec, wi);
+        v[u] = window_word(W, lvl, wi, wbeg, wend, from, to) & W.colon(wi);
+        sw[u] = W.s(wi);
+        qw[u] = W.quote(wi);
       }
     }
     load_summary(W, lvl, nb, wend, sm0, sm1);
@@ -253,10 +272,9 @@ __device__ __forceinline__ uint64_t comma_word(const uint8_t* F, int64_t n, int6
   const int64_t off = wi << 6;
   uint32_t d[16];
   if (off + 64 <= n) {
-    const uint4* src = reinterpret_cast<const uint4*>(F + off);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const uint4 x = src[q];
+      const uint4 x = gld16(F + off + 16 * q);
       d[4 * q] = x.x;
       d[4 * q + 1] = x.y;
       d[4 * q + 2] = x.z;
@@ -265,7 +283,7 @@ __device__ __forceinline__ uint64_t comma_word(const uint8_t* F, int64_t n, int6
   } else {
 #pragma unroll
     for (int k = 0; k < 16; k++) d[k] = 0;
-    for (int b = 0; off + b < n && b < 64; b++) d[b >> 2] |= (uint32_t)F[off + b] << (8 * (b & 3));
+    for (int b = 0; off + b < n && b < 64; b++) d[b >> 2] |= (uint32_t)gld8(F + off + b) << (8 * (b & 3));
   }
   uint64_t m = 0;
 #pragma unroll
@@ -359,8 +377,8 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
   const bool lead = g.r == 0;
   const uint8_t* F = J.F;
   const int64_t n = J.N;
-  const int64_t start = J.NL[i];
-  const int64_t end = (i + 1 < lines) ? J.NL[i + 1] : n;
+  const int64_t start = gldi64(J.NL + i);
+  const int64_t end = (i + 1 < lines) ? gldi64(J.NL + i + 1) : n;
   if (lead)
     for (int k = 0; k < J.R * 2; k++) out[k] = -1;
   for (int k = g.r; k < nslots; k += G) { le[k] = -1; ci[k] = -1; }
@@ -385,7 +403,7 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
         case kEnd:
           return;
         case kStoreResult: {
-          while (j < n && is_blank(F[j])) j++;
+          while (j < n && is_blank(gld8(F + j))) j++;
           if (r < J.R && lead) {
             out[r * 2] = j;
             out[r * 2 + 1] = le[lvl];
@@ -420,8 +438,8 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
         case kMoveToIndex: {
           want_idx = read_varint(ir, J.ir_len, &pc);
           if (ci[lvl] == -1) {
-            while (j < n && F[j] != '[') {
-              if (!is_blank(F[j])) return;  // not an array: the line has no further results
+            while (j < n && gld8(F + j) != '[') {
+              if (!is_blank(gld8(F + j))) return;  // not an array: the line has no further results
               j++;
             }
             if (j >= n) return;
@@ -434,10 +452,10 @@ __device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, 
           want_len = read_varint(ir, J.ir_len, &pc);
           want = ir + pc;
           pc += want_len;
-          while (j < n && is_blank(F[j])) j++;
+          while (j < n && is_blank(gld8(F + j))) j++;
           if (le[lvl] - j != want_len) return;
           for (int k = 0; k < want_len; k++)
-            if (want[k] != F[j + k]) return;
+            if (want[k] != gld8(F + j + k)) return;
           break;
         }
         default:
