@@ -10,8 +10,8 @@ constexpr int kMaxLevels = 64;
 // Per-tile lookback state: self-validating 64-bit words (bit 63 = written), so a
 // reader never needs a fence between a flag and its payload.  The tile's
 // aggregate, its inclusive prefix and (for the last tile of each group of 64
-// tiles) the group's aggregate are each two words, written and read as one
-// 16-byte access.  One 64-byte line per tile.
+// tiles) the group's aggregate are each two words; a reader takes a pair once
+// both words are valid.  One 64-byte line per tile.
 struct alignas(64) TileState {
   unsigned long long agg0, agg1;  // the tile's aggregate (packed counts, see the index kernel)
   unsigned long long inc0;        // valid | p(62) | s(61) | depth (signed 48 bits)

@@ -152,19 +152,25 @@ __device__ __forceinline__ uint4 gload16(const uint8_t* p) {
 __device__ __forceinline__ void gstore64(uint64_t* p, uint64_t v) { *(g_u64_t*)p = v; }
 __device__ __forceinline__ void gstore64i(int64_t* p, int64_t v) { *(g_i64_t*)p = v; }
 
-// Two state words as one 16-byte access that bypasses the non-coherent caches
-// (volatile: sc0 sc1), so every CU sees the other CUs' published words.
+// Two state words, each a relaxed atomic at device scope: every CU of every
+// XCD sees the words other CUs published, and a store does not wait for its
+// write to complete (a volatile access would, and the publishing wavefront
+// would hold its block's other wavefronts at the next barrier meanwhile).  No
+// ordering is needed: each word validates itself (bit 63), and a reader takes
+// a pair only when both words are valid.
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ u64x2 ld_state2(const unsigned long long* p) {
-  return *reinterpret_cast<const volatile u64x2*>(p);
+  unsigned long long* q = const_cast<unsigned long long*>(p);
+  u64x2 v;
+  v.x = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  v.y = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return v;
 }
 
 __device__ __forceinline__ void st_state2(unsigned long long* p, unsigned long long w0, unsigned long long w1) {
-  u64x2 v;
-  v.x = w0;
-  v.y = w1;
-  *reinterpret_cast<volatile u64x2*>(p) = v;
+  __hip_atomic_store(p, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Aggregate of a tile (at most 64 KiB) or of a group of 64 tiles (at most
@@ -571,19 +577,25 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   static_assert(BLOCK % 64 == 0, "a wavefront's words must start at a multiple of 64 (level summaries)");
   static_assert(TILE_WORDS * 64 <= 65536, "lookback state fields assume tiles of at most 64 KiB (groups of 4 MiB)");
 
-  __shared__ unsigned s_ticket;
-  __shared__ FileJob s_J;          // the file of the block's current tile
+  // Per-tile scalars, two sets used by alternate tiles: thread 0 prepares the
+  // block's next tile (its ticket, its file, the cleared fields) before the
+  // barrier that ends the current tile, so a tile starts without a barrier.
+  struct TileSlot {
+    unsigned ticket;
+    int w0;       // sequence index of the tile's first word that is not all '\\'
+    int w0_flip;
+    int fnl;      // sequence index of the tile's first word holding a '\n'
+    int lnl;      // sequence index of the tile's last word holding a '\n' (in-tile evaluation)
+    int ncand;    // key candidates whose bytes are still to be compared (in-tile evaluation)
+    M4 w0_m;
+  };
+  __shared__ TileSlot s_slot[2];
+  __shared__ FileJob s_J[2];       // the file of each set's tile
   __shared__ int s_esc[ROWS][NW];  // last escape code that is not 0 of each (row, wave)
   __shared__ M4 s_m[ROWS][NW];     // transfer function of each (row, wave)
-  __shared__ M4 s_pre[ROWS * NW];  // their exclusive prefixes in row-major order
-  __shared__ M4 s_mtile;           // the tile's transfer function
   static_assert(ROWS * NW <= 16, "the (row, wave) prefixes are scanned within one DPP row");
-  __shared__ int s_w0;             // sequence index of the tile's first word that is not all '\\'
-  __shared__ int s_w0_flip;
-  __shared__ M4 s_w0_m;
   __shared__ Carry s_carry;
   __shared__ Carry s_exact;        // carry-in from the lookback (speculative tiles)
-  __shared__ int s_fnl;            // sequence index of the tile's first word holding a '\n'
   __shared__ int s_mis;            // the speculative carry-in was wrong
   // the masks every word needs at emission, parked here (field-major, so the
   // lanes of a wavefront touch consecutive 8-byte slots) while the scans and
@@ -602,8 +614,6 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   constexpr int NSUM = TILE_WORDS / 64;
   __shared__ uint8_t s_keys[FUSE ? kFuseKeyBytes : 1];
   __shared__ uint64_t s_fsum[FUSE ? 2 * kFuseMaxDepth * NSUM : 1];  // [match | level][l][NSUM]
-  __shared__ int s_lnl;    // sequence index of the tile's last word holding a '\n'
-  __shared__ int s_ncand;  // key candidates whose bytes are still to be compared
   static_assert(!FUSE || TILE_WORDS / 3 <= NW * kBracketCap, "the newline positions fit the bracket lists");
 
   const int tid = threadIdx.x;
@@ -621,9 +631,47 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
   // stay in increasing order per block, so the lowest unfinished tile is always
   // one a block is working on (its predecessors are all claimed, and finish
   // first by the same argument): the lookback still cannot wait forever.
-  unsigned next_ticket = 0;  // thread 0 only
-  int cur_job = -1;          // thread 0 only: the job copied to s_J
-  if (tid == 0) next_ticket = atomicAdd(launch.ticket, 1u);
+  unsigned next_ticket = 0;            // thread 0 only
+  int cur_job = -1;                    // thread 0 only: the job of the last tile prepared
+  int64_t cur_end = 0;                 // thread 0 only: its first tile past the job
+  int slot_job[2] = {-1, -1};          // thread 0 only: the job each s_J set holds
+  auto prepare = [&](int k) {          // thread 0: set k for the block's next tile
+    const unsigned t = (unsigned)launch.tile_begin + next_ticket;
+    TileSlot& q = s_slot[k];
+    q.ticket = t;
+    q.w0 = -1;
+    q.w0_flip = 0;
+    q.fnl = TILE_WORDS;
+    q.lnl = -1;
+    q.ncand = 0;
+    if (t >= launch.total_tiles) return;
+    next_ticket = atomicAdd(launch.ticket, 1u);
+    int lo = 0;
+    if (launch.jobs) {
+      if (cur_job >= 0 && (int64_t)t < cur_end) {
+        lo = cur_job;  // tickets grow, so the tile is in the last job or a later one
+      } else {
+        int hi = (int)launch.njobs - 1;
+        while (lo < hi) {  // last job with tile0 <= t
+          const int mid = (lo + hi + 1) >> 1;
+          if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
+        }
+      }
+    }
+    if (lo != slot_job[k]) {  // a tile of another file than the set's last one
+      s_J[k] = launch.jobs ? launch.jobs[lo] : launch.one;
+      slot_job[k] = lo;
+    }
+    if (lo != cur_job) {
+      cur_job = lo;
+      cur_end = s_J[k].tile0 + s_J[k].ntiles;
+    }
+  };
+  if (tid == 0) {
+    next_ticket = atomicAdd(launch.ticket, 1u);
+    prepare(0);
+  }
+  __syncthreads();
   // This lane's input word of every row of a tile (bytes past the end of the
   // file read as zeros).
   uint4 raw[ROWS][4];
@@ -647,34 +695,11 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       }
     }
   };
-  for (;;) {
-    if (tid == 0) {
-      const unsigned t = (unsigned)launch.tile_begin + next_ticket;
-      s_ticket = t;
-      s_w0 = -1;
-      s_w0_flip = 0;
-      s_fnl = TILE_WORDS;
-      s_lnl = -1;
-      s_ncand = 0;
-      if (t < launch.total_tiles) {
-        next_ticket = atomicAdd(launch.ticket, 1u);
-        int lo = 0, hi = (int)launch.njobs - 1;
-        if (launch.jobs) {
-          while (lo < hi) {  // last job with tile0 <= t
-            int mid = (lo + hi + 1) >> 1;
-            if (launch.jobs[mid].tile0 <= (int64_t)t) lo = mid; else hi = mid - 1;
-          }
-        }
-        if (lo != cur_job) {  // a tile of another file (the first tile: always)
-          s_J = launch.jobs ? launch.jobs[lo] : launch.one;
-          cur_job = lo;
-        }
-      }
-    }
-    __syncthreads();
-    const unsigned ticket = s_ticket;
+  for (int par = 0;; par ^= 1) {
+    TileSlot& ts = s_slot[par];
+    const unsigned ticket = ts.ticket;
     if (ticket >= launch.total_tiles) break;
-    const FileJob J = s_J;
+    const FileJob J = s_J[par];
     const int64_t lt = (int64_t)ticket - J.tile0;
     const int64_t word0 = lt * (int64_t)TILE_WORDS;
 
@@ -717,8 +742,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
         s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
         if (!VERIFY) {
           const unsigned long long nlw = __ballot(w.NL != 0);
-          if (nlw && lane == __builtin_ctzll(nlw)) atomicMin(&s_fnl, r * BLOCK + tid);
-          if (FUSE && nlw && lane == 63 - __builtin_clzll(nlw)) atomicMax(&s_lnl, r * BLOCK + tid);
+          if (nlw && lane == __builtin_ctzll(nlw)) atomicMin(&ts.fnl, r * BLOCK + tid);
+          if (FUSE && nlw && lane == 63 - __builtin_clzll(nlw)) atomicMax(&ts.lnl, r * BLOCK + tid);
         }
       }
     }
@@ -771,9 +796,9 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       }
       if (ent[r] == 0 && code[r] != 0) {
         // the tile's first word that is not all backslashes
-        s_w0 = r * BLOCK + tid;
-        s_w0_flip = flip[r];
-        s_w0_m = wm[r];
+        ts.w0 = r * BLOCK + tid;
+        ts.w0_flip = flip[r];
+        ts.w0_m = wm[r];
       }
     }
 
@@ -786,23 +811,27 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       if (lane == 63) s_m[r][wave] = inc;
     }
     __syncthreads();
-    // the (row, wave) totals' exclusive prefixes, scanned once by the first
-    // lanes of wave 0 (within one 16-lane DPP row)
-    if (wave == 0) {
+    // the (row, wave) totals' exclusive prefixes: every wavefront scans them
+    // itself (within one 16-lane DPP row), so no barrier waits for one of them
+    M4 texcl[ROWS];
+    M4 mtile;
+    {
       M4 v = lane < ROWS * NW ? s_m[lane / NW][lane % NW] : m_id();
       v = m_cat(dpp_m<0x111, 0xF>(v), v);
       v = m_cat(dpp_m<0x112, 0xF>(v), v);
       v = m_cat(dpp_m<0x114, 0xF>(v), v);
       v = m_cat(dpp_m<0x118, 0xF>(v), v);
       const M4 ex = lane_below_m(v);
-      if (lane < ROWS * NW) s_pre[lane] = ex;
-      if (lane == ROWS * NW - 1) s_mtile = v;
-    }
-    __syncthreads();
-    M4 texcl[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; r++) texcl[r] = m_cat(s_pre[r * NW + wave], lex[r]);
-    const M4 mtile = s_mtile;
+      for (int r = 0; r < ROWS; r++) {
+        const int k = __builtin_amdgcn_readfirstlane(r * NW + wave);
+        const M4 e{__builtin_amdgcn_readlane(ex.q, k), __builtin_amdgcn_readlane(ex.d0, k),
+                   __builtin_amdgcn_readlane(ex.tot, k), __builtin_amdgcn_readlane(ex.nl, k)};
+        texcl[r] = m_cat(e, lex[r]);
+      }
+      mtile = M4{__builtin_amdgcn_readlane(v.q, ROWS * NW - 1), __builtin_amdgcn_readlane(v.d0, ROWS * NW - 1),
+                 __builtin_amdgcn_readlane(v.tot, ROWS * NW - 1), __builtin_amdgcn_readlane(v.nl, ROWS * NW - 1)};
+    }
 
     // ---- carry-in: speculated at the first newline, or from the lookback ----
     // In NDJSON every raw '\n' lies outside strings at depth -1 (one value per
@@ -812,12 +841,12 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     // at once, and resolves its exact carry-in with the lookback afterwards:
     // only the newline offsets need it.  A wrong guess (the input is not
     // NDJSON there) re-emits the tile with the exact carry-in.
-    const bool spec = launch.speculate && !VERIFY && lt > 0 && s_fnl < TILE_WORDS && s_lead[0][0] == 0;
+    const bool spec = launch.speculate && !VERIFY && lt > 0 && ts.fnl < TILE_WORDS && s_lead[0][0] == 0;
     TileAgg a;
     a.esc = tile_esc;
     a.m[0] = mtile;
     a.m[1] = mtile;
-    if (s_w0 >= 0 && s_w0_flip) a.m[1] = m_cat(m_flip(s_w0_m), m_left_div(s_w0_m, mtile));
+    if (ts.w0 >= 0 && ts.w0_flip) a.m[1] = m_cat(m_flip(ts.w0_m), m_left_div(ts.w0_m, mtile));
     TileState* st0 = launch.states + J.tile0;
     if (wave == 0) {
       // every tile publishes its aggregate (tile 0's too: the last tile of
@@ -841,7 +870,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     if (spec) {
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
-        if (r * BLOCK + tid == s_fnl) {
+        if (r * BLOCK + tid == ts.fnl) {
           // state at the word's first byte that makes its first '\n' lie
           // outside a string at depth -1
           const int ld = s_lead[r][tid];
@@ -862,9 +891,9 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     }
     __syncthreads();
     const Carry X = s_carry;
-    const int w0 = s_w0;
-    const bool fix = (X.p == 1) && w0 >= 0 && s_w0_flip;
-    const M4 w0m = s_w0_m;
+    const int w0 = ts.w0;
+    const bool fix = (X.p == 1) && w0 >= 0 && ts.w0_flip;
+    const M4 w0m = ts.w0_m;
     if (dl && tid == 0) {
       // deferred bookkeeping: a speculated tile publishes its inclusive prefix
       // now and never looks back (index_finish checks every carry-in against
@@ -878,7 +907,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     // every line inside it (FUSE); it then writes its index words only around
     // the lines that cross into its neighbours.
     const bool ft = FUSE && dl && launch.fuse_depth > 0 && mtile.nl <= fused_line_cap(TILE_WORDS);
-    const int fnl = s_fnl, lnl = s_lnl;
+    const int fnl = ts.fnl, lnl = ts.lnl;
     auto emit_tile = [&](const Carry& cx, bool fx) {
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
@@ -999,7 +1028,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
           if (m) {
             const int first = __builtin_ctzll(m);
             int base = 0;
-            if (lane == first) base = atomicAdd(&s_ncand, __popcll(m));
+            if (lane == first) base = atomicAdd(&ts.ncand, __popcll(m));
             base = __shfl(base, first);
             if (ss >= 0) {
               const int slot = base + __popcll(m & lanes_below);
@@ -1028,7 +1057,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
       __syncthreads();
       // (2) the candidates' key bytes, one candidate per lane: one round trip
       // to memory for the whole tile
-      const int ncand = min(s_ncand, CCAP);
+      const int ncand = min(ts.ncand, CCAP);
       for (int i = tid; i < ncand; i += BLOCK) {
         const uint64_t e = cands[i];
         const int pc = (int)(e & 0xFFFF), ss = (int)((e >> 16) & 0xFFFF), l = (int)(e >> 32);
@@ -1077,6 +1106,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
         nlm &= nlm - 1;
       }
     }
+    if (tid == 0) prepare(par ^ 1);
     __syncthreads();
   }
 }

@@ -5,7 +5,9 @@
 // wavefront's 64 lanes as a loop, ballots as masks) over a non-commutative
 // monoid (affine maps mod 2^61 - 1), with random delays at every step, and
 // checks that every launch ends and every tile's carry-in equals the
-// sequential prefix.  Launch patterns:
+// sequential prefix.  A published pair of state words is one event here (the
+// kernel's reader takes a pair only once both words are valid).  Launch
+// patterns:
 //   - one launch over all tiles (the plain and batched index builds);
 //   - several launches over consecutive tile ranges [tile_begin, end) with the
 //     ticket reset before each, the tile states kept (the staging pipeline);
@@ -174,13 +176,17 @@ void publish_inc(Model& m, int64_t lt, uint64_t v) {
 }
 
 // One block of the persistent grid: claims tiles of [begin, end) until none
-// are left (the kernel's loop body, minus the data work).
+// are left (the kernel's loop body, minus the data work).  As in the kernel,
+// the ticket for the block's next tile is claimed when its current tile
+// starts, so a block always holds one claimed tile it has not started.
 void block_main(Model& m, int64_t begin, int64_t end, uint64_t seed) {
   std::mt19937_64 rng(seed);
+  int64_t next = begin + (int64_t)m.ticket.fetch_add(1);
   for (;;) {
     if (m.abort.load()) return;
-    const int64_t lt = begin + (int64_t)m.ticket.fetch_add(1);
+    const int64_t lt = next;
     if (lt >= end) return;
+    next = begin + (int64_t)m.ticket.fetch_add(1);
     maybe_delay(m, rng);
     const Agg own = m.tile_agg[lt];
     // every tile publishes its aggregate first
