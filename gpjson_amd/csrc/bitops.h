@@ -129,14 +129,7 @@ GJ_HD void plane_swap(uint32_t& a, uint32_t& b, uint32_t m) {
   b = nb;
 }
 
-// Bit planes of 32 bytes (d[0..7], byte b of d[i] = position 4i+b): bit j of
-// pl[k] is bit k of byte j.  A bit of byte j (j = j4..j0) starts in register
-// j >> 2 at position 8*(j & 3) + k.  Two 4x4 byte transposes put byte
-// j = t + 8s in slot s of register t, so the position reads (j4 j3 k2 k1 k0)
-// and the register index (j2 j1 j0); three butterfly stages then exchange k0,
-// k1, k2 with j0, j1, j2, which leaves bit k of byte j in register k at
-// position j.  (48 shifts and bitfield inserts and 16 byte permutes per 32
-// bytes, against 84 + 16 for an 8x8 bit transpose of each 8-byte group.)
+// A mask constant the optimizer cannot see through (device code: an SGPR).
 GJ_HD uint32_t opaque_mask(uint32_t m) {
 #if defined(__HIP_DEVICE_COMPILE__)
   // an SGPR whose value the optimizer does not see: it would otherwise move
@@ -146,6 +139,14 @@ GJ_HD uint32_t opaque_mask(uint32_t m) {
   return m;
 }
 
+// Bit planes of 32 bytes (d[0..7], byte b of d[i] = position 4i+b): bit j of
+// pl[k] is bit k of byte j.  A bit of byte j (j = j4..j0) starts in register
+// j >> 2 at position 8*(j & 3) + k.  Two 4x4 byte transposes put byte
+// j = t + 8s in slot s of register t, so the position reads (j4 j3 k2 k1 k0)
+// and the register index (j2 j1 j0); three butterfly stages then exchange k0,
+// k1, k2 with j0, j1, j2, which leaves bit k of byte j in register k at
+// position j.  (48 shifts and bitfield inserts and 16 byte permutes per 32
+// bytes, against 84 + 16 for an 8x8 bit transpose of each 8-byte group.)
 GJ_HD void bit_planes32(const uint32_t d[8], uint32_t pl[8]) {
   uint32_t u[8];
   transpose4x4_bytes(d[0], d[2], d[4], d[6], u);
@@ -210,9 +211,9 @@ GJ_HD WordClass classify64(const uint32_t d[16]) {
 // (J/sequential/Sequential.java:84-100): '{' '[' increment then mark, '}' ']'
 // mark then decrement, ':' ',' mark at the current depth.  O, C, P are already
 // restricted to bytes outside strings.  Returns the depth after the word.
-template <int NLV>
-GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L,
-                           uint64_t out[NLV]) {
+// D: the depth's integer type (the kernel passes a depth clamped to 32 bits).
+template <int NLV, class D>
+GJ_HD D leveled_word(uint64_t O, uint64_t C, uint64_t P, D d, int lo, int L, uint64_t out[NLV]) {
 #pragma unroll
   for (int k = 0; k < NLV; k++) out[k] = 0;
   uint64_t T = O | C;
@@ -223,14 +224,14 @@ GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo
     uint64_t before = rem & (bit - 1);  // separators strictly before t
     uint64_t seps = P & before;
     int is_open = (O & bit) != 0;
-    int64_t lvl_open = d + 1;
+    D lvl_open = d + 1;
     // separators before t and a closing bracket at t sit at depth d; an
     // opening bracket at t sits at depth d + 1.
     uint64_t at_d = seps | (is_open ? 0 : bit);
     uint64_t at_d1 = is_open ? bit : 0;
 #pragma unroll
     for (int k = 0; k < NLV; k++) {
-      int64_t lv = lo + k;
+      D lv = lo + k;
       if (lv < L) {
         out[k] |= (lv == d) ? at_d : 0;
         out[k] |= (lv == lvl_open) ? at_d1 : 0;
@@ -243,7 +244,7 @@ GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo
   uint64_t seps = P & rem;
 #pragma unroll
   for (int k = 0; k < NLV; k++) {
-    int64_t lv = lo + k;
+    D lv = lo + k;
     if (lv < L) out[k] |= (lv == d) ? seps : 0;
   }
   return d;
@@ -255,10 +256,11 @@ GJ_HD int64_t leveled_word(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo
 // before t); the separators after it, up to the next bracket, sit at the depth
 // after it (that level for an open, one less for a close).  Returns those
 // separators.
-GJ_HD uint64_t bracket_step(uint64_t O, uint64_t C, uint64_t P, int64_t d, int t, int64_t* lvl, int64_t* after) {
+template <class D>
+GJ_HD uint64_t bracket_step(uint64_t O, uint64_t C, uint64_t P, D d, int t, D* lvl, D* after) {
   const uint64_t bit = 1ull << t;
   const uint64_t le = bit | (bit - 1);
-  const int64_t l = d + popc64(O & le) - popc64(C & (bit - 1));
+  const D l = d + popc64(O & le) - popc64(C & (bit - 1));
   *lvl = l;
   *after = (O & bit) ? l : l - 1;
   const uint64_t rest = (O | C) & ~le;

@@ -80,21 +80,64 @@ __device__ __forceinline__ M4 dpp_m(const M4& v) {
             __builtin_amdgcn_update_dpp(0, v.nl, CTRL, ROWMASK, 0xF, false)};
 }
 
-// Inclusive scan of M4 over the wavefront in lane order: prefixes within rows
-// of 16 lanes, then row 0's total into row 1 and row 2's into row 3, then rows
-// 0-1's total into rows 2-3.  VALU lane moves, no LDS traffic.
-__device__ __forceinline__ M4 wave_scan_m(M4 x) {
-  x = m_cat(dpp_m<0x111, 0xF>(x), x);
-  x = m_cat(dpp_m<0x112, 0xF>(x), x);
-  x = m_cat(dpp_m<0x114, 0xF>(x), x);
-  x = m_cat(dpp_m<0x118, 0xF>(x), x);
-  x = m_cat(dpp_m<0x142, 0xA>(x), x);
-  x = m_cat(dpp_m<0x143, 0xC>(x), x);
+// The value of the lane below (lane 0: the identity).
+__device__ __forceinline__ M4 lane_below_m(const M4& x) { return dpp_m<0x138, 0xF>(x); }
+
+// Inclusive prefix sum of an int within each row of 16 lanes (DPP row shifts).
+__device__ __forceinline__ int row16_add_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
   return x;
 }
 
-// The value of the lane below (lane 0: the identity).
-__device__ __forceinline__ M4 lane_below_m(const M4& x) { return dpp_m<0x138, 0xF>(x); }
+// Inclusive prefix sum of an int over the wavefront in lane order (DPP row
+// shifts, then row broadcasts).
+__device__ __forceinline__ int wave_add_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
+// Transfer functions composed in lane order, over the wavefront (or, ROW16,
+// within each row of 16 lanes): returns this lane's exclusive prefix, *incl
+// its inclusive one.  Add-scans instead of a scan of M4: the first sums quote
+// parities and newlines (packed in one int: quote counts below 2^8, newlines
+// at most 2^16 per tile), which gives each word the string state it is entered
+// with from the start of the scan; the depth change for that state is then
+// summed with the net depth.  Entered outside a string, the prefix's d0 is
+// exactly that sum.  Over a wavefront both depths stay within +-2^12 and share
+// one int (low half sign-extended); over a tile's rows they are scanned apart.
+template <bool ROW16>
+__device__ __forceinline__ M4 scan_words(const M4& w, M4* incl) {
+  auto scan = [](int x) { return ROW16 ? row16_add_scan(x) : wave_add_scan(x); };
+  const int a = (w.q & 1) + (w.nl << 8);
+  const int ia = scan(a);
+  const int ea = ia - a;
+  const int d = (ea & 1) ? (w.tot - w.d0) : w.d0;  // entered inside a string: the other depth change
+  int it, et, id, ed;
+  if (ROW16) {
+    it = scan(w.tot);
+    et = it - w.tot;
+    id = scan(d);
+    ed = id - d;
+  } else {
+    const int b = w.tot + d * 65536;
+    const int ib = scan(b);
+    const int eb = ib - b;
+    it = (int)(short)(ib & 0xFFFF);
+    id = (ib - it) >> 16;
+    et = (int)(short)(eb & 0xFFFF);
+    ed = (eb - et) >> 16;
+  }
+  *incl = M4{ia & 1, id, it, (int)((unsigned)ia >> 8)};
+  return M4{ea & 1, ed, et, (int)((unsigned)ea >> 8)};
+}
 
 // Carry state at a position.
 struct Carry {
@@ -389,7 +432,7 @@ constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlo
 constexpr int kBracketCap = 128;
 constexpr int kSpreadLevels = 4;  // levels per pass over the brackets
 
-// Inclusive prefix sum of an int over the wavefront (DPP, as wave_scan_m).
+// Inclusive prefix sum of an int over the wavefront (DPP, as wave_add_scan).
 __device__ __forceinline__ int wave_scan_int(int x) {
   x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
   x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
@@ -413,13 +456,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Level words [lo, lo + NLV) of this lane's word (O, C, P outside strings, d
-// the depth before it; act false: no word), every lane of the wavefront
-// taking part.  lv: this wavefront's NLV x 64 level words; items: its list.
+// the depth before it, clamped to [-66, L + 66]; act false: no word), every
+// lane of the wavefront taking part.  lv: this wavefront's NLV x 64 level
+// words; items: its list.
 template <int NLV>
-__device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P, int64_t d, int lo, int L,
+__device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P, int d, int lo, int L,
                                               bool act, int lane, uint64_t (&lv)[NLV][64],
                                               uint16_t (&items)[kBracketCap], uint64_t out[NLV]) {
-  const int64_t dmin = d - popc64(C), dmax = d + popc64(O);
+  const int dmin = d - popc64(C), dmax = d + popc64(O);
   const bool hit = act && !(dmax < lo || dmin >= lo + NLV) && (O | C | P) != 0;
   const uint64_t T = hit ? (O | C) : 0;
   const int n = popc64(T);
@@ -435,7 +479,7 @@ __device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P
 #pragma unroll
   for (int i = 0; i < NLV; i++) lv[i][lane] = 0;
   if (hit) {
-    const int64_t k = d - lo;
+    const int k = d - lo;
     if (k >= 0 && k < NLV && d < L) lv[k][lane] = leading_seps(O, C, P);
   }
   // the work list: this word's brackets at [incl - n, incl)
@@ -453,13 +497,13 @@ __device__ __forceinline__ void levels_spread(uint64_t O, uint64_t C, uint64_t P
     const int e = u < total ? items[u] : 0;
     const int o = e >> 6, t = e & 63;
     const uint64_t Oo = shfl_u64(O, o), Co = shfl_u64(C, o), Po = shfl_u64(P, o);
-    const int64_t dd = (int64_t)shfl_u64((uint64_t)d, o);
+    const int dd = __shfl(d, o);
     if (u < total) {
-      int64_t lvl, after;
+      int lvl, after;
       const uint64_t seps = bracket_step(Oo, Co, Po, dd, t, &lvl, &after);
-      const int64_t k = lvl - lo;
+      const int k = lvl - lo;
       if (k >= 0 && k < NLV && lvl < L) atomicOr(&lv[k][o], 1ull << t);
-      const int64_t k2 = after - lo;
+      const int k2 = after - lo;
       if (seps && k2 >= 0 && k2 < NLV && after < L) atomicOr(&lv[k2][o], seps);
     }
   }
@@ -540,9 +584,13 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
   }
   const uint64_t O = wc[r][kSlotO][tid] & ~S, C = wc[r][kSlotC][tid] & ~S, P = wc[r][kSlotP][tid] & ~S;
   if (fz.on) wc[r][kSlotS0][tid] = S;  // only this lane reads its slots until the next barrier
+  // A word moves the depth by at most 64, so a depth below -66 or above L + 66
+  // puts every byte of the word below level 0 or at level L or above either
+  // way: clamped there it fits 32 bits and leaves the level bits unchanged.
+  const int dc = (int)(d < -66 ? -66 : (d > (int64_t)J.L + 66 ? (int64_t)J.L + 66 : d));
   for (int lo = 0; lo < J.L; lo += NLVS) {
     uint64_t out[NLVS];
-    levels_spread<NLVS>(O, C, P, d, lo, J.L, act, tid & 63, lv, items, out);
+    levels_spread<NLVS>(O, C, P, dc, lo, J.L, act, tid & 63, lv, items, out);
 #pragma unroll
     for (int i = 0; i < NLVS; i++)
       if (wr && lo + i < J.L) gstore64(J.LB + (int64_t)(lo + i) * J.Wp + w, out[i]);
@@ -806,8 +854,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     M4 lex[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
-      const M4 inc = wave_scan_m(wm[r]);
-      lex[r] = lane_below_m(inc);
+      M4 inc;
+      lex[r] = scan_words<false>(wm[r], &inc);
       if (lane == 63) s_m[r][wave] = inc;
     }
     __syncthreads();
@@ -816,12 +864,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY) ? 3 :
     M4 texcl[ROWS];
     M4 mtile;
     {
-      M4 v = lane < ROWS * NW ? s_m[lane / NW][lane % NW] : m_id();
-      v = m_cat(dpp_m<0x111, 0xF>(v), v);
-      v = m_cat(dpp_m<0x112, 0xF>(v), v);
-      v = m_cat(dpp_m<0x114, 0xF>(v), v);
-      v = m_cat(dpp_m<0x118, 0xF>(v), v);
-      const M4 ex = lane_below_m(v);
+      M4 v;
+      const M4 ex = scan_words<true>(lane < ROWS * NW ? s_m[lane / NW][lane % NW] : m_id(), &v);
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
         const int k = __builtin_amdgcn_readfirstlane(r * NW + wave);
