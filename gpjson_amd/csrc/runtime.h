@@ -61,13 +61,12 @@ int hip_fail(hipError_t e, const char* what);
     if (gj_e_ != hipSuccess) return hip_fail(gj_e_, #expr); \
   } while (0)
 
-// Default index tile: 256 threads x 3 rows (48 KiB).  On 4 GiB of tweets it
-// measured 4.39 / 4.26 / 4.42 / 4.65 ms for L = 2, 1, 2, 3 against 4.48 / 4.28 /
-// 4.47 / 4.89 ms with 4 rows (2 blocks per CU by LDS and 187 VGPRs; 3 rows fit
-// 3 blocks) and 5.1-5.5 ms with 2 rows.
+// Default index tile: 256 threads x 2 rows (32 KiB), 4 blocks per CU.  On 4 GiB
+// of tweets it indexed in 1.72 / 1.68 / 1.73 / 1.78 ms for L = 2, 1, 2, 3
+// against 1.79 / 1.73 / 1.79 / 1.88 ms with 3 rows (3 blocks per CU by LDS).
 struct IndexTune {
   int block = 256;
-  int wpt = 3;
+  int wpt = 2;
 };
 
 // Interpreter modes: one lane group per line (the general interpreter);
