@@ -179,7 +179,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
     J.F = b->bufs[f]->d;
     J.rec = (uint64_t*)b->arena_rec.p + s_off[f] * kRecWords;
     J.LB = (uint64_t*)b->arena_lb.p + lb_off[f];
-    J.LS = J.LB + J.Wp * std::max(J.L, 1);
+    J.LS = summaries_of(J.LB, J.Wp, J.L);
     J.Ws = summary_stride(J.Wp);
     J.NL = (int64_t*)b->arena_nl.p + nl_off[f];
     J.E = nullptr;

@@ -19,29 +19,36 @@ struct alignas(64) TileState {
   unsigned long long pad[4];      // [0..1]: group aggregate (last tile of a group)
 };
 
-// Level summaries: bit k of summary word s of level l is set iff leveled word
-// 64*s + k of level l is not zero, so a scan over a level can skip 64 words
-// (4 KB of input) with one load where the level has no structure.  They follow
-// the L levels in the same allocation, Ws words per level.
-#if defined(__HIPCC__)
-__host__ __device__
-#endif
-inline int64_t summary_stride(int64_t Wp) { return (Wp + 63) / 64; }
-// Words of a leveled-bitmap allocation (levels, then their summaries).
-inline int64_t leveled_alloc_words(int64_t Wp, int L) { return (Wp + summary_stride(Wp)) * (L > 1 ? L : 1); }
-
-// The string, colon and raw-quote words of a 64-byte input word form one
-// 24-byte record, so an interpreter step that needs all three touches one
-// cache line instead of three.
-constexpr int kRecWords = 3;
 #if defined(__HIPCC__)
 #define GJ_REC_HD __host__ __device__ __forceinline__
 #else
 #define GJ_REC_HD static inline
 #endif
+
+// The string, colon, raw-quote and level-0 words of a 64-byte input word form
+// one 32-byte record, so an interpreter step at level 0 touches one cache line
+// for all four (four records per 128-byte line).  Levels 1 .. L-1 of the
+// leveled bitmaps are level-major arrays of Wp words each; most of a member
+// chain's walk is at level 0, and a step at a deeper level adds one line.
+constexpr int kRecWords = 4;
+constexpr int kRecLevel0 = 3;  // word of the record holding leveled bitmap 0
 GJ_REC_HD uint64_t rec_s(const uint64_t* R, int64_t w) { return R[w * kRecWords]; }      // in-string bits
 GJ_REC_HD uint64_t rec_colon(const uint64_t* R, int64_t w) { return R[w * kRecWords + 1]; }  // ':' bytes
 GJ_REC_HD uint64_t rec_quote(const uint64_t* R, int64_t w) { return R[w * kRecWords + 2]; }  // '"' bytes
+
+// Level summaries: bit k of summary word s of level l is set iff leveled word
+// 64*s + k of level l is not zero, so a scan over a level can skip 64 words
+// (4 KB of input) with one load where the level has no structure.  They follow
+// levels 1 .. L-1 in the same allocation, Ws words per level (L >= 1 levels).
+GJ_REC_HD int64_t summary_stride(int64_t Wp) { return (Wp + 63) / 64; }
+// Leveled words of levels 1 .. L-1 in the allocation (level 0 is in the records).
+GJ_REC_HD int64_t deep_level_words(int64_t Wp, int L) { return Wp * (L > 1 ? L - 1 : 0); }
+// Words of a leveled-bitmap allocation (levels 1 .. L-1, then the summaries).
+GJ_REC_HD int64_t leveled_alloc_words(int64_t Wp, int L) {
+  return deep_level_words(Wp, L) + summary_stride(Wp) * (L > 1 ? L : 1);
+}
+// Start of the summaries in a leveled-bitmap allocation.
+GJ_REC_HD uint64_t* summaries_of(uint64_t* LB, int64_t Wp, int L) { return LB + deep_level_words(Wp, L); }
 
 // One file of an index launch.  The plain path launches with one file.
 struct FileJob {
@@ -49,8 +56,8 @@ struct FileJob {
   int64_t N;            // bytes
   int64_t W;            // words = ceil(N/64)
   int64_t Wp;           // level stride (W rounded up to 8)
-  uint64_t* rec;        // records (string, colon, raw-quote words), kRecWords * W words
-  uint64_t* LB;         // leveled bitmaps, L * Wp words
+  uint64_t* rec;        // records (string, colon, raw-quote, level-0 words), kRecWords * W words
+  uint64_t* LB;         // leveled bitmaps of levels 1 .. L-1, (L - 1) * Wp words
   uint64_t* LS;         // level summaries, L * Ws words (may be null)
   int64_t Ws;           // summary stride
   int64_t* NL;          // newline index, capacity nl_cap

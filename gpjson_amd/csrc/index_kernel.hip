@@ -194,6 +194,16 @@ __device__ __forceinline__ uint4 gload16(const uint8_t* p) {
 }
 __device__ __forceinline__ void gstore64(uint64_t* p, uint64_t v) { *(g_u64_t*)p = v; }
 __device__ __forceinline__ void gstore64i(int64_t* p, int64_t v) { *(g_i64_t*)p = v; }
+typedef __attribute__((address_space(1))) u32x4_t g_u32x4w_t;
+// 16-byte store of two words (p 16-byte aligned).
+__device__ __forceinline__ void gstore128(uint64_t* p, uint64_t a, uint64_t b) {
+  u32x4_t v;
+  v.x = (unsigned)a;
+  v.y = (unsigned)(a >> 32);
+  v.z = (unsigned)b;
+  v.w = (unsigned)(b >> 32);
+  *(g_u32x4w_t*)p = v;
+}
 
 // Two state words, each a relaxed atomic at device scope: every CU of every
 // XCD sees the words other CUs published, and a store does not wait for its
@@ -569,14 +579,6 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
   uint64_t S = wc[r][kSlotS0][tid];
   if (p && (ld & 0x80)) S ^= ~0ull << (ld & 0x7F);
   S = (s ? ~S : S) & valid;
-  // the word's record: string, colon and raw-quote words (bytes past the end
-  // of the file were loaded as zeros, which are neither ':' nor '"')
-  if (wr) {
-    uint64_t* rec = J.rec + w * kRecWords;
-    gstore64(rec, S);
-    gstore64(rec + 1, wc[r][kSlotK][tid]);
-    gstore64(rec + 2, wc[r][kSlotQ][tid]);
-  }
   if (VERIFY && act) {
     const uint64_t E = escape_word(wc[r][kSlotB][tid], (uint32_t)p);
     if (J.E) J.E[w] = E & valid;
@@ -588,12 +590,15 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
   // puts every byte of the word below level 0 or at level L or above either
   // way: clamped there it fits 32 bits and leaves the level bits unchanged.
   const int dc = (int)(d < -66 ? -66 : (d > (int64_t)J.L + 66 ? (int64_t)J.L + 66 : d));
+  uint64_t lb0 = 0;  // level 0, stored with the record
   for (int lo = 0; lo < J.L; lo += NLVS) {
     uint64_t out[NLVS];
     levels_spread<NLVS>(O, C, P, dc, lo, J.L, act, tid & 63, lv, items, out);
 #pragma unroll
-    for (int i = 0; i < NLVS; i++)
-      if (wr && lo + i < J.L) gstore64(J.LB + (int64_t)(lo + i) * J.Wp + w, out[i]);
+    for (int i = 0; i < NLVS; i++) {
+      if (lo + i == 0) lb0 = out[i];
+      else if (wr && lo + i < J.L) gstore64(J.LB + (int64_t)(lo + i - 1) * J.Wp + w, out[i]);
+    }
     // summaries: the 64 lanes of a wavefront hold 64 consecutive words that
     // start at a multiple of 64 (lanes past the file end are inactive, and
     // lane 0 holds the lowest word, so it is active whenever any lane is)
@@ -610,6 +615,13 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
         }
       }
     }
+  }
+  // the word's record: string, colon, raw-quote and level-0 words (bytes past
+  // the end of the file were loaded as zeros, which are neither ':' nor '"')
+  if (wr) {
+    uint64_t* rec = J.rec + w * kRecWords;
+    gstore128(rec, S, wc[r][kSlotK][tid]);
+    gstore128(rec + 2, wc[r][kSlotQ][tid], lb0);
   }
 }
 

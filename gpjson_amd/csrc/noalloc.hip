@@ -171,7 +171,7 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
   j.rec = (uint64_t*)(base + o.rec);
   j.LB = (uint64_t*)(base + o.lb);
   j.Ws = summary_stride(Wp);
-  j.LS = j.LB + Wp * std::max(L, 1);
+  j.LS = summaries_of(j.LB, Wp, L);
   j.NL = newlines;
   j.nl_cap = newline_cap;
   j.tile_lines = (int64_t*)(base + o.tile_lines);

@@ -24,12 +24,12 @@ struct QueryJob {
   int64_t N;
   const int64_t* NL;  // newline index
   int64_t lines;
-  const uint64_t* rec; // records: string, colon (key candidates) and raw-quote words per input word
-  const uint64_t* LB; // leveled bitmaps, level stride Wp
+  const uint64_t* rec; // records: string, colon (key candidates), raw-quote and level-0 words per input word
+  const uint64_t* LB; // leveled bitmaps of levels 1 .. L-1, level stride Wp
   const uint64_t* LS; // level summaries (bit per leveled word), stride Ws; may be null
   int64_t Ws;
   int64_t Wp;
-  int L;              // levels present in LB
+  int L;              // levels present (0 in the records, the rest in LB)
   int R;              // result slots per line
   const uint8_t* ir;  // device copy of the IR bytes
   int64_t ir_len;

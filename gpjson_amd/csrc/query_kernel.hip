@@ -97,12 +97,14 @@ struct Group {
 
 // Where the leveled and string words come from: device memory.
 struct GlobalWords {
-  const uint64_t* LB;
+  const uint64_t* LB;   // levels 1 .. L-1, stride Wp
   int64_t Wp;
-  const uint64_t* rec;  // records: string, colon and raw-quote words
+  const uint64_t* rec;  // records: string, colon, raw-quote and level-0 words
   const uint64_t* LS;   // level summaries (may be null: no skipping)
   int64_t Ws;
-  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const { return gld64(LB + (int64_t)lvl * Wp + wi); }
+  __device__ __forceinline__ uint64_t lb(int lvl, int64_t wi) const {
+    return lvl == 0 ? gld64(rec + wi * kRecWords + kRecLevel0) : gld64(LB + (int64_t)(lvl - 1) * Wp + wi);
+  }
   __device__ __forceinline__ uint64_t s(int64_t wi) const { return gld64(rec + wi * kRecWords); }
   __device__ __forceinline__ uint64_t colon(int64_t wi) const { return gld64(rec + wi * kRecWords + 1); }
   __device__ __forceinline__ uint64_t quote(int64_t wi) const { return gld64(rec + wi * kRecWords + 2); }

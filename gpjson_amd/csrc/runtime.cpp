@@ -695,7 +695,7 @@ static int alloc_index(gpjson_ctx* ctx, gpjson_buf* buf, int levels, int flags,
   alloc((void**)&idx->rec, (size_t)idx->W * kRecWords * 8);
   alloc((void**)&idx->LB, (size_t)leveled_alloc_words(idx->Wp, levels) * 8);
   idx->Ws = summary_stride(idx->Wp);
-  if (idx->LB) idx->LS = idx->LB + idx->Wp * std::max(levels, 1);
+  if (idx->LB) idx->LS = summaries_of(idx->LB, idx->Wp, levels);
   alloc((void**)&idx->NL, (size_t)idx->nl_cap * 8);
   alloc((void**)&idx->lines_dev, 64);
   // one entry per tile of the smallest supported tile size, plus the total
@@ -864,8 +864,12 @@ int gpjson_index_export(const gpjson_index* idx, int which, void* dst, int64_t c
   if (cap_bytes < size) return fail(GPJSON_ERR_CAPACITY, "export buffer too small");
   if (size == 0) return GPJSON_OK;
   if (which == GPJSON_INDEX_LEVELED) {
-    GJ_HIP(hipMemcpy2D(dst, (size_t)idx->W * 8, src, (size_t)idx->Wp * 8, (size_t)idx->W * 8, (size_t)idx->L,
+    // level 0 from the records, levels 1 .. L-1 from their level-major rows
+    GJ_HIP(hipMemcpy2D(dst, 8, idx->rec + kRecLevel0, (size_t)kRecWords * 8, 8, (size_t)idx->W,
                        hipMemcpyDeviceToHost));
+    if (idx->L > 1)
+      GJ_HIP(hipMemcpy2D((uint8_t*)dst + idx->W * 8, (size_t)idx->W * 8, src, (size_t)idx->Wp * 8,
+                         (size_t)idx->W * 8, (size_t)(idx->L - 1), hipMemcpyDeviceToHost));
   } else if (which == GPJSON_INDEX_STRING || which == GPJSON_INDEX_COLON || which == GPJSON_INDEX_RAW_QUOTE) {
     // one word of each record
     GJ_HIP(hipMemcpy2D(dst, 8, src, (size_t)kRecWords * 8, 8, (size_t)idx->W, hipMemcpyDeviceToHost));
