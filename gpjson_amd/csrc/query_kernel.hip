@@ -370,117 +370,163 @@ __device__ __forceinline__ int read_varint(const uint8_t* ir, int64_t ir_len, in
   return v;
 }
 
-// Evaluates line i of job J with group g.  le/ci: this group's nslots level
+// Interpreter state of one line (uniform within its group).
+struct LineState {
+  int64_t start;           // first byte of the line
+  int64_t j;               // byte cursor
+  int64_t pc;              // IR position
+  int64_t* out;            // the line's R span slots
+  const uint8_t* want;     // key of the pending MOVE_TO_KEY
+  int lvl, r, op, want_len, want_idx;
+  bool exec;               // run IR ops before the next wait
+};
+
+// Starts line i of job J with group g.  le/ci: this group's nslots level
 // slots in LDS, written by the group's first lane and read by all of them (LDS
 // operations of one wavefront complete in program order).
-template <int G, class Words>
-__device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, const uint8_t* ir, int64_t lines,
-                          int64_t i, int64_t* out, int64_t* le, int* ci, int nslots) {
+template <int G>
+__device__ __forceinline__ void line_begin(const Group<G>& g, const QueryJob& J, int64_t lines, int64_t i,
+                                           int64_t* out, int64_t* le, int* ci, int nslots, LineState& st) {
   const bool lead = g.r == 0;
-  const uint8_t* F = J.F;
-  const int64_t n = J.N;
   const int64_t start = gldi64(J.NL + i);
-  const int64_t end = (i + 1 < lines) ? gldi64(J.NL + i + 1) : n;
+  const int64_t end = (i + 1 < lines) ? gldi64(J.NL + i + 1) : J.N;
   if (lead)
     for (int k = 0; k < J.R * 2; k++) out[k] = -1;
   for (int k = g.r; k < nslots; k += G) { le[k] = -1; ci[k] = -1; }
   if (lead) le[0] = end;
+  st.start = start;
+  st.j = start;
+  st.pc = 0;
+  st.out = out;
+  st.want = nullptr;
+  st.lvl = 0;
+  st.r = 0;
+  st.op = kEnd;
+  st.want_len = 0;
+  st.want_idx = 0;
+  st.exec = true;
+}
 
-  int lvl = 0;
-  int64_t j = start;
-  int64_t pc = 0;
-  int r = 0;
-  int op = kEnd;
-  int want_len = 0, want_idx = 0;
-  const uint8_t* want = nullptr;
-  bool exec = true;
+// One round of a line: the IR ops up to the next wait, then the wait (a scan
+// of the current level for the event).  Returns false once the line is done.
+template <int G, class Words>
+__device__ bool line_step(const Group<G>& g, const QueryJob& J, const Words& W, const uint8_t* ir, LineState& st,
+                          int64_t* le, int* ci, int nslots) {
+  const bool lead = g.r == 0;
+  const uint8_t* F = J.F;
+  const int64_t n = J.N;
+  int lvl = st.lvl;
+  int64_t j = st.j;
+  int64_t pc = st.pc;
+  int r = st.r;
+  int op = st.op;
+  int want_len = st.want_len, want_idx = st.want_idx;
+  const uint8_t* want = st.want;
+  bool exec = st.exec;
+  int64_t* out = st.out;
 
-  for (;;) {
-    if (lvl < 0) return;
-    if (!(j < le[lvl] && j < n)) return;
-    while (exec) {
-      if (pc >= J.ir_len) return;
-      op = ir[pc++];
-      switch (op) {
-        case kEnd:
-          return;
-        case kStoreResult: {
-          while (j < n && is_blank(gld8(F + j))) j++;
-          if (r < J.R && lead) {
-            out[r * 2] = j;
-            out[r * 2 + 1] = le[lvl];
-          }
-          r++;
-          break;
+  if (lvl < 0) return false;
+  if (!(j < le[lvl] && j < n)) return false;
+  while (exec) {
+    if (pc >= J.ir_len) return false;
+    op = ir[pc++];
+    switch (op) {
+      case kEnd:
+        return false;
+      case kStoreResult: {
+        while (j < n && is_blank(gld8(F + j))) j++;
+        if (r < J.R && lead) {
+          out[r * 2] = j;
+          out[r * 2 + 1] = le[lvl];
         }
-        case kMoveUp:
-          j = le[lvl];
-          if (lead) le[lvl] = -1;
-          lvl--;
-          // a level whose end was never found (-1) ends the line: the walk
-          // would otherwise continue from before the line's first byte
-          if (lvl < 0 || j < 0) return;
-          break;
-        case kMoveDown: {
-          lvl++;
-          if (lvl >= nslots) return;
-          if (le[lvl] == -1) {
-            int64_t e = -1;
-            if (lvl - 1 < J.L) e = group_first_bit(g, W, lvl - 1, j + 1, le[lvl - 1]);
-            if (lead) le[lvl] = e;
-          }
-          break;
-        }
-        case kMoveToKey:
-          want_len = read_varint(ir, J.ir_len, &pc);
-          want = ir + pc;
-          pc += want_len;
-          exec = false;
-          break;
-        case kMoveToIndex: {
-          want_idx = read_varint(ir, J.ir_len, &pc);
-          if (ci[lvl] == -1) {
-            while (j < n && gld8(F + j) != '[') {
-              if (!is_blank(gld8(F + j))) return;  // not an array: the line has no further results
-              j++;
-            }
-            if (j >= n) return;
-            if (lead) ci[lvl] = 0;
-          }
-          if (want_idx == 0) { j++; exec = true; } else { exec = false; }
-          break;
-        }
-        case kExpressionStringEquals: {
-          want_len = read_varint(ir, J.ir_len, &pc);
-          want = ir + pc;
-          pc += want_len;
-          while (j < n && is_blank(gld8(F + j))) j++;
-          if (le[lvl] - j != want_len) return;
-          for (int k = 0; k < want_len; k++)
-            if (want[k] != gld8(F + j + k)) return;
-          break;
-        }
-        default:
-          return;
+        r++;
+        break;
       }
-    }
-    if (!(j < n)) return;
-    // wait for the next event at this level
-    const int64_t lim = le[lvl];
-    const int64_t to = lim < n ? lim : n;
-    int64_t p = -1;
-    if (lvl < J.L) {
-      if (op == kMoveToKey) {
-        p = group_find_key(g, W, J.rec, lvl, F, start, j, to, want, want_len);
-      } else if (op == kMoveToIndex) {
-        int cnt = ci[lvl];
-        p = group_find_comma(g, W, lvl, F, n, j, to, &cnt, want_idx);
-        if (lead) ci[lvl] = cnt;
+      case kMoveUp:
+        j = le[lvl];
+        if (lead) le[lvl] = -1;
+        lvl--;
+        // a level whose end was never found (-1) ends the line: the walk
+        // would otherwise continue from before the line's first byte
+        if (lvl < 0 || j < 0) return false;
+        break;
+      case kMoveDown: {
+        lvl++;
+        if (lvl >= nslots) return false;
+        if (le[lvl] == -1) {
+          int64_t e = -1;
+          if (lvl - 1 < J.L) e = group_first_bit(g, W, lvl - 1, j + 1, le[lvl - 1]);
+          if (lead) le[lvl] = e;
+        }
+        break;
       }
+      case kMoveToKey:
+        want_len = read_varint(ir, J.ir_len, &pc);
+        want = ir + pc;
+        pc += want_len;
+        exec = false;
+        break;
+      case kMoveToIndex: {
+        want_idx = read_varint(ir, J.ir_len, &pc);
+        if (ci[lvl] == -1) {
+          while (j < n && gld8(F + j) != '[') {
+            if (!is_blank(gld8(F + j))) return false;  // not an array: the line has no further results
+            j++;
+          }
+          if (j >= n) return false;
+          if (lead) ci[lvl] = 0;
+        }
+        if (want_idx == 0) { j++; exec = true; } else { exec = false; }
+        break;
+      }
+      case kExpressionStringEquals: {
+        want_len = read_varint(ir, J.ir_len, &pc);
+        want = ir + pc;
+        pc += want_len;
+        while (j < n && is_blank(gld8(F + j))) j++;
+        if (le[lvl] - j != want_len) return false;
+        for (int k = 0; k < want_len; k++)
+          if (want[k] != gld8(F + j + k)) return false;
+        break;
+      }
+      default:
+        return false;
     }
-    if (p < 0) return;
-    exec = true;
-    j = p + 1;
+  }
+  if (!(j < n)) return false;
+  // wait for the next event at this level
+  const int64_t lim = le[lvl];
+  const int64_t to = lim < n ? lim : n;
+  int64_t p = -1;
+  if (lvl < J.L) {
+    if (op == kMoveToKey) {
+      p = group_find_key(g, W, J.rec, lvl, F, st.start, j, to, want, want_len);
+    } else if (op == kMoveToIndex) {
+      int cnt = ci[lvl];
+      p = group_find_comma(g, W, lvl, F, n, j, to, &cnt, want_idx);
+      if (lead) ci[lvl] = cnt;
+    }
+  }
+  if (p < 0) return false;
+  st.lvl = lvl;
+  st.j = p + 1;
+  st.pc = pc;
+  st.r = r;
+  st.op = op;
+  st.want_len = want_len;
+  st.want_idx = want_idx;
+  st.want = want;
+  st.exec = true;
+  return true;
+}
+
+// Evaluates line i of job J with group g.
+template <int G, class Words>
+__device__ void eval_line(const Group<G>& g, const QueryJob& J, const Words& W, const uint8_t* ir, int64_t lines,
+                          int64_t i, int64_t* out, int64_t* le, int* ci, int nslots) {
+  LineState st;
+  line_begin(g, J, lines, i, out, le, ci, nslots, st);
+  while (line_step(g, J, W, ir, st, le, ci, nslots)) {
   }
 }
 
@@ -520,6 +566,68 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
   // 64/G times fewer atomics than with per-group claims.
   constexpr int kGroupsPerWave = 64 / G;
   const int lane = __lane_id();
+  // defers a long line to the wavefront-per-line launch (false: list full)
+  auto defer = [&](const QueryJob& J, int64_t lines, int64_t i, int64_t it) -> bool {
+    if (!launch.long_items || G == 64) return false;
+    const int64_t start = J.NL[i], end = (i + 1 < lines) ? J.NL[i + 1] : J.N;
+    if (end - start < launch.long_min) return false;
+    int64_t slot = 0;
+    if (g.r == 0) slot = (int64_t)atomicAdd(&launch.long_count[0], 1ull);
+    slot = g.shfl_i64(slot, 0);
+    if (slot >= launch.long_cap) return false;
+    if (g.r == 0) launch.long_items[slot] = it;
+    return true;
+  };
+  if (G == 1 && launch.lines_dev && launch.njobs == 1) {
+    // One query, a lane per line: a lane starts its next line as soon as its
+    // current one ends, so the wavefront's lanes stay busy until the lines run
+    // out instead of waiting at the end of every claim for the slowest one.
+    // Every round each lane with a line runs its IR ops up to the next wait
+    // and that wait; lanes without one take the next items of the wavefront's
+    // pool, which claims 64 items from the counter when it runs short.
+    const QueryJob J = launch.jobs[0];
+    const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
+    const uint8_t* ir = ir_of(J);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int64_t pool = 0, pool_end = 0;  // items claimed, not yet handed out (wavefront-uniform)
+    bool has = false, done = false;
+    LineState st;
+    for (;;) {
+      const unsigned long long need = __ballot(!has && !done);
+      if (need) {
+        const int k = __popcll(need);
+        const int64_t avail = pool_end - pool;
+        int64_t fresh = 0;
+        if (avail < k) {
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(launch.counter, 64ull);
+          fresh = (int64_t)shfl64_lane(c, 0);
+        }
+        if (!has && !done) {
+          const int rank = __popcll(need & below);
+          const int64_t item = rank < avail ? pool + rank : fresh + (rank - avail);
+          if (item >= total) {
+            done = true;
+          } else {
+            const int64_t i = launch.line_list ? launch.line_list[item] : item;
+            if (i >= 0 && i < common && !defer(J, common, i, item)) {
+              line_begin(g, J, common, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots, st);
+              has = true;
+            }
+          }
+        }
+        if (avail < k) {
+          pool = fresh + (k - avail);
+          pool_end = fresh + 64;
+        } else {
+          pool += k;
+        }
+      }
+      if (!__ballot(has || !done)) break;  // every lane is past the last item
+      if (has) has = line_step(g, J, W, ir, st, le, ci, nslots);
+    }
+    return;
+  }
   for (;;) {
     unsigned long long w0 = 0;
     if (lane == 0) w0 = atomicAdd(launch.counter, (unsigned long long)kGroupsPerWave * lpw);
@@ -527,18 +635,6 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     if ((int64_t)w0 >= total) break;
     const unsigned long long g0 = w0 + (unsigned long long)(lane / G) * lpw;
     if ((int64_t)g0 >= total) continue;
-    // defers a long line to the wavefront-per-line launch (false: list full)
-    auto defer = [&](const QueryJob& J, int64_t lines, int64_t i, int64_t it) -> bool {
-      if (!launch.long_items || G == 64) return false;
-      const int64_t start = J.NL[i], end = (i + 1 < lines) ? J.NL[i + 1] : J.N;
-      if (end - start < launch.long_min) return false;
-      int64_t slot = 0;
-      if (g.r == 0) slot = (int64_t)atomicAdd(&launch.long_count[0], 1ull);
-      slot = g.shfl_i64(slot, 0);
-      if (slot >= launch.long_cap) return false;
-      if (g.r == 0) launch.long_items[slot] = it;
-      return true;
-    };
     if (launch.lines_dev && launch.njobs == 1) {
       // one query: its job once, in scalar registers, not reloaded per line
       const QueryJob J = launch.jobs[0];
