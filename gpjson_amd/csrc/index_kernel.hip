@@ -566,12 +566,14 @@ struct FuseTile {
 template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK, int NLVS>
 __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t (&wc)[ROWS][NSLOT][BLOCK],
                                           const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
-                                          int64_t d, bool act, uint64_t (&lv)[NLVS][64],
+                                          int64_t d, bool act, bool full, uint64_t (&lv)[NLVS][64],
                                           uint16_t (&items)[kBracketCap], const FuseEmit& fz) {
   const bool wr = act && (!fz.on || fz.wr);
-  uint64_t valid = ~0ull;
-  const int64_t rem = J.N - w * 64;
-  if (rem < 64) valid = rem > 0 ? (1ull << rem) - 1ull : 0;
+  uint64_t valid = ~0ull;  // bytes of the word inside the file (full: the tile's words all are)
+  if (!full) {
+    const int64_t rem = J.N - w * 64;
+    if (rem < 64) valid = rem > 0 ? (1ull << rem) - 1ull : 0;
+  }
   // in-string mask from the one for escape carry 0 outside a string: carry 1
   // escapes the leading backslash run's first byte instead of its second, which
   // toggles only a quote right after the run, and so every bit from there on
@@ -969,6 +971,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     // the lines that cross into its neighbours.
     const bool ft = FUSE && dl && launch.fuse_depth > 0 && mtile.nl <= fused_line_cap(TILE_WORDS);
     const int fnl = ts.fnl, lnl = ts.lnl;
+    const bool full = (word0 + TILE_WORDS) * 64 <= J.N;  // every byte of the tile is in the file
     auto emit_tile = [&](const Carry& cx, bool fx) {
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
@@ -980,7 +983,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
         const int s = cx.s ^ mx.q;
         const int64_t d = cx.d + (cx.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
         const FuseEmit fz{ft, sidx <= fnl || sidx >= lnl, s_fsum + kFuseMaxDepth * NSUM, NSUM};
-        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, s_lv[wave], s_items[wave], fz);
+        emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, full, s_lv[wave], s_items[wave], fz);
       }
     };
     emit_tile(X, fix);
