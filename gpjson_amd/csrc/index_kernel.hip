@@ -566,7 +566,7 @@ struct FuseTile {
 template <int NLV, bool VERIFY, int ROWS, int NSLOT, int BLOCK, int NLVS>
 __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t (&wc)[ROWS][NSLOT][BLOCK],
                                           const uint8_t (&lead)[ROWS][BLOCK], int r, int tid, int p, int s,
-                                          int64_t d, bool act, bool full, uint64_t (&lv)[NLVS][64],
+                                          int d, bool act, bool full, uint64_t (&lv)[NLVS][64],
                                           uint16_t (&items)[kBracketCap], const FuseEmit& fz) {
   const bool wr = act && (!fz.on || fz.wr);
   uint64_t valid = ~0ull;  // bytes of the word inside the file (full: the tile's words all are)
@@ -590,8 +590,8 @@ __device__ __forceinline__ void emit_word(const FileJob& J, int64_t w, uint64_t 
   if (fz.on) wc[r][kSlotS0][tid] = S;  // only this lane reads its slots until the next barrier
   // A word moves the depth by at most 64, so a depth below -66 or above L + 66
   // puts every byte of the word below level 0 or at level L or above either
-  // way: clamped there it fits 32 bits and leaves the level bits unchanged.
-  const int dc = (int)(d < -66 ? -66 : (d > (int64_t)J.L + 66 ? (int64_t)J.L + 66 : d));
+  // way: clamped there it leaves the level bits unchanged.
+  const int dc = min(max(d, -66), J.L + 66);
   uint64_t lb0 = 0;  // level 0, stored with the record
   for (int lo = 0; lo < J.L; lo += NLVS) {
     uint64_t out[NLVS];
@@ -973,6 +973,10 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     const int fnl = ts.fnl, lnl = ts.lnl;
     const bool full = (word0 + TILE_WORDS) * 64 <= J.N;  // every byte of the tile is in the file
     auto emit_tile = [&](const Carry& cx, bool fx) {
+      // the carry-in depth held to 32 bits: a tile moves the depth by at most
+      // its bytes, so a carry-in beyond +-2^30 leaves every word's depth beyond
+      // the clamp in emit_word either way
+      const int cxd = (int)(cx.d < -(1ll << 30) ? -(1ll << 30) : (cx.d > (1ll << 30) ? (1ll << 30) : cx.d));
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
         const int sidx = r * BLOCK + tid;
@@ -981,7 +985,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
         if (fx && sidx > w0) mx = m_cat(m_flip(w0m), m_left_div(w0m, mx));
         const int p = ent[r] ? (ent[r] - 2) : cx.p;
         const int s = cx.s ^ mx.q;
-        const int64_t d = cx.d + (cx.s ? (int64_t)(mx.tot - mx.d0) : (int64_t)mx.d0);
+        const int d = cxd + (cx.s ? mx.tot - mx.d0 : mx.d0);
         const FuseEmit fz{ft, sidx <= fnl || sidx >= lnl, s_fsum + kFuseMaxDepth * NSUM, NSUM};
         emit_word<NLV, VERIFY>(J, w, s_wc, s_lead, r, tid, p, s, d, w < J.W, full, s_lv[wave], s_items[wave], fz);
       }
