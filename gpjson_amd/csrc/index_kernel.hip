@@ -1339,6 +1339,37 @@ __device__ void fuse_place_lines(const IndexLaunch& L, const FileJob& J, int64_t
 // their slots (tile t's first newline takes slot 1 + the newlines before it).
 // A tile with more newlines than its row holds (lines shorter than 64 bytes on
 // average) sets *overflow for step 4.
+// Copies the newline offsets of a wavefront's 64 tiles (this lane's tile:
+// cnt offsets from src to dst[0 .. min(cnt, room))) with consecutive lanes
+// on consecutive offsets: the wavefront walks its tiles' offsets as one run,
+// so the reads within a row and the writes (consecutive across the tiles of
+// a file) are coalesced.  Every lane of the wavefront takes part.
+__device__ void place_tile_newlines(const int64_t* src, int cnt, int64_t* dst, int64_t room) {
+  __shared__ int s_incl[kTileBlock];  // inclusive scan of each wavefront's counts
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int* inc = s_incl + wave * 64;
+  const int incl = wave_scan_int(cnt);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  inc[lane] = incl;
+  wave_lds_sync();
+  for (int u0 = 0; u0 < total; u0 += 64) {
+    const int u = u0 + lane;
+    int lo = 0, hi = 63;  // the lane whose tile holds offset u: first with inc > u
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (inc[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    const int64_t* s_ = (const int64_t*)shfl_u64((uint64_t)src, lo);
+    int64_t* d_ = (int64_t*)shfl_u64((uint64_t)dst, lo);
+    const int64_t rm = (int64_t)shfl_u64((uint64_t)room, lo);
+    if (u < total) {
+      const int k = u - (lo ? inc[lo - 1] : 0);
+      if (k < rm) d_[k] = s_[k];
+    }
+  }
+  wave_lds_sync();
+}
+
 __global__ __launch_bounds__(kTileBlock) void index_tile_lines_kernel(IndexLaunch L, const int64_t* block_base,
                                                                       int* overflow) {
   if (L.run_if && *L.run_if == 0) return;
@@ -1347,17 +1378,14 @@ __global__ __launch_bounds__(kTileBlock) void index_tile_lines_kernel(IndexLaunc
   const int cnt = t < J.ntiles ? (int)(L.states[J.tile0 + t].agg0 & kField24) : 0;
   int excl;
   tile_block_scan(cnt, &excl);
-  if (t >= J.ntiles) return;
   const int64_t before = block_base[blockIdx.x] + excl;  // newlines before tile t
-  if (J.tile_lines) J.tile_lines[t] = t == 0 ? 0 : 1 + before;
-  if (L.fuse_depth > 0) fuse_place_lines(L, J, t, cnt, before);
-  if (cnt > L.nl_scratch_cap) {
-    atomicOr(overflow, 1);
-    return;
-  }
-  const int64_t* src = L.nl_scratch + (J.tile0 + t) * L.nl_scratch_cap;
-  for (int k = 0; k < cnt; k++)
-    if (1 + before + k < J.nl_cap) J.NL[1 + before + k] = src[k];
+  const bool in = t < J.ntiles;  // threads past the last tile take part in the placement only
+  if (in && J.tile_lines) J.tile_lines[t] = t == 0 ? 0 : 1 + before;
+  if (in && L.fuse_depth > 0) fuse_place_lines(L, J, t, cnt, before);
+  const bool over = cnt > L.nl_scratch_cap;  // step 4 places this tile's offsets
+  if (over) atomicOr(overflow, 1);
+  place_tile_newlines(L.nl_scratch + (J.tile0 + t) * L.nl_scratch_cap, over ? 0 : cnt, J.NL + 1 + before,
+                      J.nl_cap - 1 - before);
 }
 
 // Step 4 (only when *overflow): the offsets of tiles with more newlines than
@@ -1494,20 +1522,27 @@ __global__ __launch_bounds__(kTileBlock) void batch_tile_lines_kernel(IndexLaunc
   const int cnt = t < L.total_tiles ? (int)(L.states[t].agg0 & kField24) : 0;
   int excl;
   tile_block_scan(cnt, &excl);
-  if (t >= L.total_tiles) return;
-  const int f = file_of_tile(L.jobs, L.njobs, t);
-  const FileJob& J = L.jobs[f];
-  const int64_t lt = t - J.tile0;
-  const int64_t before = block_base[blockIdx.x] + excl - file_base[f];  // the file's newlines before tile lt
-  if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : 1 + before;
-  int64_t* row = L.nl_scratch + t * L.nl_scratch_cap;
-  if (cnt > L.nl_scratch_cap) {
-    row[0] = before;
-    atomicOr(overflow, 1);
-    return;
+  int ccnt = 0;  // offsets to copy (threads past the last tile take part in the copy only)
+  int64_t* dst = nullptr;
+  int64_t room = 0;
+  int64_t* row = L.nl_scratch;
+  if (t < L.total_tiles) {
+    const int f = file_of_tile(L.jobs, L.njobs, t);
+    const FileJob& J = L.jobs[f];
+    const int64_t lt = t - J.tile0;
+    const int64_t before = block_base[blockIdx.x] + excl - file_base[f];  // the file's newlines before tile lt
+    if (J.tile_lines) J.tile_lines[lt] = lt == 0 ? 0 : 1 + before;
+    row = L.nl_scratch + t * L.nl_scratch_cap;
+    if (cnt > L.nl_scratch_cap) {
+      row[0] = before;
+      atomicOr(overflow, 1);
+    } else {
+      ccnt = cnt;
+      dst = J.NL + 1 + before;
+      room = J.nl_cap - 1 - before;
+    }
   }
-  for (int k = 0; k < cnt; k++)
-    if (1 + before + k < J.nl_cap) J.NL[1 + before + k] = row[k];
+  place_tile_newlines(row, ccnt, dst, room);
 }
 
 // Step 4 (only when *overflow): as index_tile_overflow_kernel.
