@@ -774,6 +774,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     int code[ROWS];  // escape summary: 0 all backslashes, else 2 + carry out
     M4 wm0[ROWS];    // the word's transfer function for escape carry-in 0
     int flip[ROWS];  // the byte after the leading backslash run is a quote
+    bool rowb[ROWS];  // the wavefront's words of the row hold a backslash (wave-uniform)
     {
       load_rows(J, word0);
 #pragma unroll
@@ -787,10 +788,21 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
           d[4 * q + 3] = raw[r][q].w;
         }
         const WordClass w = classify64(d);
-        code[r] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
-        flip[r] = code[r] ? (int)((w.Q >> ctz64(~w.B)) & 1) : 0;
-        const uint64_t E = escape_word(w.B, 0u);
-        const uint64_t Qe = w.Q & ~E;
+        // Most rows hold no backslash at all: then no byte is escaped with
+        // carry-in 0, every word passes carry 0 on, and its leading backslash
+        // run is empty (so a quote after it is a quote at byte 0).
+        rowb[r] = __ballot(w.B != 0) != 0;
+        uint64_t Qe = w.Q;
+        if (rowb[r]) {
+          code[r] = (w.B == ~0ull) ? 0 : (2 + (int)escape_carry_out(w.B, 0));
+          flip[r] = code[r] ? (int)((w.Q >> ctz64(~w.B)) & 1) : 0;
+          Qe &= ~escape_word(w.B, 0u);
+          s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
+        } else {
+          code[r] = 2;
+          flip[r] = (int)(w.Q & 1);
+          s_lead[r][tid] = (uint8_t)(flip[r] << 7);
+        }
         const uint64_t S0 = prefix_xor(Qe);
         M4 m;
         m.q = popc64(Qe) & 1;
@@ -806,7 +818,6 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
         if (VERIFY) s_wc[r][kSlotB][tid] = w.B;
         s_wc[r][kSlotQ][tid] = w.Q;
         s_wc[r][kSlotK][tid] = w.K;
-        s_lead[r][tid] = (uint8_t)((code[r] ? ctz64(~w.B) : 64) | (flip[r] << 7));
         if (!VERIFY) {
           const unsigned long long nlw = __ballot(w.NL != 0);
           if (nlw && lane == __builtin_ctzll(nlw)) atomicMin(&ts.fnl, r * BLOCK + tid);
@@ -821,6 +832,11 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
       unsigned long long nzm[ROWS];
 #pragma unroll
       for (int r = 0; r < ROWS; r++) {
+        if (!rowb[r]) {  // every word's code is 2 (carry 0)
+          if (lane == 0) s_esc[r][wave] = 2;
+          ent[r] = lane ? 2 : -1;
+          continue;
+        }
         nzm[r] = __ballot(code[r] != 0);
         const int last = nzm[r] ? __shfl(code[r], 63 - __clzll(nzm[r])) : 0;
         if (lane == 0) s_esc[r][wave] = last;

@@ -130,6 +130,20 @@ def test_backslash_runs_across_boundaries(ctx, seed):
     check_index(ctx, backslash_runs(rng, 1_500_000 + seed * 100_000), 4, f"runs seed={seed}")
 
 
+@pytest.mark.parametrize("run", [1, 2, 3, 63, 64, 65])
+def test_backslash_run_then_quote_at_word_start(ctx, run):
+    """A backslash run ending right before a '"' that starts a word, at word,
+    64-word row and tile boundaries, with no other backslash near it: the row
+    after the run holds none, and its first quote is escaped or not by the
+    carry alone (odd runs escape it, even runs do not)."""
+    filler = b'{"a":"bc","d":[1,2,{"e":null}],"f":"g h"}\n'
+    data = bytearray((filler * (300_000 // len(filler) + 1))[:300_000])
+    for b in (64 * 7, 4096 * 3, 4096 * 5 + 64, 32768 * 2, 32768 * 4 + 4096, 49152 * 3, 65536 * 3):
+        data[b - run:b] = b"\\" * run
+        data[b] = 0x22
+    check_index(ctx, bytes(data), 3, f"run={run} then a quote at word starts")
+
+
 def test_whole_tiles_of_backslashes(ctx):
     # tiles that are entirely backslashes carry the escape state through
     for run in (65536 * 3 + 1, 65536 * 3, 32768 * 5 + 3, 64 * 1000 + 1):
