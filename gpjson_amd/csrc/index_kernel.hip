@@ -1458,23 +1458,29 @@ __device__ __forceinline__ int file_of_tile(const FileJob* jobs, int64_t njobs, 
 // Step 1: as index_tile_sums_kernel; a file's first tile has no predecessor
 // to check against.
 __global__ __launch_bounds__(kTileBlock) void batch_tile_sums_kernel(IndexLaunch L, int64_t* block_sums,
-                                                                     int* mismatch) {
+                                                                     int64_t* file_base, int* mismatch) {
   if (L.run_if && *L.run_if == 0) return;
   const int64_t t = (int64_t)blockIdx.x * kTileBlock + threadIdx.x;
   int cnt = 0;
+  int f = -1;  // the file of a file's first tile
   if (t < L.total_tiles) {
     const TileState* st = L.states + t;
     cnt = (int)(st->agg0 & kField24);
-    if (t > L.jobs[file_of_tile(L.jobs, L.njobs, t)].tile0) {
+    const int ft = file_of_tile(L.jobs, L.njobs, t);
+    if (t > L.jobs[ft].tile0) {
       bool ps, cs;
       const Carry e = apply_agg(decode_agg(st[-1].agg0, st[-1].agg1), load_carry_in(st - 1, &ps));
       const Carry c = load_carry_in(st, &cs);
       if (c.s != e.s || c.d != e.d || (!cs && c.p != e.p)) atomicOr(mismatch, 1);
+    } else {
+      f = ft;
     }
   }
   int excl;
   const int total = tile_block_scan(cnt, &excl);
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+  // a file's first tile: the newlines before it within its block (step 2 adds the block's base)
+  if (f >= 0) file_base[f] = excl;
 }
 
 // Step 2 (one block): exclusive scan of the block sums in place, then per
@@ -1482,50 +1488,48 @@ __global__ __launch_bounds__(kTileBlock) void batch_tile_sums_kernel(IndexLaunch
 __global__ __launch_bounds__(kTileBlock) void batch_tile_base_kernel(IndexLaunch L, int64_t* block_sums, int64_t nb,
                                                                      int64_t* file_base, int64_t* lines_out) {
   if (L.run_if && *L.run_if == 0) return;
-  __shared__ int64_t s_total;
-  int64_t carry = 0;
-  for (int64_t c0 = 0; c0 < nb; c0 += kTileBlock) {
-    const int64_t i = c0 + threadIdx.x;
-    const int64_t v = i < nb ? block_sums[i] : 0;
-    int64_t excl;
-    const int64_t total = tile_block_scan64(v, &excl);
-    if (i < nb) block_sums[i] = carry + excl;
-    carry += total;
+  // exclusive scan of the block sums in place: each thread takes a run of
+  // consecutive sums, one block scan joins the runs
+  const int64_t per = (nb + kTileBlock - 1) / kTileBlock;
+  const int64_t i0 = (int64_t)threadIdx.x * per, i1 = min(i0 + per, nb);
+  int64_t part = 0;
+  for (int64_t i = i0; i < i1; i++) part += block_sums[i];
+  int64_t run;
+  const int64_t total = tile_block_scan64(part, &run);
+  for (int64_t i = i0; i < i1; i++) {
+    const int64_t v = block_sums[i];
+    block_sums[i] = run;
+    run += v;
   }
-  if (threadIdx.x == 0) s_total = carry;
   __syncthreads();
-  const int64_t total = s_total;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // newlines in the tiles before tile x (x <= total_tiles; x the same in every
-  // lane): the scan at x's block plus the counts of the block's tiles before
-  // x, summed over the wavefront (at most 255 counts of at most 65536)
-  auto newlines_before = [&](int64_t x) -> int64_t {
-    const int64_t blk = x / kTileBlock;
-    if (blk >= nb) return total;
-    int v = 0;
-    for (int64_t i = blk * kTileBlock + lane; i < x; i += 64) v += (int)(L.states[i].agg0 & kField24);
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return block_sums[blk] + v;
-  };
-  // a wavefront per file
-  for (int64_t f = wave; f < L.njobs; f += kTileBlock / 64) {
+  // per file, a thread each: the newlines before its first tile (step 1 left
+  // the count within the tile's block in file_base) ...
+  for (int64_t f = threadIdx.x; f < L.njobs; f += kTileBlock) {
     const FileJob& J = L.jobs[f];
     if (J.ntiles == 0) {  // empty file: one line starting at 0
-      if (lane == 0) {
-        if (J.nl_cap > 0) J.NL[0] = 0;
-        if (J.tile_lines) J.tile_lines[0] = 1;
-        file_base[f] = 0;
-        lines_out[f] = 1;
-      }
-      continue;
+      if (J.nl_cap > 0) J.NL[0] = 0;
+      if (J.tile_lines) J.tile_lines[0] = 1;
+      file_base[f] = 0;
+      lines_out[f] = 1;
+    } else {
+      file_base[f] += block_sums[J.tile0 / kTileBlock];
     }
-    const int64_t b0 = newlines_before(J.tile0);
-    const int64_t lines = 1 + newlines_before(J.tile0 + J.ntiles) - b0;
-    if (lane == 0) {
-      file_base[f] = b0;
-      if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
-      lines_out[f] = lines;
+  }
+  __syncthreads();
+  // ... then its line count: its last tile ends where the next file with
+  // tiles starts (files are consecutive in tile order), or at the total
+  for (int64_t f = threadIdx.x; f < L.njobs; f += kTileBlock) {
+    const FileJob& J = L.jobs[f];
+    if (J.ntiles == 0) continue;
+    int64_t end = total;
+    if (J.tile0 + J.ntiles < L.total_tiles) {
+      int64_t g = f + 1;
+      while (L.jobs[g].ntiles == 0) g++;
+      end = file_base[g];
     }
+    const int64_t lines = 1 + end - file_base[f];
+    if (J.tile_lines) J.tile_lines[J.ntiles] = lines;
+    lines_out[f] = lines;
   }
 }
 
@@ -1706,7 +1710,8 @@ hipError_t index_finish_batch(const IndexLaunch& L, int64_t* block_sums, int64_t
   const int64_t nb = (nt + kTileBlock - 1) / kTileBlock;
   const unsigned grid = (unsigned)(nb > 0 ? nb : 1);
   if (nt > 0)
-    hipLaunchKernelGGL(batch_tile_sums_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L, block_sums, mismatch);
+    hipLaunchKernelGGL(batch_tile_sums_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L, block_sums, file_base,
+                       mismatch);
   hipLaunchKernelGGL(batch_tile_base_kernel, dim3(1), dim3(kTileBlock), 0, stream, L, block_sums, nb, file_base,
                      lines_out);
   if (nt > 0) {
