@@ -430,7 +430,7 @@ __device__ Carry lookback_wave(TileState* st0, int64_t lt, int lane, const TileA
 // in-string mask for escape carry 0 outside a string, newlines, open, close,
 // ':' ',', quotes and ':'; verify mode also keeps the backslashes.  (The colon
 // and quote words need no carry, but writing them apart from the string word
-// of the same 24-byte record sends many record lines to memory twice.)
+// of the same record sends many record lines to memory twice.)
 constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlotQ = 5, kSlotK = 6, kSlotB = 7;
 
 // Leveled bitmaps with the brackets spread over the wavefront.  Lane-per-word bracket loops run as long as the word

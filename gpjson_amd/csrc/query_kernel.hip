@@ -15,8 +15,10 @@
 // by a group ballot, so the first match in byte order is the one taken, as in
 // the sequential loop.  Level ends (MOVE_DOWN) are found the same way.  Small
 // G keeps many lines in flight (latency-bound short lines); G = 64 suits long
-// lines.  Groups claim lines_per_wave lines at a time from an atomic counter.
-// Per-level state lives in LDS: levels+2 slots per group.
+// lines.  With one query and a lane per line, a lane starts its next line as
+// soon as its current one ends (a per-wavefront pool of claimed lines);
+// otherwise groups claim lines_per_wave lines at a time from an atomic
+// counter.  Per-level state lives in LDS: levels+2 slots per group.
 #include <hip/hip_runtime.h>
 #include "index_common.h"
 #include "query_common.h"

@@ -162,8 +162,8 @@ struct gpjson_index {
   int64_t n = 0, W = 0, Wp = 0;
   int L = 0;
   int flags = 0;
-  uint64_t* rec = nullptr;  // records: string, colon and raw-quote words (kRecWords per word)
-  uint64_t* LB = nullptr;
+  uint64_t* rec = nullptr;  // records: string, colon, raw-quote and level-0 words (kRecWords per word)
+  uint64_t* LB = nullptr;  // leveled bitmaps of levels 1 .. L-1 (level 0 is in the records)
   uint64_t* LS = nullptr;  // level summaries, inside the LB allocation
   int64_t Ws = 0;
   int64_t* NL = nullptr;

@@ -135,7 +135,7 @@ namespace {
 
 int64_t index_bytes(int64_t n, int levels) {
   const int64_t W = (n + 63) / 64, Wp = (W + 7) / 8 * 8;
-  // records (string, colon and raw-quote words), leveled words, newline and tile tables
+  // records (string, colon, raw-quote and level-0 words), leveled words, newline and tile tables
   return kRecWords * W * 8 + leveled_alloc_words(Wp, levels) * 8 + (n / 256 + 4096) * 8 + (n / 16384 + 2) * 8 + 64;
 }
 
