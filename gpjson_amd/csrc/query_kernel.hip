@@ -652,74 +652,82 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     }
     // several jobs: each lane keeps its current job in registers and reloads
     // it only when its items cross into another job (items of a claim are
-    // consecutive), so a line's walk does not start with a load of its job
-    if (launch.lines_dev) {
-      auto split = [&](int64_t it, int64_t& q, int64_t& i) {
-        if (launch.qblock > 0) {
-          split_item(it, common, launch.njobs, launch.qblock, &q, &i);
-        } else {
-          q = it / common;
-          i = it - q * common;
-        }
-      };
-      int64_t qcur, i0;
-      split((int64_t)g0, qcur, i0);
-      QueryJob J = launch.jobs[qcur];
-      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-        int64_t q, i;
-        split(it, q, i);
-        if (q != qcur) {
-          qcur = q;
-          J = launch.jobs[q];
-        }
-        if (defer(J, common, i, it)) continue;
-        int64_t* out = J.out + (q * common + i) * (int64_t)J.R * 2;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), common, i, out, le, ci, nslots);
+    // consecutive), so a line's walk does not start with a load of its job.
+    // Work item -> (job, line): by the launch's shared line count (one index,
+    // several queries); or by the jobs' first items (several files), where a
+    // file's queries may take turns over blocks of its lines (its items are
+    // [base, base + nq * lines), its jobs j0 .. j0 + nq - 1).  One loop and
+    // one call of the line walk serve all three.
+    const bool by_lines = launch.lines_dev != nullptr;
+    auto split = [&](int64_t it, int64_t& q, int64_t& i) {
+      if (launch.qblock > 0) {
+        split_item(it, common, launch.njobs, launch.qblock, &q, &i);
+      } else {
+        q = it / common;
+        i = it - q * common;
       }
-      continue;
+    };
+    int64_t jcur = 0;
+    if (by_lines) {
+      int64_t i0;
+      split((int64_t)g0, jcur, i0);
+    } else {
+      int lo = 0, hi = (int)launch.njobs - 1;
+      while (lo < hi) {  // job of the first item
+        const int mid = (lo + hi + 1) >> 1;
+        if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
+      }
+      jcur = lo;
     }
-    int lo = 0, hi = (int)launch.njobs - 1;
-    while (lo < hi) {  // job of the first item
-      const int mid = (lo + hi + 1) >> 1;
-      if (launch.jobs[mid].item0 <= (int64_t)g0) lo = mid; else hi = mid - 1;
+    QueryJob J = launch.jobs[jcur];
+    const bool turns = !by_lines && launch.qblock > 0 && J.nqf > 0;
+    int64_t j0 = 0, nq = 0, flines = 0, base = 0, next0 = INT64_MAX;
+    if (turns) {
+      j0 = jcur - J.qf;
+      nq = J.nqf;
+      flines = J.lines;
+      base = launch.jobs[j0].item0;
+    } else if (!by_lines && jcur + 1 < launch.njobs) {
+      next0 = launch.jobs[jcur + 1].item0;  // first item of the next job
     }
-    QueryJob J = launch.jobs[lo];
-    if (launch.qblock > 0 && J.nqf > 0) {
-      // the queries of a file take turns over blocks of its lines: the file's
-      // items are [base, base + nq * lines), and its jobs j0 .. j0 + nq - 1
-      int64_t j0 = lo - J.qf, nq = J.nqf, lines = J.lines;
-      int64_t base = launch.jobs[j0].item0, jcur = lo;
-      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-        while (it >= base + nq * lines && j0 + nq < launch.njobs) {  // into the next file
+    for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+      int64_t jq, i = 0, lines = 0, outrow = 0;
+      if (by_lines) {
+        split(it, jq, i);
+        lines = common;
+        outrow = jq * common + i;
+      } else if (turns) {
+        while (it >= base + nq * flines && j0 + nq < launch.njobs) {  // into the next file
           j0 += nq;
           const QueryJob& F0 = launch.jobs[j0];
           nq = F0.nqf;
-          lines = F0.lines;
+          flines = F0.lines;
           base = F0.item0;
         }
-        int64_t q, i;
-        split_item(it - base, lines, nq, launch.qblock, &q, &i);
-        if (j0 + q != jcur) {
-          jcur = j0 + q;
-          J = launch.jobs[jcur];
+        int64_t q;
+        split_item(it - base, flines, nq, launch.qblock, &q, &i);
+        jq = j0 + q;
+        lines = flines;
+        outrow = i;
+      } else {
+        jq = jcur;
+        if (it >= next0) {
+          while (jq + 1 < launch.njobs && launch.jobs[jq + 1].item0 <= it) jq++;
+          next0 = jq + 1 < launch.njobs ? launch.jobs[jq + 1].item0 : INT64_MAX;
         }
-        if (defer(J, lines, i, it)) continue;
-        eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), lines, i,
-                  J.out + i * (int64_t)J.R * 2, le, ci, nslots);
       }
-      continue;
-    }
-    int64_t next0 = lo + 1 < launch.njobs ? launch.jobs[lo + 1].item0 : INT64_MAX;  // first item of the next job
-    for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
-      if (it >= next0) {
-        while (lo + 1 < launch.njobs && launch.jobs[lo + 1].item0 <= it) lo++;
-        J = launch.jobs[lo];
-        next0 = lo + 1 < launch.njobs ? launch.jobs[lo + 1].item0 : INT64_MAX;
+      if (jq != jcur) {
+        jcur = jq;
+        J = launch.jobs[jq];
       }
-      const int64_t i = it - J.item0;
-      if (defer(J, J.lines, i, it)) continue;
-      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), J.lines, i, J.out + i * (int64_t)J.R * 2, le, ci,
-                nslots);
+      if (!by_lines && !turns) {
+        i = it - J.item0;
+        lines = J.lines;
+        outrow = i;
+      }
+      if (defer(J, lines, i, it)) continue;
+      eval_line(g, J, GlobalWords{J.LB, J.Wp, J.rec, J.LS, J.Ws}, ir_of(J), lines, i,
+                J.out + outrow * (int64_t)J.R * 2, le, ci, nslots);
     }
   }
 }
