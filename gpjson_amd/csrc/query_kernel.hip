@@ -637,7 +637,7 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     if ((int64_t)w0 >= total) break;
     const unsigned long long g0 = w0 + (unsigned long long)(lane / G) * lpw;
     if ((int64_t)g0 >= total) continue;
-    if (launch.lines_dev && launch.njobs == 1) {
+    if (G > 1 && launch.lines_dev && launch.njobs == 1) {  // (G = 1: the refill above)
       // one query: its job once, in scalar registers, not reloaded per line
       const QueryJob J = launch.jobs[0];
       const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
