@@ -1697,7 +1697,9 @@ hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, 
   int64_t n8 = ntiles * 8;
   int grid = (int)((n8 + 255) / 256);
   if (grid < 1) grid = 1;
-  if (grid > 4096) grid = 4096;
+  // a conditional reset (the fallback pass) nearly always returns at once,
+  // and a launch's cost then is its workgroups' dispatch
+  if (grid > (run_if ? 256 : 4096)) grid = run_if ? 256 : 4096;
   hipLaunchKernelGGL(index_reset_kernel, dim3(grid), dim3(256), 0, stream, states, ntiles, ticket, run_if, also);
   return hipGetLastError();
 }
@@ -1717,7 +1719,7 @@ hipError_t index_finish_batch(const IndexLaunch& L, int64_t* block_sums, int64_t
   if (nt > 0) {
     hipLaunchKernelGGL(batch_tile_lines_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L,
                        (const int64_t*)block_sums, (const int64_t*)file_base, overflow);
-    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 1024);
+    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 256);  // nearly always returns at once
     hipLaunchKernelGGL(batch_tile_overflow_kernel, dim3(ogrid), dim3(256), 0, stream, L, (const int*)overflow);
   }
   return hipGetLastError();
@@ -1736,7 +1738,7 @@ hipError_t index_finish(const IndexLaunch& L, int64_t* block_sums, int64_t* line
   if (nt > 0) {
     hipLaunchKernelGGL(index_tile_lines_kernel, dim3(grid), dim3(kTileBlock), 0, stream, L,
                        (const int64_t*)block_sums, overflow);
-    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 1024);
+    const unsigned ogrid = (unsigned)std::min<int64_t>((nt + 3) / 4, 256);  // nearly always returns at once
     hipLaunchKernelGGL(index_tile_overflow_kernel, dim3(ogrid), dim3(256), 0, stream, L, (const int*)overflow);
   }
   return hipGetLastError();

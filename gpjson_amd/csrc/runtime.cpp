@@ -236,7 +236,8 @@ int enqueue_index_launches(const IndexLaunch& L, const IndexTune& t, int levels,
   L2.speculate = 0;
   L2.run_if = flags;
   GJ_HIP(index_reset(L.states, ntiles, L.ticket, stream, flags, L.fuse_count));
-  if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)ntiles, stream));
+  // a quarter of the resident grid at most: the launch nearly always returns at once
+  if (ntiles > 0) GJ_HIP(index_launch(L2, t.block, t.wpt, nlv, verify, (int)std::min<int64_t>(ntiles, 256), stream));
   GJ_HIP(index_finish(L2, block_sums, lines_dev, flags + 16, flags + 8, stream));
   return GPJSON_OK;
 }
