@@ -87,6 +87,10 @@ struct gpjson_batch {
   std::vector<int64_t> lines;  // per file, after a run
 };
 
+// dev_misc: line count total (0), overflow flag (8), index ticket (64), work
+// counters (128), then per-file line counts and per-job file slots
+static constexpr size_t kMiscCtlBytes = (128 + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
+
 static int levels_of(const uint8_t* ir, int64_t len, int* R) {
   int depth = 0, maxd = 0, stores = 0;
   for (int64_t pc = 0; pc < len;) {
@@ -172,7 +176,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   if (!rc) rc = b->bk.ensure(128 + (size_t)(tiles / 256 + 2) * 8 + (size_t)b->nfiles * 8);
   if (!rc) rc = b->dev_files.ensure(sizeof(FileJob) * (size_t)b->nfiles);
   if (!rc) rc = b->dev_qjobs.ensure(sizeof(QueryJob) * (size_t)b->njobs);
-  if (!rc) rc = b->dev_misc.ensure(4096 + (size_t)b->nfiles * 8 + (size_t)b->njobs * 4);
+  if (!rc) rc = b->dev_misc.ensure(kMiscCtlBytes + (size_t)b->nfiles * 8 + (size_t)b->njobs * 4);
   if (rc) return rc;
   for (int f = 0; f < b->nfiles; f++) {
     FileJob& J = b->files[f];
@@ -226,8 +230,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   b->overflow_dev = (int*)(misc + 8);
   b->ticket = (unsigned int*)(misc + 64);
   b->counter = (unsigned long long*)(misc + 128);
-  b->lines_dev = (int64_t*)(misc + 4096);
-  b->job_file_dev = (int*)(misc + 4096 + (size_t)b->nfiles * 8);
+  b->lines_dev = (int64_t*)(misc + kMiscCtlBytes);
+  b->job_file_dev = (int*)(misc + kMiscCtlBytes + (size_t)b->nfiles * 8);
   GJ_HIP(hipMemcpy(b->dev_ir.p, flat.data(), flat.size(), hipMemcpyHostToDevice));
   // the device tables in tile order: file table by slot, each job's file slot
   std::vector<FileJob> table(b->nfiles);
@@ -237,7 +241,7 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   GJ_HIP(hipMemcpy(b->dev_files.p, table.data(), sizeof(FileJob) * b->nfiles, hipMemcpyHostToDevice));
   GJ_HIP(hipMemcpy(b->dev_qjobs.p, b->qjobs.data(), sizeof(QueryJob) * b->njobs, hipMemcpyHostToDevice));
   GJ_HIP(hipMemcpy(b->job_file_dev, job_slot.data(), sizeof(int) * b->njobs, hipMemcpyHostToDevice));
-  GJ_HIP(hipMemset(b->dev_misc.p, 0, 4096));
+  GJ_HIP(hipMemset(b->dev_misc.p, 0, kMiscCtlBytes));
   const int bpc = 2048 / b->itune.block;  // resident blocks per CU the grid aims for
   b->index_grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ctx->num_cus * bpc));
   b->query_grid = std::max(1, ctx->num_cus * b->qtune.waves_per_cu / (b->qtune.block / 64));
@@ -287,7 +291,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
   };
   GJ_HIP(index_reset(L.states, b->total_tiles, L.ticket, s));
   GJ_HIP(zero_u64((unsigned long long*)flags, 16, s));
-  GJ_HIP(zero_u64(b->counter, 1, s));
+  GJ_HIP(zero_u64(b->counter, kQueryCounterWords, s));
   if (e0) GJ_HIP(hipEventRecord(e0, s));
   int rc = index_pass(L);
   if (rc) return rc;

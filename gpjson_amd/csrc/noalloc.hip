@@ -67,9 +67,11 @@ __global__ void clamp_lines_kernel(const int64_t* lines, int64_t cap_lines, int6
 
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
+constexpr int64_t kCtlBytes = (1024 + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
+
 // Offsets of one call's arrays in the scratch array.
 struct Layout {
-  int64_t ctl;         // 4 KiB: ticket (0), flags (64), lines (512), query lines (520), work counter (576)
+  int64_t ctl;         // kCtlBytes: ticket (0), flags (64), lines (512), query lines (520), work counters (1024)
   int64_t states;      // TileState[ntiles]
   int64_t nl_scratch;  // int64[ntiles * tile words] + block sums
   int64_t rec;         // uint64[W * kRecWords]
@@ -88,7 +90,7 @@ Layout layout_for(int64_t n, int L, const IndexTune& t, int nq, int64_t ir_bytes
   Layout o{};
   int64_t p = 0;
   o.ctl = p;
-  p += 4096;
+  p += kCtlBytes;
   o.states = p;
   p = align256(p + std::max<int64_t>(ntiles, 1) * (int64_t)sizeof(TileState));
   o.nl_scratch = p;
@@ -160,7 +162,7 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
   const int64_t W = (n + 63) / 64, Wp = (W + 7) / 8 * 8;
   int64_t* lines_dev = (int64_t*)(base + o.ctl + 512);
   int64_t* query_lines = (int64_t*)(base + o.ctl + 520);
-  unsigned long long* counter = (unsigned long long*)(base + o.ctl + 576);
+  unsigned long long* counter = (unsigned long long*)(base + o.ctl + 1024);
 
   // ---- index ----
   FileJob j{};
@@ -229,7 +231,7 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
   GJ_HIP(hipGetLastError());
 
   // ---- one interpreter launch (line mode, the context's shape) ----
-  GJ_HIP(zero_u64(counter, 1, stream));
+  GJ_HIP(zero_u64(counter, kQueryCounterWords, stream));
   QueryLaunch ql{};
   ql.jobs = jobs_dev;
   ql.njobs = nq;

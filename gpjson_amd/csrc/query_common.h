@@ -53,7 +53,7 @@ struct QueryLaunch {
   // When set (and lines_dev is null), the job table's lines / item0 were
   // written on the device and the item count is read from here.
   const int64_t* total_dev;
-  unsigned long long* counter;    // zeroed before launch
+  unsigned long long* counter;    // kQueryCounterWords words, zeroed before launch
   int lines_per_wave;             // lines each lane group takes per wavefront claim
   // Long lines (line mode): a lane group that claims a line longer than
   // long_min bytes appends the work item to long_items (up to long_cap items)
@@ -103,6 +103,16 @@ inline void split_item(int64_t r, int64_t lines, int64_t nq, int64_t B, int64_t*
 }
 
 constexpr int kIrLdsBytes = 4096;
+// Work counters of an interpreter launch: kQuerySegments counters,
+// kQueryCounterStride words (256 bytes) apart, at QueryLaunch.counter; the
+// host zeroes kQueryCounterWords words there before each launch.  A launch
+// splits its work items into up to kQuerySegments ranges with a counter each,
+// so its claims do not all queue on one address.
+constexpr int kQuerySegments = 8;
+constexpr int kQueryCounterStride = 32;
+constexpr int kQueryCounterWords = kQuerySegments * kQueryCounterStride;
+static_assert(kQuerySegments < 32 && (kQuerySegments & (kQuerySegments - 1)) == 0,
+              "a power of two of ranges, tracked in a 32-bit mask");
 // Lines at least this long are evaluated by a whole wavefront (line mode).
 constexpr int64_t kLongLineBytes = 8192;
 

@@ -58,6 +58,34 @@ __device__ __forceinline__ unsigned long long shfl64_lane(unsigned long long v, 
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// A launch's work items [0, total) lie in 2^lg ranges (at most
+// kQuerySegments), each with a counter of its own, kQueryCounterStride words
+// from the next (atomics on one cache line take turns).  A wavefront claims up
+// to n consecutive items of range seg; once seg is used up it moves on to the
+// lowest range not yet seen used up (live).  Wavefront-uniform; false once
+// every range is used up.  (Reading every counter at once when a range runs
+// out, instead of trying the ranges one by one, ends a launch sooner but
+// costs the interpreter spilled registers, and is slower overall.)
+__device__ __forceinline__ bool claim_items(unsigned long long* counter, int64_t total, int lg, int64_t n, int lane,
+                                            int& seg, unsigned& live, int64_t& lo, int64_t& hi) {
+  for (;;) {
+    if (live >> seg & 1u) {
+      unsigned long long c = 0;
+      if (lane == 0) c = atomicAdd(counter + seg * kQueryCounterStride, (unsigned long long)n);
+      const int64_t f = ((total * seg) >> lg) + (int64_t)shfl64_lane(c, 0);
+      const int64_t e = (total * (seg + 1)) >> lg;
+      if (f < e) {
+        lo = f;
+        hi = min(f + n, e);
+        return true;
+      }
+      live &= ~(1u << seg);
+    }
+    if (!live) return false;
+    seg = __builtin_ctz(live);
+  }
+}
+
 // Lanes [base, base+G) of the wavefront work on one line.
 template <int G>
 struct Group {
@@ -568,6 +596,14 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
   // 64/G times fewer atomics than with per-group claims.
   constexpr int kGroupsPerWave = 64 / G;
   const int lane = __lane_id();
+  // Ranges of work items: 2^lg of them, one per kSegItems items or more, at
+  // most kQuerySegments (few items make few claims, which one counter
+  // serves); the wavefront starts in range wavefront % 2^lg.
+  constexpr int64_t kSegItems = 65536;
+  int lg = 0;
+  while ((2 << lg) <= kQuerySegments && total >= (kSegItems << (lg + 1))) lg++;
+  int seg = (int)((blockIdx.x * (BLOCK / 64) + threadIdx.x / 64) & ((1 << lg) - 1));
+  unsigned live = (1u << (1 << lg)) - 1u;
   // defers a long line to the wavefront-per-line launch (false: list full)
   auto defer = [&](const QueryJob& J, int64_t lines, int64_t i, int64_t it) -> bool {
     if (!launch.long_items || G == 64) return false;
@@ -586,7 +622,7 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     // out instead of waiting at the end of every claim for the slowest one.
     // Every round each lane with a line runs its IR ops up to the next wait
     // and that wait; lanes without one take the next items of the wavefront's
-    // pool, which claims 64 items from the counter when it runs short.
+    // pool, which claims 64 items when it runs short.
     const QueryJob J = launch.jobs[0];
     const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
     const uint8_t* ir = ir_of(J);
@@ -599,28 +635,24 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
       if (need) {
         const int k = __popcll(need);
         const int64_t avail = pool_end - pool;
-        int64_t fresh = 0;
-        if (avail < k) {
-          unsigned long long c = 0;
-          if (lane == 0) c = atomicAdd(launch.counter, 64ull);
-          fresh = (int64_t)shfl64_lane(c, 0);
-        }
+        int64_t fresh = 0, fresh_end = 0;  // a new claim [fresh, fresh_end)
+        const bool claimed = avail < k && claim_items(launch.counter, total, lg, 64, lane, seg, live, fresh, fresh_end);
         if (!has && !done) {
           const int rank = __popcll(need & below);
-          const int64_t item = rank < avail ? pool + rank : fresh + (rank - avail);
-          if (item >= total) {
-            done = true;
-          } else {
+          if (rank < avail || (claimed && fresh + (rank - avail) < fresh_end)) {
+            const int64_t item = rank < avail ? pool + rank : fresh + (rank - avail);
             const int64_t i = launch.line_list ? launch.line_list[item] : item;
             if (i >= 0 && i < common && !defer(J, common, i, item)) {
               line_begin(g, J, common, i, J.out + i * (int64_t)J.R * 2, le, ci, nslots, st);
               has = true;
             }
+          } else if (avail < k && !claimed) {
+            done = true;  // every range is used up
           }
         }
         if (avail < k) {
-          pool = fresh + (k - avail);
-          pool_end = fresh + 64;
+          pool = min(fresh + (k - avail), fresh_end);
+          pool_end = fresh_end;
         } else {
           pool += k;
         }
@@ -631,18 +663,16 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     return;
   }
   for (;;) {
-    unsigned long long w0 = 0;
-    if (lane == 0) w0 = atomicAdd(launch.counter, (unsigned long long)kGroupsPerWave * lpw);
-    w0 = shfl64_lane(w0, 0);
-    if ((int64_t)w0 >= total) break;
-    const unsigned long long g0 = w0 + (unsigned long long)(lane / G) * lpw;
-    if ((int64_t)g0 >= total) continue;
+    int64_t w0 = 0, wend = 0;  // the wavefront's claim [w0, wend)
+    if (!claim_items(launch.counter, total, lg, (int64_t)kGroupsPerWave * lpw, lane, seg, live, w0, wend)) break;
+    const int64_t g0 = w0 + (int64_t)(lane / G) * lpw;
+    if (g0 >= wend) continue;
     if (G > 1 && launch.lines_dev && launch.njobs == 1) {  // (G = 1: the refill above)
       // one query: its job once, in scalar registers, not reloaded per line
       const QueryJob J = launch.jobs[0];
       const GlobalWords W{J.LB, J.Wp, J.rec, J.LS, J.Ws};
       const uint8_t* ir = ir_of(J);
-      for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+      for (int64_t it = g0; it < g0 + lpw && it < wend; it++) {
         const int64_t i = launch.line_list ? launch.line_list[it] : it;
         if (i < 0 || i >= common) continue;
         if (defer(J, common, i, it)) continue;
@@ -690,7 +720,7 @@ __global__ __launch_bounds__(BLOCK, 4) void query_kernel(QueryLaunch launch, int
     } else if (!by_lines && jcur + 1 < launch.njobs) {
       next0 = launch.jobs[jcur + 1].item0;  // first item of the next job
     }
-    for (int64_t it = (int64_t)g0; it < (int64_t)g0 + lpw && it < total; it++) {
+    for (int64_t it = g0; it < g0 + lpw && it < wend; it++) {
       int64_t jq, i = 0, lines = 0, outrow = 0;
       if (by_lines) {
         split(it, jq, i);

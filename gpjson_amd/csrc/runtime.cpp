@@ -323,9 +323,9 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
                            PreparedQueries* out) {
   int64_t ir_total = 0;
   for (int q = 0; q < nq; q++) ir_total += ir_lens[q];
-  // table layout: [work counter, 64 B][QueryJob x nq][IR bytes][long-line
+  // table layout: [work counters][QueryJob x nq][IR bytes][long-line
   // counters, 64 B][long-line items]
-  const size_t jobs_off = 64;
+  const size_t jobs_off = kQueryCounterWords * 8;
   const size_t jobs_bytes = sizeof(QueryJob) * (size_t)nq;
   const size_t long_off = (jobs_off + jobs_bytes + (size_t)ir_total + 16 + 63) / 64 * 64;
   const int64_t long_cap = (int64_t)nq * (idx->n / kLongLineBytes + 1);
@@ -394,7 +394,7 @@ int prepare_queries(gpjson_ctx* ctx, const gpjson_index* idx, const uint8_t* con
 }
 
 int launch_prepared(gpjson_ctx* ctx, const PreparedQueries& pq, hipEvent_t before_kernel, hipEvent_t after_kernel) {
-  GJ_HIP(zero_u64(pq.launch.counter, 1, ctx->stream));
+  GJ_HIP(zero_u64(pq.launch.counter, kQueryCounterWords, ctx->stream));
   if (pq.launch.long_count) GJ_HIP(zero_u64(pq.launch.long_count, 2, ctx->stream));
   if (before_kernel) GJ_HIP(hipEventRecord(before_kernel, ctx->stream));
   GJ_HIP(query_launch(pq.launch, pq.block, pq.group, pq.levels, pq.grid, ctx->stream, pq.long_grid));

@@ -286,8 +286,8 @@ int launch_batch(gpjson_server* s, Batch& b) {
   int maxL = 0;
   b.offsets.assign(n, 0);
   for (int i = 0; i < n; i++) ir_total += (int64_t)b.reqs[i]->cp.ir.size();
-  // device layout: [counter (64 B)][jobs][IR bytes]
-  const size_t jobs_off = 64;
+  // device layout: [work counters][jobs][IR bytes]
+  const size_t jobs_off = kQueryCounterWords * 8;
   const size_t jobs_bytes = sizeof(QueryJob) * n;
   const size_t table_bytes = jobs_off + jobs_bytes + (size_t)ir_total + 16;
   int rc = b.jobs.ensure(table_bytes);
@@ -354,7 +354,7 @@ int launch_batch(gpjson_server* s, Batch& b) {
   L.lines_per_wave = ctx->qtune.lines_per_wave;
   L.ir_base = (const uint8_t*)b.jobs.p + jobs_off + jobs_bytes;
   L.ir_bytes = ir_total;
-  GJ_HIP(zero_u64(L.counter, 1, stream));
+  GJ_HIP(zero_u64(L.counter, kQueryCounterWords, stream));
   const int waves_per_block = ctx->qtune.block / 64;
   const int grid = std::max(1, ctx->num_cus * ctx->qtune.waves_per_cu / waves_per_block);
   GJ_HIP(query_launch(L, ctx->qtune.block, ctx->qtune.group, maxL, grid, stream));

@@ -334,3 +334,28 @@ def test_large_against_mt_oracle(ctx):
     got = ctx.run_queries(idx, [cp.ir], 1)
     ref = O.find_value_mt(data, nl, s, lb, 2, cp.ir, 1, 16)
     assert np.array_equal(got[0].reshape(-1), ref)
+
+
+def short_lines(rng, lines: int) -> bytes:
+    """NDJSON of `lines` short records; every seventh lacks "a", every eleventh is blank."""
+    v = rng.integers(0, 10 ** 6, size=lines)
+    out = []
+    for i in range(lines):
+        if i % 11 == 5:
+            out.append("")
+        elif i % 7 == 3:
+            out.append('{"c":[%d]}' % v[i])
+        else:
+            out.append('{"a":{"b":%d},"c":[%d,"x%d"]}' % (v[i], i, v[i] % 97))
+    return ("\n".join(out) + "\n").encode()
+
+
+@pytest.mark.parametrize("lines", [(1 << 17) + 13, (1 << 18) + 5, 600001])
+def test_many_short_lines(ctx, lines):
+    """An interpreter launch of 2^17 work items or more splits them into 2, 4 or 8
+    ranges with a counter each: one query (the lane refill) and three queries (the
+    claim loop, 3 * lines items) over every line, against the oracle."""
+    data = short_lines(np.random.default_rng(lines), lines)
+    assert data.count(b"\n") == lines
+    spans_equal(ctx, data, ["$.a.b"], f"lines={lines}")
+    spans_equal(ctx, data, ["$.a.b", "$.c[1]", "$.c[0]"], f"lines={lines}")
