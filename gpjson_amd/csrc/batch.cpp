@@ -87,9 +87,12 @@ struct gpjson_batch {
   std::vector<int64_t> lines;  // per file, after a run
 };
 
-// dev_misc: line count total (0), overflow flag (8), index ticket (64), work
-// counters (128), then per-file line counts and per-job file slots
-static constexpr size_t kMiscCtlBytes = (128 + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
+// dev_misc: line count total (0), overflow flag (8), index tickets
+// (kMiscTicket), work counters (kMiscCounters), then per-file line counts and
+// per-job file slots
+static constexpr size_t kMiscTicket = 256;
+static constexpr size_t kMiscCounters = kMiscTicket + kTicketBytes;
+static constexpr size_t kMiscCtlBytes = (kMiscCounters + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
 
 static int levels_of(const uint8_t* ir, int64_t len, int* R) {
   int depth = 0, maxd = 0, stores = 0;
@@ -228,8 +231,8 @@ static int layout(gpjson_batch* b, const std::vector<int64_t>& nl_caps) {
   uint8_t* misc = (uint8_t*)b->dev_misc.p;
   b->total_dev = (int64_t*)misc;
   b->overflow_dev = (int*)(misc + 8);
-  b->ticket = (unsigned int*)(misc + 64);
-  b->counter = (unsigned long long*)(misc + 128);
+  b->ticket = (unsigned int*)(misc + kMiscTicket);
+  b->counter = (unsigned long long*)(misc + kMiscCounters);
   b->lines_dev = (int64_t*)(misc + kMiscCtlBytes);
   b->job_file_dev = (int*)(misc + kMiscCtlBytes + (size_t)b->nfiles * 8);
   GJ_HIP(hipMemcpy(b->dev_ir.p, flat.data(), flat.size(), hipMemcpyHostToDevice));
@@ -284,7 +287,7 @@ static int enqueue_batch(gpjson_batch* b, hipEvent_t e0, hipEvent_t e1, hipEvent
                         (int)std::min<int64_t>(b->index_grid, b->split_tile), s));
     IndexLaunch B = P;
     B.tile_begin = b->split_tile;
-    GJ_HIP(zero_u64((unsigned long long*)P.ticket, 1, s));
+    GJ_HIP(zero_u64((unsigned long long*)P.ticket, kTicketBytes / 8, s));
     GJ_HIP(index_launch(B, b->itune.block, b->itune.wpt, nlv, false,
                         (int)std::min<int64_t>(b->index_grid, b->total_tiles - b->split_tile), s));
     return GPJSON_OK;

@@ -76,6 +76,15 @@ struct FileJob {
 constexpr int kFuseMaxDepth = 3;
 constexpr int kFuseKeyBytes = 120;
 
+// Tile tickets of an index launch: kTicketParts counters, kTicketStride
+// 32-bit words (256 bytes) apart, in kTicketBytes at IndexLaunch.ticket.
+// Block b of a grid of at least kTicketParts blocks claims the tiles
+// p, p + P, p + 2P, ... (p = b % P, P = kTicketParts) from counter p: atomics on
+// one address take turns, and one counter was the index launch's limit.
+constexpr int kTicketParts = 8;
+constexpr int kTicketStride = 64;
+constexpr int kTicketBytes = kTicketParts * kTicketStride * 4;
+
 struct IndexLaunch {
   const FileJob* jobs;     // device array, or null: the single job `one`
   FileJob one;
@@ -83,7 +92,7 @@ struct IndexLaunch {
   int64_t total_tiles;     // tiles [tile_begin, total_tiles) are this launch's
   int64_t tile_begin;      // earlier tiles were indexed by earlier launches
   TileState* states;       // total_tiles entries, zeroed before the first launch
-  unsigned int* ticket;    // zeroed before launch
+  unsigned int* ticket;    // kTicketBytes: kTicketParts tile tickets, zeroed before launch
   int speculate;           // tiles holding a newline may speculate their carry-in
   // Deferred newline bookkeeping (one file): each tile writes its newline
   // offsets to nl_scratch[tile * nl_scratch_cap + k] and its carry-in to its

@@ -5,8 +5,8 @@
 // J/stages/LeveledBitmapsIndex.java:40-103; kernels K/create_combined_*.cu,
 // K/create_quote_index.cu, K/create_string_index.cu, K/create_leveled_*.cu):
 //   newline index   int64[lines]   ([0] = 0, then every '\n')
-//   records         uint64[W][3]   (in-string bits, raw ':' and '"' bytes)
-//   leveled bitmaps uint64[L][Wp]  (structural chars per depth level)
+//   records         uint64[W][4]   (in-string bits, raw ':' and '"' bytes, level 0)
+//   leveled bitmaps uint64[L-1][Wp] (structural chars per depth level 1 .. L-1)
 //   level summaries uint64[L][Ws]  (a bit per leveled word that is not zero)
 //   escape / quote  uint64[W]      (only in verify mode)
 //
@@ -16,8 +16,10 @@
 // (16-byte loads), classifies them with byte-parallel (SWAR) compares into
 // 64-bit masks, and every index store of the wavefront is coalesced.  All
 // further bit arithmetic is word-parallel across the wavefront.  Tiles are
-// claimed in order through an atomic ticket, so every tile's predecessors are
-// already running.  Scans run in row-major word order.
+// claimed in increasing order per block through atomic tickets (one counter
+// per residue class of tiles, so the claims do not all queue on one address),
+// so the lowest unfinished tile is always one a block is working on.  Scans
+// run in row-major word order.
 //
 // Cross-word state is a transfer function, not a value: a word's escape carry
 // out is fixed unless the word is all backslashes, and given the escape carry,
@@ -694,10 +696,18 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
 
   // Thread 0 claims each tile one tile ahead: the ticket for the block's next
   // tile is taken when its current tile starts, so the atomic's round trip
-  // overlaps the current tile instead of standing before the next one.  Claims
-  // stay in increasing order per block, so the lowest unfinished tile is always
-  // one a block is working on (its predecessors are all claimed, and finish
-  // first by the same argument): the lookback still cannot wait forever.
+  // overlaps the current tile instead of standing before the next one.  The
+  // tiles lie in npart residue classes (tile % npart), each with its counter;
+  // block b claims from class b % npart, and every class has a block because
+  // npart <= gridDim.x.  Claims stay in increasing order per block, and a
+  // class's tiles are claimed in increasing order, so the lowest unfinished
+  // tile is always one a block is working on or its class's next (its class's
+  // blocks have finished every earlier tile of the class, so one of them claims
+  // it next): the lookback still cannot wait forever.
+  unsigned npart = 1;
+  while (npart * 2 <= (unsigned)kTicketParts && npart * 2 <= gridDim.x) npart *= 2;
+  const unsigned part = blockIdx.x & (npart - 1);
+  unsigned int* const my_ticket = launch.ticket + part * kTicketStride;
   unsigned next_ticket = 0;            // thread 0 only
   int cur_job = -1;                    // thread 0 only: the job of the last tile prepared
   int64_t cur_end = 0;                 // thread 0 only: its first tile past the job
@@ -712,7 +722,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     q.lnl = -1;
     q.ncand = 0;
     if (t >= launch.total_tiles) return;
-    next_ticket = atomicAdd(launch.ticket, 1u);
+    next_ticket = atomicAdd(my_ticket, 1u) * npart + part;
     int lo = 0;
     if (launch.jobs) {
       if (cur_job >= 0 && (int64_t)t < cur_end) {
@@ -735,7 +745,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     }
   };
   if (tid == 0) {
-    next_ticket = atomicAdd(launch.ticket, 1u);
+    next_ticket = atomicAdd(my_ticket, 1u) * npart + part;
     prepare(0);
   }
   __syncthreads();
@@ -1195,7 +1205,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
   }
 }
 
-// Zeroes the lookback states and the ticket (a kernel, not hipMemsetAsync, so
+// Zeroes the lookback states and the tickets (a kernel, not hipMemsetAsync, so
 // the index pipeline can be captured into a graph).
 __global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* ticket, const int* run_if,
                                    unsigned long long* also) {
@@ -1204,10 +1214,8 @@ __global__ void index_reset_kernel(TileState* states, int64_t n, unsigned int* t
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long* p = reinterpret_cast<unsigned long long*>(states);
   for (int64_t k = i; k < n * 8; k += stride) p[k] = 0;
-  if (i == 0) {
-    *ticket = 0;
-    if (also) *also = 0;
-  }
+  if (i < kTicketParts) ticket[i * kTicketStride] = 0;
+  if (i == 0 && also) *also = 0;
 }
 
 // The carry-in a tile stored for the deferred bookkeeping.

@@ -67,11 +67,16 @@ __global__ void clamp_lines_kernel(const int64_t* lines, int64_t cap_lines, int6
 
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-constexpr int64_t kCtlBytes = (1024 + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
+// control block: index tickets (0), bookkeeping flags (kCtlFlags), line count
+// (kCtlLines), query line count (kCtlLines + 8), work counters (kCtlCounters)
+constexpr int64_t kCtlFlags = kTicketBytes;
+constexpr int64_t kCtlLines = kCtlFlags + 512;
+constexpr int64_t kCtlCounters = kCtlLines + 512;
+constexpr int64_t kCtlBytes = (kCtlCounters + kQueryCounterWords * 8 + 4095) / 4096 * 4096;
 
 // Offsets of one call's arrays in the scratch array.
 struct Layout {
-  int64_t ctl;         // kCtlBytes: ticket (0), flags (64), lines (512), query lines (520), work counters (1024)
+  int64_t ctl;         // kCtlBytes of control block
   int64_t states;      // TileState[ntiles]
   int64_t nl_scratch;  // int64[ntiles * tile words] + block sums
   int64_t rec;         // uint64[W * kRecWords]
@@ -160,9 +165,9 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
   uint8_t* base = (uint8_t*)ctx->noalloc_scratch;
   hipStream_t stream = ctx->stream;
   const int64_t W = (n + 63) / 64, Wp = (W + 7) / 8 * 8;
-  int64_t* lines_dev = (int64_t*)(base + o.ctl + 512);
-  int64_t* query_lines = (int64_t*)(base + o.ctl + 520);
-  unsigned long long* counter = (unsigned long long*)(base + o.ctl + 1024);
+  int64_t* lines_dev = (int64_t*)(base + o.ctl + kCtlLines);
+  int64_t* query_lines = (int64_t*)(base + o.ctl + kCtlLines + 8);
+  unsigned long long* counter = (unsigned long long*)(base + o.ctl + kCtlCounters);
 
   // ---- index ----
   FileJob j{};
@@ -189,7 +194,7 @@ extern "C" int gpjson_query_noalloc(gpjson_ctx* ctx, const uint8_t* input, int64
   il.speculate = 1;
   il.nl_scratch_cap = index_tile_bytes(t.block, t.wpt) / 64;
   il.nl_scratch = (int64_t*)(base + o.nl_scratch);
-  int rc = enqueue_index_launches(il, t, L, false, (int*)(base + o.ctl + 64), lines_dev, stream);
+  int rc = enqueue_index_launches(il, t, L, false, (int*)(base + o.ctl + kCtlFlags), lines_dev, stream);
   if (rc) return rc;
 
   // ---- lines the outputs hold ----

@@ -310,10 +310,11 @@ int enqueue_index(gpjson_index* idx, hipEvent_t before_kernel, hipEvent_t after_
     L.fuse_list_cap = fuse->list_cap;
     L.fuse_count = fuse->count;
   }
-  // bookkeeping flags, after the ticket: [16] a speculated carry-in was wrong
-  // (it gates the second pass), [24] a tile overflowed its scratch row, [32]
+  // bookkeeping flags, after the tickets: [0] a speculated carry-in was wrong
+  // (it gates the second pass), [8] a tile overflowed its scratch row, [16]
   // the second pass's check (not read)
-  int* flags = (int*)ctx->ticket.p + 16;
+  static_assert(kTicketBytes + 128 <= 4096, "flags past the ticket buffer");
+  int* flags = (int*)((uint8_t*)ctx->ticket.p + kTicketBytes);
   return enqueue_index_launches(L, t, idx->L, (idx->flags & GPJSON_BUILD_VERIFY) != 0, flags, idx->lines_dev,
                                 ctx->stream, before_kernel, after_kernel);
 }
@@ -796,7 +797,7 @@ static int index_file(gpjson_ctx* ctx, const char* path, int levels, int flags, 
         GJ_HIP(hipStreamWaitEvent(ctx->stream, ctx->pin_ev[pb], 0));
         L.tile_begin = done_tiles;
         L.total_tiles = end_tiles;
-        GJ_HIP(zero_u64((unsigned long long*)L.ticket, 1, ctx->stream));
+        GJ_HIP(zero_u64((unsigned long long*)L.ticket, kTicketBytes / 8, ctx->stream));
         GJ_HIP(index_launch(L, t.block, t.wpt, nlv, verify, (int)(end_tiles - done_tiles), ctx->stream));
         done_tiles = end_tiles;
       }

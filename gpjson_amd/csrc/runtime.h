@@ -17,7 +17,7 @@ namespace gj {
 // Kernel entry points (index_kernel.hip, query_kernel.hip).
 hipError_t index_launch(const IndexLaunch& L, int block, int wpt, int nlv, bool verify, int grid,
                         hipStream_t stream);
-// Zeroes the tile states and the ticket (and *also, when set) with a kernel.
+// Zeroes the tile states and the tickets (and *also, when set) with a kernel.
 hipError_t index_reset(TileState* states, int64_t ntiles, unsigned int* ticket, hipStream_t stream,
                        const int* run_if = nullptr, unsigned long long* also = nullptr);
 hipError_t index_lines(const IndexLaunch& L, int64_t* lines_out, hipStream_t stream);
@@ -131,7 +131,7 @@ struct gpjson_ctx {
   hipStream_t copy_stream = nullptr;  // host-to-device staging overlapped with indexing
   int num_cus = 0;
   gj::DevBuf states;      // lookback tile states
-  gj::DevBuf ticket;      // index ticket + query counter + line counts
+  gj::DevBuf ticket;      // index tickets (kTicketBytes), then the bookkeeping flags
   gj::DevBuf jobs;        // device job arrays
   gj::DevBuf irbuf;       // IR bytes of a query launch
   gj::DevBuf nl_scratch;  // an index build's newline offsets per tile, before compaction

@@ -10,7 +10,10 @@
 // patterns:
 //   - one launch over all tiles (the plain and batched index builds);
 //   - several launches over consecutive tile ranges [tile_begin, end) with the
-//     ticket reset before each, the tile states kept (the staging pipeline);
+//     tickets reset before each, the tile states kept (the staging pipeline);
+// tiles are claimed as the kernel claims them: npart residue classes (the
+// largest power of two up to 8 and the grid), block b claiming the tiles of
+// class b % npart in order from that class's counter;
 // and tile kinds:
 //   - tiles that look back before emitting;
 //   - speculating tiles that publish their inclusive prefix at once (deferred
@@ -64,7 +67,7 @@ struct Model {
   bool deferred;               // speculating tiles publish their inclusive prefix at once
   uint64_t start = 12345;      // carry before tile 0
   std::atomic<bool> abort{false};
-  std::atomic<unsigned> ticket{0};
+  std::atomic<unsigned> ticket[8];    // a counter per residue class of tiles
   int delay_pct;               // chance of a delay at each step
 
   explicit Model(int64_t n) : ntiles(n), tile_agg(n), spec(n), st(n), carry_in(n), prefix(n) {}
@@ -178,15 +181,20 @@ void publish_inc(Model& m, int64_t lt, uint64_t v) {
 // One block of the persistent grid: claims tiles of [begin, end) until none
 // are left (the kernel's loop body, minus the data work).  As in the kernel,
 // the ticket for the block's next tile is claimed when its current tile
-// starts, so a block always holds one claimed tile it has not started.
-void block_main(Model& m, int64_t begin, int64_t end, uint64_t seed) {
+// starts, so a block always holds one claimed tile it has not started, and
+// block b of a grid of `grid` claims the tiles of class b % npart.
+void block_main(Model& m, int64_t begin, int64_t end, int b, int grid, uint64_t seed) {
   std::mt19937_64 rng(seed);
-  int64_t next = begin + (int64_t)m.ticket.fetch_add(1);
+  unsigned npart = 1;
+  while (npart * 2 <= 8u && npart * 2 <= (unsigned)grid) npart *= 2;
+  const unsigned part = (unsigned)b & (npart - 1);
+  auto claim = [&] { return begin + (int64_t)m.ticket[part].fetch_add(1) * npart + part; };
+  int64_t next = claim();
   for (;;) {
     if (m.abort.load()) return;
     const int64_t lt = next;
     if (lt >= end) return;
-    next = begin + (int64_t)m.ticket.fetch_add(1);
+    next = claim();
     maybe_delay(m, rng);
     const Agg own = m.tile_agg[lt];
     // every tile publishes its aggregate first
@@ -221,7 +229,7 @@ void block_main(Model& m, int64_t begin, int64_t end, uint64_t seed) {
 }  // namespace
 
 extern "C" {
-// Runs the launches [bounds[k], bounds[k+1]) one after another (the ticket
+// Runs the launches [bounds[k], bounds[k+1]) one after another (the tickets
 // reset before each, states kept), each on `grid` threads.  spec: tiles that
 // speculate (1) or look back (0); deferred: speculating tiles publish their
 // prefix at once.  Returns 0 when every launch ended and every carry-in is
@@ -243,9 +251,10 @@ int lb_run(int64_t ntiles, const int64_t* bounds, int nlaunch, int grid, const u
   m.deferred = deferred != 0;
   m.delay_pct = delay_pct;
   for (int k = 0; k < nlaunch; k++) {
-    m.ticket.store(0);
+    for (auto& t : m.ticket) t.store(0);
     std::vector<std::thread> th;
-    for (int b = 0; b < grid; b++) th.emplace_back(block_main, std::ref(m), bounds[k], bounds[k + 1], seed * 7919 + b + k * 104729);
+    for (int b = 0; b < grid; b++)
+      th.emplace_back(block_main, std::ref(m), bounds[k], bounds[k + 1], b, grid, seed * 7919 + b + k * 104729);
     // watchdog: a launch that does not end is reported, and its waiters released
     std::atomic<bool> done{false};
     std::thread dog([&] {

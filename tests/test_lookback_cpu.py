@@ -1,10 +1,12 @@
 """CPU model of the index kernel's carry-in protocol (tests/harness/
 lookback_model.cpp): threads as the persistent grid's blocks claim tiles from
-a ticket, publish aggregates, walk back over tiles and 64-tile groups, and
+per-class tickets (block b takes the tiles of class b % npart, npart the
+largest power of two up to 8 and the grid), publish aggregates, walk back over
+tiles and 64-tile groups, and
 publish inclusive prefixes, with random delays at every step.  Every launch
 must end and every carry-in must equal the sequential prefix: for one launch
 over all tiles, and for the staging pipeline's form -- consecutive launches
-over [tile_begin, end) with the ticket reset before each, tile states kept,
+over [tile_begin, end) with the tickets reset before each, tile states kept,
 grids smaller than the launch, and speculating tiles that publish their prefix
 only after their own walk."""
 import ctypes
@@ -58,3 +60,11 @@ def test_grid_of_one(lib):
     """A single block claims every tile in order: each walk only reads tiles
     that are already done."""
     assert run(lib, 3000, 700, 1, 0.5, False, 5, delay_pct=0) == 0
+
+
+@pytest.mark.parametrize("grid", [2, 3, 5, 8, 9, 13])
+def test_small_grids_take_fewer_classes(lib, grid):
+    """Grids below 8 blocks split the tiles into 2 or 4 classes (every class
+    has a block); grids of 8 or more into 8, some classes with more blocks."""
+    for spec_frac in (0.0, 0.6):
+        assert run(lib, 1500, 1500, grid, spec_frac, False, 31 + grid) == 0, (grid, spec_frac)
