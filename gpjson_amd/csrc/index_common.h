@@ -79,9 +79,13 @@ constexpr int kFuseKeyBytes = 120;
 // Tile tickets of an index launch: kTicketParts counters, kTicketStride
 // 32-bit words (256 bytes) apart, in kTicketBytes at IndexLaunch.ticket.
 // Block b of a grid of at least kTicketParts blocks claims the tiles
-// p, p + P, p + 2P, ... (p = b % P, P = kTicketParts) from counter p: atomics on
-// one address take turns, and one counter was the index launch's limit.
-constexpr int kTicketParts = 8;
+// p, p + P, p + 2P, ... (p = b % P, P = kTicketParts) from counter p.  With
+// P = 8 the launch ran 4 % faster alone (one address's atomics take turns),
+// but its progress then needs a running block of every class, and a class
+// can wait for CUs that other kernels hold (a server builds indexes beside
+// its query launches); a test of that mix hung once, with no cause shown, so
+// P stays 1: one counter, and any running block keeps the launch going.
+constexpr int kTicketParts = 1;
 constexpr int kTicketStride = 64;
 constexpr int kTicketBytes = kTicketParts * kTicketStride * 4;
 

@@ -440,8 +440,11 @@ constexpr int kSlotS0 = 0, kSlotNL = 1, kSlotO = 2, kSlotC = 3, kSlotP = 4, kSlo
 // the wavefront's 64 words is one work item.  Per wavefront: the word's level
 // words (up to kSpreadLevels x 64) and the work list (kBracketCap items of
 // word lane << 6 | bit); a row with more brackets than that takes the
-// per-word loop.
-constexpr int kBracketCap = 128;
+// per-word loop.  320 items (5 rounds of the wavefront) is as many as fit
+// beside the rest of the block's LDS at 4 blocks per CU (160 KiB): on
+// bracket-dense input most rows then stay on the spread path (128 items left
+// more than half of them to the per-word loop).
+constexpr int kBracketCap = 320;
 constexpr int kSpreadLevels = 4;  // levels per pass over the brackets
 
 // Inclusive prefix sum of an int over the wavefront (DPP, as wave_add_scan).

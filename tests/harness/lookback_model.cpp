@@ -12,8 +12,9 @@
 //   - several launches over consecutive tile ranges [tile_begin, end) with the
 //     tickets reset before each, the tile states kept (the staging pipeline);
 // tiles are claimed as the kernel claims them: npart residue classes (the
-// largest power of two up to 8 and the grid), block b claiming the tiles of
-// class b % npart in order from that class's counter;
+// largest power of two up to the kernel's kTicketParts -- set here by
+// lb_set_parts, 1 by default as in the kernel -- and the grid), block b
+// claiming the tiles of class b % npart in order from that class's counter;
 // and tile kinds:
 //   - tiles that look back before emitting;
 //   - speculating tiles that publish their inclusive prefix at once (deferred
@@ -183,10 +184,12 @@ void publish_inc(Model& m, int64_t lt, uint64_t v) {
 // the ticket for the block's next tile is claimed when its current tile
 // starts, so a block always holds one claimed tile it has not started, and
 // block b of a grid of `grid` claims the tiles of class b % npart.
+unsigned g_max_parts = 1;
+
 void block_main(Model& m, int64_t begin, int64_t end, int b, int grid, uint64_t seed) {
   std::mt19937_64 rng(seed);
   unsigned npart = 1;
-  while (npart * 2 <= 8u && npart * 2 <= (unsigned)grid) npart *= 2;
+  while (npart * 2 <= g_max_parts && npart * 2 <= (unsigned)grid) npart *= 2;
   const unsigned part = (unsigned)b & (npart - 1);
   auto claim = [&] { return begin + (int64_t)m.ticket[part].fetch_add(1) * npart + part; };
   int64_t next = claim();
@@ -229,6 +232,9 @@ void block_main(Model& m, int64_t begin, int64_t end, int b, int grid, uint64_t 
 }  // namespace
 
 extern "C" {
+// Largest number of ticket classes (a power of two, at most 8).
+void lb_set_parts(int parts) { g_max_parts = (unsigned)parts; }
+
 // Runs the launches [bounds[k], bounds[k+1]) one after another (the tickets
 // reset before each, states kept), each on `grid` threads.  spec: tiles that
 // speculate (1) or look back (0); deferred: speculating tiles publish their

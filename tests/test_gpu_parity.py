@@ -359,3 +359,16 @@ def test_many_short_lines(ctx, lines):
     assert data.count(b"\n") == lines
     spans_equal(ctx, data, ["$.a.b"], f"lines={lines}")
     spans_equal(ctx, data, ["$.a.b", "$.c[1]", "$.c[0]"], f"lines={lines}")
+
+
+@pytest.mark.parametrize("pairs", [157, 158, 159, 318])
+def test_rows_at_the_bracket_list_capacity(ctx, pairs):
+    """Every 4096-byte row (a wavefront's 64 words) is one line `{"x":[[],[],...]}`
+    of 4 + 2 * pairs brackets, padded with spaces: 318, 320, 322 and 640
+    brackets per row, on both sides of the spread list's 320 items."""
+    body = '{"x":[' + ",".join(["[]"] * pairs) + "]}"
+    line = (body + " " * (4095 - len(body)) + "\n").encode()
+    assert len(line) == 4096 and line.count(b"[") + line.count(b"]") + 2 == 4 + 2 * pairs
+    data = line * 24
+    check_index(ctx, data, 3, f"pairs={pairs}")
+    spans_equal(ctx, data, ["$.x[0]", "$.x[100]", f"$.x[{pairs - 1}]"], f"pairs={pairs}")

@@ -25,6 +25,7 @@ def lib(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-pthread",
                     os.path.join(ROOT, "tests", "harness", "lookback_model.cpp"), "-o", str(out)], check=True)
     L = ctypes.CDLL(str(out))
+    L.lb_set_parts.argtypes = [ctypes.c_int]
     L.lb_run.restype = ctypes.c_int
     L.lb_run.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                          ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
@@ -63,8 +64,15 @@ def test_grid_of_one(lib):
 
 
 @pytest.mark.parametrize("grid", [2, 3, 5, 8, 9, 13])
-def test_small_grids_take_fewer_classes(lib, grid):
-    """Grids below 8 blocks split the tiles into 2 or 4 classes (every class
-    has a block); grids of 8 or more into 8, some classes with more blocks."""
-    for spec_frac in (0.0, 0.6):
-        assert run(lib, 1500, 1500, grid, spec_frac, False, 31 + grid) == 0, (grid, spec_frac)
+def test_ticket_classes(lib, grid):
+    """The kernel claims from one ticket (kTicketParts = 1).  With up to 8
+    classes, grids below 8 blocks split the tiles into 2 or 4 classes (every
+    class has a block) and grids of 8 or more into 8; with every block running,
+    as here, every launch ends."""
+    try:
+        for parts in (1, 8):
+            lib.lb_set_parts(parts)
+            for spec_frac in (0.0, 0.6):
+                assert run(lib, 1500, 1500, grid, spec_frac, False, 31 + grid) == 0, (grid, parts, spec_frac)
+    finally:
+        lib.lb_set_parts(1)
