@@ -707,11 +707,17 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
   // tile is always one a block is working on or its class's next (its class's
   // blocks have finished every earlier tile of the class, so one of them claims
   // it next): the lookback still cannot wait forever.
+  // The 2-level variant takes two consecutive tiles of its class per atomic
+  // (half the claims on the counter: 1.65 -> 1.59 ms on 4 GiB at L = 1, 2);
+  // the deeper variants, whose tiles take longer, measured slower with pairs.
+  // A block still holds its tiles in increasing order.
+  constexpr unsigned kClaim = NLV <= 2 ? 2u : 1u;
   unsigned npart = 1;
   while (npart * 2 <= (unsigned)kTicketParts && npart * 2 <= gridDim.x) npart *= 2;
   const unsigned part = blockIdx.x & (npart - 1);
   unsigned int* const my_ticket = launch.ticket + part * kTicketStride;
   unsigned next_ticket = 0;            // thread 0 only
+  unsigned in_claim = 0;               // thread 0 only: tiles of the current claim handed out
   int cur_job = -1;                    // thread 0 only: the job of the last tile prepared
   int64_t cur_end = 0;                 // thread 0 only: its first tile past the job
   int slot_job[2] = {-1, -1};          // thread 0 only: the job each s_J set holds
@@ -725,7 +731,12 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     q.lnl = -1;
     q.ncand = 0;
     if (t >= launch.total_tiles) return;
-    next_ticket = atomicAdd(my_ticket, 1u) * npart + part;
+    if (++in_claim < kClaim) {
+      next_ticket += npart;
+    } else {
+      in_claim = 0;
+      next_ticket = atomicAdd(my_ticket, kClaim) * npart + part;
+    }
     int lo = 0;
     if (launch.jobs) {
       if (cur_job >= 0 && (int64_t)t < cur_end) {
@@ -748,7 +759,7 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
     }
   };
   if (tid == 0) {
-    next_ticket = atomicAdd(my_ticket, 1u) * npart + part;
+    next_ticket = atomicAdd(my_ticket, kClaim) * npart + part;
     prepare(0);
   }
   __syncthreads();

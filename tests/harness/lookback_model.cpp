@@ -185,19 +185,26 @@ void publish_inc(Model& m, int64_t lt, uint64_t v) {
 // starts, so a block always holds one claimed tile it has not started, and
 // block b of a grid of `grid` claims the tiles of class b % npart.
 unsigned g_max_parts = 1;
+unsigned g_claim = 1;  // consecutive tiles of the class per atomic
 
 void block_main(Model& m, int64_t begin, int64_t end, int b, int grid, uint64_t seed) {
   std::mt19937_64 rng(seed);
   unsigned npart = 1;
   while (npart * 2 <= g_max_parts && npart * 2 <= (unsigned)grid) npart *= 2;
   const unsigned part = (unsigned)b & (npart - 1);
-  auto claim = [&] { return begin + (int64_t)m.ticket[part].fetch_add(1) * npart + part; };
+  unsigned in_claim = 0;
+  auto claim = [&] { return begin + (int64_t)m.ticket[part].fetch_add(g_claim) * npart + part; };
   int64_t next = claim();
   for (;;) {
     if (m.abort.load()) return;
     const int64_t lt = next;
     if (lt >= end) return;
-    next = claim();
+    if (++in_claim < g_claim) {
+      next += npart;
+    } else {
+      in_claim = 0;
+      next = claim();
+    }
     maybe_delay(m, rng);
     const Agg own = m.tile_agg[lt];
     // every tile publishes its aggregate first
@@ -234,6 +241,8 @@ void block_main(Model& m, int64_t begin, int64_t end, int b, int grid, uint64_t 
 extern "C" {
 // Largest number of ticket classes (a power of two, at most 8).
 void lb_set_parts(int parts) { g_max_parts = (unsigned)parts; }
+// Consecutive tiles of a class per claim (the kernel: 2 in its 2-level variant).
+void lb_set_claim(int claim) { g_claim = (unsigned)claim; }
 
 // Runs the launches [bounds[k], bounds[k+1]) one after another (the tickets
 // reset before each, states kept), each on `grid` threads.  spec: tiles that

@@ -26,6 +26,7 @@ def lib(tmp_path_factory):
                     os.path.join(ROOT, "tests", "harness", "lookback_model.cpp"), "-o", str(out)], check=True)
     L = ctypes.CDLL(str(out))
     L.lb_set_parts.argtypes = [ctypes.c_int]
+    L.lb_set_claim.argtypes = [ctypes.c_int]
     L.lb_run.restype = ctypes.c_int
     L.lb_run.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                          ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
@@ -76,3 +77,16 @@ def test_ticket_classes(lib, grid):
                 assert run(lib, 1500, 1500, grid, spec_frac, False, 31 + grid) == 0, (grid, parts, spec_frac)
     finally:
         lib.lb_set_parts(1)
+
+
+@pytest.mark.parametrize("grid", [1, 3, 32])
+def test_two_tiles_per_claim(lib, grid):
+    """The 2-level variant claims two consecutive tiles per atomic: one launch
+    and the staging pipeline's consecutive launches, speculating and not."""
+    try:
+        lib.lb_set_claim(2)
+        for spec_frac in (0.0, 0.7, 0.97):
+            assert run(lib, 3000, 3000, grid, spec_frac, False, 41 + grid) == 0, (grid, spec_frac)
+            assert run(lib, 3000, 63, grid, spec_frac, False, 43 + grid) == 0, (grid, spec_frac)
+    finally:
+        lib.lb_set_claim(1)
