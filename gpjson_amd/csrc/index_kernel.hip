@@ -708,10 +708,12 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
   // blocks have finished every earlier tile of the class, so one of them claims
   // it next): the lookback still cannot wait forever.
   // The 2-level variant takes two consecutive tiles of its class per atomic
-  // (half the claims on the counter: 1.65 -> 1.59 ms on 4 GiB at L = 1, 2);
-  // the deeper variants, whose tiles take longer, measured slower with pairs.
-  // A block still holds its tiles in increasing order.
-  constexpr unsigned kClaim = NLV <= 2 ? 2u : 1u;
+  // when every block gets 16 tiles or more (half the claims on the counter:
+  // 1.66 -> 1.57-1.62 ms on 4 GiB at L = 1, 2; on fewer tiles pairs leave
+  // blocks idle at the end, and the deeper variants measured slower with
+  // pairs).  A block still holds its tiles in increasing order.
+  const unsigned kClaim =
+      NLV <= 2 && launch.total_tiles - launch.tile_begin >= 16 * (int64_t)gridDim.x ? 2u : 1u;
   unsigned npart = 1;
   while (npart * 2 <= (unsigned)kTicketParts && npart * 2 <= gridDim.x) npart *= 2;
   const unsigned part = blockIdx.x & (npart - 1);
