@@ -708,12 +708,14 @@ __global__ __launch_bounds__(BLOCK, (BLOCK == 256 && ROWS == 3 && !VERIFY)   ? 3
   // blocks have finished every earlier tile of the class, so one of them claims
   // it next): the lookback still cannot wait forever.
   // The 2-level variant takes two consecutive tiles of its class per atomic
-  // when every block gets 16 tiles or more (half the claims on the counter:
-  // 1.66 -> 1.57-1.62 ms on 4 GiB at L = 1, 2; on fewer tiles pairs leave
-  // blocks idle at the end, and the deeper variants measured slower with
-  // pairs).  A block still holds its tiles in increasing order.
+  // when every block gets 64 tiles or more (half the claims on the counter:
+  // 1.66 -> 1.57-1.62 ms on 4 GiB of tweets at L = 1, 2).  On fewer tiles
+  // pairs leave blocks idle at the end (1 GiB of bracket-dense lines, 32 tiles
+  // per block: 0.61 -> 0.68 ms; one-query passes over a mix of 1-512 MiB
+  // files: 377 -> 399 ms), and the deeper variants measured slower with
+  // pairs.  A block still holds its tiles in increasing order.
   const unsigned kClaim =
-      NLV <= 2 && launch.total_tiles - launch.tile_begin >= 16 * (int64_t)gridDim.x ? 2u : 1u;
+      NLV <= 2 && launch.total_tiles - launch.tile_begin >= 64 * (int64_t)gridDim.x ? 2u : 1u;
   unsigned npart = 1;
   while (npart * 2 <= (unsigned)kTicketParts && npart * 2 <= gridDim.x) npart *= 2;
   const unsigned part = blockIdx.x & (npart - 1);
